@@ -1,0 +1,1002 @@
+// cbz_api.hip — the C ABI (include/cubez_amd.h): context, device state,
+// scene tables and the phase launches.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "cbz_launch.h"
+#include "cbz_types.h"
+#include "cubez_amd.h"
+
+using namespace cbz;
+
+namespace {
+
+struct FieldInfo {
+  int width;
+  int kind;  // 0 = real, 1 = int32, 2 = uint8, 3 = host-only real
+};
+
+const FieldInfo kFields[CBZ_NUM_FIELDS] = {
+    {1, 1},  {3, 0}, {3, 0}, {4, 0}, {3, 0},  {3, 0}, {3, 0}, {1, 0}, {9, 0},
+    {9, 0},  {12, 0}, {12, 0}, {12, 0}, {3, 0}, {1, 0}, {1, 3}, {1, 3}, {1, 2},
+};
+
+constexpr uint8_t kUserFlags = CBZ_FLAG_AWAKE | CBZ_FLAG_CAN_SLEEP | CBZ_FLAG_STATIC;
+
+}  // namespace
+
+struct cbz_context {
+  int precision = CBZ_F64;
+  int device = 0;
+  int64_t cap_cfg = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // scene
+  int32_t nw = 0;
+  int64_t nb = 0;
+  int32_t np = 0;
+  std::vector<int32_t> body_off, plane_off;
+  std::vector<double> lin_damp, ang_damp;
+  std::vector<int32_t> h_shape;
+  std::vector<double> h_half;
+  std::vector<uint8_t> h_offset_id;
+  bool brad_dirty = true;
+  bool pow_dirty = true;
+  double pow_dt = std::numeric_limits<double>::quiet_NaN();
+  double sleep_bias = 1.0;
+
+  // device state
+  void* field[CBZ_NUM_FIELDS] = {};
+  void* pow_lin = nullptr;
+  void* pow_ang = nullptr;
+  double* brad = nullptr;
+  void* plane_normal = nullptr;
+  void* plane_offset = nullptr;
+  int32_t* d_body_off = nullptr;
+  int32_t* d_plane_off = nullptr;
+
+  // all-pairs pair space
+  int32_t n_tiles = 0;
+  int32_t* d_tile_world = nullptr;
+  int32_t* d_world_tile0 = nullptr;
+  int32_t* d_world_q = nullptr;
+  // explicit-list pair space (grown on demand)
+  int32_t l_tiles_cap = 0, l_pairs_cap = 0;
+  int32_t* l_tile_world = nullptr;
+  int32_t* l_world_tile0 = nullptr;
+  int32_t* l_world_q = nullptr;
+  int32_t* l_off = nullptr;
+  int32_t* l_one = nullptr;
+  int32_t* l_two = nullptr;
+  // tile scratch, sized for the larger pair space
+  int32_t tiles_cap = 0;
+  int32_t* tile_cnt = nullptr;
+  int32_t* tile_off = nullptr;
+  uint64_t* tile_mask = nullptr;
+
+  // contacts
+  int64_t cap = 0;
+  void* c_body0 = nullptr;
+  void* c_body1 = nullptr;
+  void* c_real = nullptr;  // point 3, normal 3, pen 1, c2w 9, rel 6, cvel 3, ddv 1  (26 planes)
+  int32_t* world_cnt = nullptr;
+  int64_t* world_coff = nullptr;
+  int64_t* total = nullptr;
+  int32_t* overflow = nullptr;
+  int64_t* world_iters = nullptr;
+  int32_t* order = nullptr;
+  bool have_contacts = false;
+
+  // profiling
+  bool prof = false;
+  hipEvent_t ev[4] = {};
+  double acc_ms[4] = {0, 0, 0, 0};
+  int64_t acc_steps = 0;
+};
+
+namespace {
+
+int fail(cbz_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, x)                                                                            \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess) return fail((ctx), CBZ_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t real_size(const cbz_ctx* c) { return c->precision == CBZ_F64 ? sizeof(double) : sizeof(float); }
+
+template <typename T>
+int dalloc(cbz_ctx* c, T** p, size_t count) {
+  if (*p) {
+    HIPCHK(c, hipFree(*p));
+    *p = nullptr;
+  }
+  if (count == 0) count = 1;
+  HIPCHK(c, hipMalloc((void**)p, count * sizeof(T)));
+  return CBZ_OK;
+}
+
+int dalloc_bytes(cbz_ctx* c, void** p, size_t bytes) {
+  if (*p) {
+    HIPCHK(c, hipFree(*p));
+    *p = nullptr;
+  }
+  if (bytes == 0) bytes = 8;
+  HIPCHK(c, hipMalloc(p, bytes));
+  return CBZ_OK;
+}
+
+void dfree(void*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+// Host [n][k] doubles -> device [k][n] of the context's real type.
+int upload_real(cbz_ctx* c, void* dst, const double* src, int64_t n, int k) {
+  if (n == 0) return CBZ_OK;
+  if (c->precision == CBZ_F64) {
+    std::vector<double> t((size_t)n * k);
+    for (int64_t i = 0; i < n; ++i)
+      for (int j = 0; j < k; ++j) t[(size_t)j * n + i] = src[(size_t)i * k + j];
+    HIPCHK(c, hipMemcpy(dst, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> t((size_t)n * k);
+    for (int64_t i = 0; i < n; ++i)
+      for (int j = 0; j < k; ++j) t[(size_t)j * n + i] = (float)src[(size_t)i * k + j];
+    HIPCHK(c, hipMemcpy(dst, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return CBZ_OK;
+}
+
+// Device [k][stride] (from element `first`, n elements) -> host [n][k] doubles.
+int download_real(cbz_ctx* c, const void* srcbase, int64_t stride, int64_t first, int64_t n, int k, double* dst) {
+  if (n == 0) return CBZ_OK;
+  if (c->precision == CBZ_F64) {
+    std::vector<double> t((size_t)n);
+    for (int j = 0; j < k; ++j) {
+      const double* s = (const double*)srcbase + (size_t)j * stride + first;
+      HIPCHK(c, hipMemcpy(t.data(), s, n * sizeof(double), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n; ++i) dst[(size_t)i * k + j] = t[i];
+    }
+  } else {
+    std::vector<float> t((size_t)n);
+    for (int j = 0; j < k; ++j) {
+      const float* s = (const float*)srcbase + (size_t)j * stride + first;
+      HIPCHK(c, hipMemcpy(t.data(), s, n * sizeof(float), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n; ++i) dst[(size_t)i * k + j] = (double)t[i];
+    }
+  }
+  return CBZ_OK;
+}
+
+void free_scene(cbz_ctx* c) {
+  for (auto& f : c->field) dfree(f);
+  dfree(c->pow_lin);
+  dfree(c->pow_ang);
+  dfree(c->brad);
+  dfree(c->plane_normal);
+  dfree(c->plane_offset);
+  dfree(c->d_body_off);
+  dfree(c->d_plane_off);
+  dfree(c->d_tile_world);
+  dfree(c->d_world_tile0);
+  dfree(c->d_world_q);
+  dfree(c->l_tile_world);
+  dfree(c->l_world_tile0);
+  dfree(c->l_world_q);
+  dfree(c->l_off);
+  dfree(c->l_one);
+  dfree(c->l_two);
+  dfree(c->tile_cnt);
+  dfree(c->tile_off);
+  dfree(c->tile_mask);
+  dfree(c->c_body0);
+  dfree(c->c_body1);
+  dfree(c->c_real);
+  dfree(c->world_cnt);
+  dfree(c->world_coff);
+  dfree(c->total);
+  dfree(c->overflow);
+  dfree(c->world_iters);
+  dfree(c->order);
+  c->l_tiles_cap = c->l_pairs_cap = c->tiles_cap = 0;
+  c->have_contacts = false;
+}
+
+template <typename R>
+Bodies<R> bodies_view(cbz_ctx* c) {
+  Bodies<R> b;
+  b.n = (int32_t)c->nb;
+  b.stride = c->nb;
+  b.shape = (const int32_t*)c->field[CBZ_SHAPE];
+  b.half = (const R*)c->field[CBZ_HALF];
+  b.pos = (R*)c->field[CBZ_POS];
+  b.quat = (R*)c->field[CBZ_QUAT];
+  b.vel = (R*)c->field[CBZ_VEL];
+  b.rot = (R*)c->field[CBZ_ROT];
+  b.acc = (const R*)c->field[CBZ_ACC];
+  b.inv_mass = (const R*)c->field[CBZ_INV_MASS];
+  b.iit_body = (const R*)c->field[CBZ_IIT_BODY];
+  b.iit_world = (R*)c->field[CBZ_IIT_WORLD];
+  b.transform = (R*)c->field[CBZ_TRANSFORM];
+  b.ctransform = (R*)c->field[CBZ_CTRANSFORM];
+  b.offset = (const R*)c->field[CBZ_OFFSET];
+  b.last_acc = (R*)c->field[CBZ_LAST_ACC];
+  b.motion = (R*)c->field[CBZ_MOTION];
+  b.pow_lin = (const R*)c->pow_lin;
+  b.pow_ang = (const R*)c->pow_ang;
+  b.flags = (uint8_t*)c->field[CBZ_FLAGS];
+  return b;
+}
+
+template <typename R>
+Contacts<R> contacts_view(cbz_ctx* c) {
+  Contacts<R> v;
+  const int64_t S = c->cap;
+  R* base = (R*)c->c_real;
+  v.cap = S;
+  v.body0 = (int32_t*)c->c_body0;
+  v.body1 = (int32_t*)c->c_body1;
+  v.point = base;
+  v.normal = base + 3 * S;
+  v.pen = base + 6 * S;
+  v.c2w = base + 7 * S;
+  v.rel = base + 16 * S;
+  v.cvel = base + 22 * S;
+  v.ddv = base + 25 * S;
+  return v;
+}
+
+int ensure_tiles(cbz_ctx* c, int32_t n) {
+  if (n <= c->tiles_cap) return CBZ_OK;
+  int rc;
+  if ((rc = dalloc(c, &c->tile_cnt, (size_t)n))) return rc;
+  if ((rc = dalloc(c, &c->tile_off, (size_t)n))) return rc;
+  if ((rc = dalloc(c, &c->tile_mask, (size_t)n))) return rc;
+  c->tiles_cap = n;
+  return CBZ_OK;
+}
+
+int need_scene(cbz_ctx* c) {
+  if (!c) return CBZ_ERR_ARG;
+  if (c->nw <= 0) return fail(c, CBZ_ERR_STATE, "no scene: call cbz_set_scene first");
+  return CBZ_OK;
+}
+
+// Conservative bounding radii for the narrowphase's far-pair rejection.
+int refresh_brad(cbz_ctx* c) {
+  if (!c->brad_dirty) return CBZ_OK;
+  std::vector<double> r((size_t)c->nb);
+  for (int64_t i = 0; i < c->nb; ++i) {
+    const double* h = &c->h_half[(size_t)i * 3];
+    double v = c->h_shape[i] == CBZ_SPHERE ? std::fabs(h[0])
+                                           : std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    r[i] = v * (1.0 + 1e-9);
+    if (!std::isfinite(r[i])) r[i] = std::numeric_limits<double>::infinity();
+  }
+  if (c->nb > 0) HIPCHK(c, hipMemcpy(c->brad, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->brad_dirty = false;
+  return CBZ_OK;
+}
+
+// Pow(LinearDamping, dt), Pow(AngularDamping, dt) per body and Pow(0.5, dt)
+// (rigidbody.go:233-234, 250), evaluated with the C library.
+int refresh_pow(cbz_ctx* c, double dt) {
+  if (!c->pow_dirty && dt == c->pow_dt) return CBZ_OK;
+  std::vector<double> pl((size_t)c->nb), pa((size_t)c->nb);
+  double last_l = std::numeric_limits<double>::quiet_NaN(), last_a = last_l, vl = 0, va = 0;
+  for (int64_t i = 0; i < c->nb; ++i) {
+    if (!(c->lin_damp[i] == last_l)) {
+      last_l = c->lin_damp[i];
+      vl = std::pow(last_l, dt);
+    }
+    if (!(c->ang_damp[i] == last_a)) {
+      last_a = c->ang_damp[i];
+      va = std::pow(last_a, dt);
+    }
+    pl[i] = vl;
+    pa[i] = va;
+  }
+  int rc;
+  if ((rc = upload_real(c, c->pow_lin, pl.data(), c->nb, 1))) return rc;
+  if ((rc = upload_real(c, c->pow_ang, pa.data(), c->nb, 1))) return rc;
+  c->sleep_bias = std::pow(0.5, dt);
+  c->pow_dt = dt;
+  c->pow_dirty = false;
+  return CBZ_OK;
+}
+
+
+template <typename R>
+int do_integrate(cbz_ctx* c, double dt) {
+  int rc = refresh_pow(c, dt);
+  if (rc) return rc;
+  HIPCHK(c, launch_integrate<R>(bodies_view<R>(c), (R)dt, (R)c->sleep_bias, c->stream));
+  return CBZ_OK;
+}
+
+// list_tiles < 0 selects the all-pairs space.
+template <typename R>
+int do_narrowphase(cbz_ctx* c, int32_t list_tiles) {
+  int rc = refresh_brad(c);
+  if (rc) return rc;
+  NarrowLaunch<R> L;
+  L.b = bodies_view<R>(c);
+  L.pl.normal = (const R*)c->plane_normal;
+  L.pl.offset = (const R*)c->plane_offset;
+  L.pl.stride = c->np;
+  L.ps.body_off = c->d_body_off;
+  L.ps.plane_off = c->d_plane_off;
+  if (list_tiles < 0) {
+    L.ps.n_tiles = c->n_tiles;
+    L.ps.tile_world = c->d_tile_world;
+    L.ps.world_tile0 = c->d_world_tile0;
+    L.ps.world_q = c->d_world_q;
+    L.ps.list_off = nullptr;
+    L.ps.list_one = nullptr;
+    L.ps.list_two = nullptr;
+  } else {
+    L.ps.n_tiles = list_tiles;
+    L.ps.tile_world = c->l_tile_world;
+    L.ps.world_tile0 = c->l_world_tile0;
+    L.ps.world_q = c->l_world_q;
+    L.ps.list_off = c->l_off;
+    L.ps.list_one = c->l_one;
+    L.ps.list_two = c->l_two;
+  }
+  L.c = contacts_view<R>(c);
+  L.brad = c->brad;
+  L.tile_cnt = c->tile_cnt;
+  L.tile_mask = c->tile_mask;
+  L.tile_off = c->tile_off;
+  L.world_cnt = c->world_cnt;
+  L.world_coff = c->world_coff;
+  L.total = c->total;
+  L.overflow = c->overflow;
+  L.n_worlds = c->nw;
+  HIPCHK(c, launch_fill_i32(c->overflow, 1, 0, c->stream));
+  HIPCHK(c, launch_narrowphase<R>(L, c->stream));
+  c->have_contacts = true;
+  return CBZ_OK;
+}
+
+template <typename R>
+int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed) {
+  ResolveLaunch<R> L;
+  L.b = bodies_view<R>(c);
+  L.c = contacts_view<R>(c);
+  L.world_cnt = c->world_cnt;
+  L.world_coff = c->world_coff;
+  L.overflow = c->overflow;
+  L.order = c->order;
+  L.world_iters = c->world_iters;
+  L.n_worlds = c->nw;
+  L.dt = (R)dt;
+  L.iter_mult = mult;
+  L.iter_fixed = fixed;
+  HIPCHK(c, launch_resolve<R>(L, c->stream));
+  return CBZ_OK;
+}
+
+int check_overflow(cbz_ctx* c) {
+  int32_t of = 0;
+  int64_t tot = 0;
+  HIPCHK(c, hipMemcpyAsync(&of, c->overflow, sizeof of, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&tot, c->total, sizeof tot, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (of)
+    return fail(c, CBZ_ERR_CAPACITY,
+                "contact capacity exceeded: " + std::to_string(tot) + " contacts for " + std::to_string(c->cap) +
+                    " slots (raise cbz_config.contact_capacity)");
+  return CBZ_OK;
+}
+
+// World order for the resolver: larger worlds first, so the longest
+// serial chains start earliest.
+void build_order(cbz_ctx* c, std::vector<int32_t>& ord) {
+  ord.resize((size_t)c->nw);
+  for (int32_t w = 0; w < c->nw; ++w) ord[w] = w;
+  std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+    return (c->body_off[a + 1] - c->body_off[a]) > (c->body_off[b + 1] - c->body_off[b]);
+  });
+}
+
+}  // namespace
+
+// ======================================================================
+// C ABI
+// ======================================================================
+extern "C" {
+
+int cbz_field_width(int32_t field) {
+  if (field < 0 || field >= CBZ_NUM_FIELDS) return -1;
+  return kFields[field].width;
+}
+
+int cbz_create(const cbz_config* cfg, cbz_ctx** out) {
+  if (!out) return CBZ_ERR_ARG;
+  *out = nullptr;
+  cbz_ctx* c = new cbz_ctx();
+  if (cfg) {
+    if (cfg->precision != CBZ_F64 && cfg->precision != CBZ_F32) {
+      delete c;
+      return CBZ_ERR_ARG;
+    }
+    c->precision = cfg->precision;
+    c->device = cfg->device;
+    c->cap_cfg = cfg->contact_capacity;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || c->device < 0 || c->device >= ndev) {
+    delete c;
+    return CBZ_ERR_HIP;
+  }
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return CBZ_ERR_HIP;
+  }
+  for (auto& e : c->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete c;
+      return CBZ_ERR_HIP;
+    }
+  }
+  *out = c;
+  return CBZ_OK;
+}
+
+int cbz_destroy(cbz_ctx* c) {
+  if (!c) return CBZ_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_scene(c);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CBZ_OK;
+}
+
+const char* cbz_last_error(const cbz_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int32_t cbz_num_worlds(const cbz_ctx* c) { return c ? c->nw : 0; }
+int64_t cbz_num_bodies(const cbz_ctx* c) { return c ? c->nb : 0; }
+
+int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const int32_t* plane_off,
+                  const double* plane_normal, const double* plane_offset) {
+  if (!c) return CBZ_ERR_ARG;
+  if (n_worlds <= 0 || !body_off || !plane_off) return fail(c, CBZ_ERR_ARG, "set_scene: bad arguments");
+  if (body_off[0] != 0 || plane_off[0] != 0) return fail(c, CBZ_ERR_ARG, "set_scene: offsets must start at 0");
+  for (int32_t w = 0; w < n_worlds; ++w) {
+    if (body_off[w + 1] < body_off[w] || plane_off[w + 1] < plane_off[w])
+      return fail(c, CBZ_ERR_ARG, "set_scene: offsets must be non-decreasing");
+    const int64_t n = body_off[w + 1] - body_off[w], np = plane_off[w + 1] - plane_off[w];
+    if (n * (np + n) > (int64_t)INT32_MAX - 64)
+      return fail(c, CBZ_ERR_ARG, "set_scene: world " + std::to_string(w) + " has too many ordered pairs");
+  }
+  const int32_t np = plane_off[n_worlds];
+  if (np > 0 && (!plane_normal || !plane_offset)) return fail(c, CBZ_ERR_ARG, "set_scene: planes missing");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  free_scene(c);
+  c->nw = n_worlds;
+  c->nb = body_off[n_worlds];
+  c->np = np;
+  c->body_off.assign(body_off, body_off + n_worlds + 1);
+  c->plane_off.assign(plane_off, plane_off + n_worlds + 1);
+  const int64_t nb = c->nb;
+  const size_t rs = real_size(c);
+  int rc;
+  for (int f = 0; f < CBZ_NUM_FIELDS; ++f) {
+    const FieldInfo& fi = kFields[f];
+    if (fi.kind == 3) continue;
+    size_t el = fi.kind == 0 ? rs : fi.kind == 1 ? sizeof(int32_t) : sizeof(uint8_t);
+    if ((rc = dalloc_bytes(c, &c->field[f], el * fi.width * (size_t)std::max<int64_t>(nb, 1)))) return rc;
+  }
+  if ((rc = dalloc_bytes(c, &c->pow_lin, rs * (size_t)std::max<int64_t>(nb, 1)))) return rc;
+  if ((rc = dalloc_bytes(c, &c->pow_ang, rs * (size_t)std::max<int64_t>(nb, 1)))) return rc;
+  if ((rc = dalloc(c, &c->brad, (size_t)nb))) return rc;
+  if ((rc = dalloc_bytes(c, &c->plane_normal, rs * 3 * (size_t)std::max(np, 1)))) return rc;
+  if ((rc = dalloc_bytes(c, &c->plane_offset, rs * (size_t)std::max(np, 1)))) return rc;
+  if (np > 0) {
+    if ((rc = upload_real(c, c->plane_normal, plane_normal, np, 3))) return rc;
+    if ((rc = upload_real(c, c->plane_offset, plane_offset, np, 1))) return rc;
+  }
+  if ((rc = dalloc(c, &c->d_body_off, (size_t)n_worlds + 1))) return rc;
+  if ((rc = dalloc(c, &c->d_plane_off, (size_t)n_worlds + 1))) return rc;
+  HIPCHK(c, hipMemcpy(c->d_body_off, body_off, (n_worlds + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_plane_off, plane_off, (n_worlds + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+
+  // all-pairs tiles
+  std::vector<int32_t> wq(n_worlds), wt0(n_worlds + 1), tw;
+  int64_t nt = 0;
+  for (int32_t w = 0; w < n_worlds; ++w) {
+    const int64_t n = body_off[w + 1] - body_off[w], npw = plane_off[w + 1] - plane_off[w];
+    wq[w] = (int32_t)(n * (npw + n));
+    wt0[w] = (int32_t)nt;
+    nt += (wq[w] + 63) / 64;
+    if (nt > INT32_MAX / 2) return fail(c, CBZ_ERR_ARG, "set_scene: pair space too large");
+  }
+  wt0[n_worlds] = (int32_t)nt;
+  tw.resize((size_t)nt);
+  for (int32_t w = 0; w < n_worlds; ++w)
+    for (int32_t t = wt0[w]; t < wt0[w + 1]; ++t) tw[t] = w;
+  c->n_tiles = (int32_t)nt;
+  if ((rc = dalloc(c, &c->d_tile_world, (size_t)nt))) return rc;
+  if ((rc = dalloc(c, &c->d_world_tile0, (size_t)n_worlds + 1))) return rc;
+  if ((rc = dalloc(c, &c->d_world_q, (size_t)n_worlds))) return rc;
+  if (nt > 0) HIPCHK(c, hipMemcpy(c->d_tile_world, tw.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_world_tile0, wt0.data(), wt0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_world_q, wq.data(), wq.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  if ((rc = ensure_tiles(c, (int32_t)std::max<int64_t>(nt, 1)))) return rc;
+
+  // contacts
+  c->cap = c->cap_cfg > 0 ? c->cap_cfg : 16 * nb + 4096;
+  if ((rc = dalloc_bytes(c, &c->c_body0, sizeof(int32_t) * (size_t)c->cap))) return rc;
+  if ((rc = dalloc_bytes(c, &c->c_body1, sizeof(int32_t) * (size_t)c->cap))) return rc;
+  if ((rc = dalloc_bytes(c, &c->c_real, rs * 26 * (size_t)c->cap))) return rc;
+  if ((rc = dalloc(c, &c->world_cnt, (size_t)n_worlds))) return rc;
+  if ((rc = dalloc(c, &c->world_coff, (size_t)n_worlds))) return rc;
+  if ((rc = dalloc(c, &c->total, 1))) return rc;
+  if ((rc = dalloc(c, &c->overflow, 1))) return rc;
+  if ((rc = dalloc(c, &c->world_iters, 2 * (size_t)n_worlds))) return rc;
+  if ((rc = dalloc(c, &c->order, (size_t)n_worlds))) return rc;
+  HIPCHK(c, hipMemset(c->world_cnt, 0, n_worlds * sizeof(int32_t)));
+  HIPCHK(c, hipMemset(c->world_coff, 0, n_worlds * sizeof(int64_t)));
+  HIPCHK(c, hipMemset(c->world_iters, 0, 2 * n_worlds * sizeof(int64_t)));
+  HIPCHK(c, hipMemset(c->overflow, 0, sizeof(int32_t)));
+  HIPCHK(c, hipMemset(c->total, 0, sizeof(int64_t)));
+  std::vector<int32_t> ord;
+  build_order(c, ord);
+  HIPCHK(c, hipMemcpy(c->order, ord.data(), ord.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+
+  // NewRigidBody (rigidbody.go:104-114) inside NewCollisionCube(nil, 0)
+  // (colliders.go:265-274).
+  c->lin_damp.assign((size_t)nb, 0.95);
+  c->ang_damp.assign((size_t)nb, 0.95);  // AngularDamping = defaultLinearDamping, rigidbody.go:108
+  c->h_shape.assign((size_t)nb, CBZ_CUBE);
+  c->h_half.assign((size_t)nb * 3, 0.0);
+  c->h_offset_id.assign((size_t)nb, 1);
+  std::vector<double> z((size_t)nb * 12, 0.0);
+  const int zero_fields[] = {CBZ_HALF, CBZ_POS, CBZ_VEL, CBZ_ROT, CBZ_INV_MASS, CBZ_IIT_BODY, CBZ_TRANSFORM,
+                             CBZ_CTRANSFORM, CBZ_LAST_ACC};
+  for (int f : zero_fields)
+    if ((rc = upload_real(c, c->field[f], z.data(), nb, kFields[f].width))) return rc;
+  std::vector<double> t((size_t)nb * 12);
+  for (int64_t i = 0; i < nb; ++i) {
+    double* q = &t[(size_t)i * 4];
+    q[0] = 1.0;
+    q[1] = q[2] = q[3] = 0.0;
+  }
+  if ((rc = upload_real(c, c->field[CBZ_QUAT], t.data(), nb, 4))) return rc;
+  for (int64_t i = 0; i < nb; ++i) {
+    double* a = &t[(size_t)i * 3];
+    a[0] = 0.0;
+    a[1] = -9.78;  // defaultAcceleration, rigidbody.go:18
+    a[2] = 0.0;
+  }
+  if ((rc = upload_real(c, c->field[CBZ_ACC], t.data(), nb, 3))) return rc;
+  for (int64_t i = 0; i < nb; ++i) {
+    double* m = &t[(size_t)i * 9];
+    for (int k = 0; k < 9; ++k) m[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  }
+  if ((rc = upload_real(c, c->field[CBZ_IIT_WORLD], t.data(), nb, 9))) return rc;
+  for (int64_t i = 0; i < nb; ++i) {
+    double* m = &t[(size_t)i * 12];
+    for (int k = 0; k < 12; ++k) m[k] = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+  }
+  if ((rc = upload_real(c, c->field[CBZ_OFFSET], t.data(), nb, 12))) return rc;
+  for (int64_t i = 0; i < nb; ++i) t[i] = 0.6;  // SetAwake(true): motion = 2*sleepEpsilon
+  if ((rc = upload_real(c, c->field[CBZ_MOTION], t.data(), nb, 1))) return rc;
+  std::vector<int32_t> sh((size_t)nb, CBZ_CUBE);
+  if (nb > 0) HIPCHK(c, hipMemcpy(c->field[CBZ_SHAPE], sh.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice));
+  std::vector<uint8_t> fl((size_t)nb, CBZ_FLAG_AWAKE | CBZ_FLAG_CAN_SLEEP | F_OFFSET_ID);
+  if (nb > 0) HIPCHK(c, hipMemcpy(c->field[CBZ_FLAGS], fl.data(), nb, hipMemcpyHostToDevice));
+  c->brad_dirty = true;
+  c->pow_dirty = true;
+  c->have_contacts = false;
+  return CBZ_OK;
+}
+
+int cbz_set_field(cbz_ctx* c, int32_t field, const void* host) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (field < 0 || field >= CBZ_NUM_FIELDS || !host) return fail(c, CBZ_ERR_ARG, "set_field: bad field");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const FieldInfo& fi = kFields[field];
+  const int64_t nb = c->nb;
+  if (field == CBZ_LIN_DAMP || field == CBZ_ANG_DAMP) {
+    auto& dst = field == CBZ_LIN_DAMP ? c->lin_damp : c->ang_damp;
+    std::memcpy(dst.data(), host, nb * sizeof(double));
+    c->pow_dirty = true;
+    return CBZ_OK;
+  }
+  if (field == CBZ_SHAPE) {
+    const int32_t* s = (const int32_t*)host;
+    for (int64_t i = 0; i < nb; ++i)
+      if (s[i] != CBZ_CUBE && s[i] != CBZ_SPHERE) return fail(c, CBZ_ERR_ARG, "set_field: unknown shape");
+    std::memcpy(c->h_shape.data(), s, nb * sizeof(int32_t));
+    if (nb > 0) HIPCHK(c, hipMemcpy(c->field[field], s, nb * sizeof(int32_t), hipMemcpyHostToDevice));
+    c->brad_dirty = true;
+    return CBZ_OK;
+  }
+  if (field == CBZ_FLAGS) {
+    std::vector<uint8_t> fl((size_t)nb);
+    const uint8_t* s = (const uint8_t*)host;
+    for (int64_t i = 0; i < nb; ++i) fl[i] = uint8_t((s[i] & kUserFlags) | (c->h_offset_id[i] ? F_OFFSET_ID : 0));
+    if (nb > 0) HIPCHK(c, hipMemcpy(c->field[field], fl.data(), nb, hipMemcpyHostToDevice));
+    return CBZ_OK;
+  }
+  const double* d = (const double*)host;
+  if ((rc = upload_real(c, c->field[field], d, nb, fi.width))) return rc;
+  if (field == CBZ_HALF) {
+    std::memcpy(c->h_half.data(), d, nb * 3 * sizeof(double));
+    c->brad_dirty = true;
+  }
+  if (field == CBZ_OFFSET) {
+    static const double id[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    std::vector<uint8_t> fl((size_t)nb);
+    if (nb > 0) HIPCHK(c, hipMemcpy(fl.data(), c->field[CBZ_FLAGS], nb, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nb; ++i) {
+      bool same = std::memcmp(d + 12 * i, id, sizeof id) == 0;  // bitwise: +0.0 entries
+      c->h_offset_id[i] = same ? 1 : 0;
+      fl[i] = uint8_t((fl[i] & ~F_OFFSET_ID) | (same ? F_OFFSET_ID : 0));
+    }
+    if (nb > 0) HIPCHK(c, hipMemcpy(c->field[CBZ_FLAGS], fl.data(), nb, hipMemcpyHostToDevice));
+  }
+  return CBZ_OK;
+}
+
+int cbz_get_field(cbz_ctx* c, int32_t field, void* host) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (field < 0 || field >= CBZ_NUM_FIELDS || !host) return fail(c, CBZ_ERR_ARG, "get_field: bad field");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const FieldInfo& fi = kFields[field];
+  const int64_t nb = c->nb;
+  if (field == CBZ_LIN_DAMP || field == CBZ_ANG_DAMP) {
+    const auto& src = field == CBZ_LIN_DAMP ? c->lin_damp : c->ang_damp;
+    std::memcpy(host, src.data(), nb * sizeof(double));
+    return CBZ_OK;
+  }
+  if (fi.kind == 1) {
+    if (nb > 0) HIPCHK(c, hipMemcpy(host, c->field[field], nb * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return CBZ_OK;
+  }
+  if (fi.kind == 2) {
+    if (nb > 0) HIPCHK(c, hipMemcpy(host, c->field[field], nb, hipMemcpyDeviceToHost));
+    uint8_t* h = (uint8_t*)host;
+    for (int64_t i = 0; i < nb; ++i) h[i] &= kUserFlags;
+    return CBZ_OK;
+  }
+  return download_real(c, c->field[field], nb, 0, nb, fi.width, (double*)host);
+}
+
+int cbz_calculate_derived(cbz_ctx* c) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->precision == CBZ_F64)
+    HIPCHK(c, launch_calculate_derived<double>(bodies_view<double>(c), c->stream));
+  else
+    HIPCHK(c, launch_calculate_derived<float>(bodies_view<float>(c), c->stream));
+  return CBZ_OK;
+}
+
+int cbz_integrate(cbz_ctx* c, double dt) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  return c->precision == CBZ_F64 ? do_integrate<double>(c, dt) : do_integrate<float>(c, dt);
+}
+
+int cbz_narrowphase(cbz_ctx* c, int64_t* total) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, -1) : do_narrowphase<float>(c, -1);
+  if (rc) return rc;
+  if ((rc = check_overflow(c))) return rc;
+  if (total) HIPCHK(c, hipMemcpy(total, c->total, sizeof(int64_t), hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+int cbz_narrowphase_pairs(cbz_ctx* c, int64_t n_pairs, const int32_t* one, const int32_t* two, int64_t* total) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (n_pairs < 0 || (n_pairs > 0 && (!one || !two)) || n_pairs > INT32_MAX / 2)
+    return fail(c, CBZ_ERR_ARG, "narrowphase_pairs: bad arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  // world of every operand; pairs grouped by non-decreasing world
+  auto world_of_body = [&](int32_t b) -> int32_t {
+    auto it = std::upper_bound(c->body_off.begin(), c->body_off.end(), b);
+    return (int32_t)(it - c->body_off.begin()) - 1;
+  };
+  auto world_of_plane = [&](int32_t p) -> int32_t {
+    auto it = std::upper_bound(c->plane_off.begin(), c->plane_off.end(), p);
+    return (int32_t)(it - c->plane_off.begin()) - 1;
+  };
+  std::vector<int32_t> loff(c->nw + 1, 0), wq(c->nw, 0);
+  int32_t prev = 0;
+  for (int64_t k = 0; k < n_pairs; ++k) {
+    int32_t wa = -1, wb = -1;
+    if (one[k] >= 0) {
+      if (one[k] >= c->nb) return fail(c, CBZ_ERR_ARG, "narrowphase_pairs: body index out of range");
+      wa = world_of_body(one[k]);
+    } else {
+      if (-one[k] - 1 >= c->np) return fail(c, CBZ_ERR_ARG, "narrowphase_pairs: plane index out of range");
+      wa = world_of_plane(-one[k] - 1);
+    }
+    if (two[k] >= 0) {
+      if (two[k] >= c->nb) return fail(c, CBZ_ERR_ARG, "narrowphase_pairs: body index out of range");
+      wb = world_of_body(two[k]);
+    } else {
+      if (-two[k] - 1 >= c->np) return fail(c, CBZ_ERR_ARG, "narrowphase_pairs: plane index out of range");
+      wb = world_of_plane(-two[k] - 1);
+    }
+    if (wa != wb) return fail(c, CBZ_ERR_ARG, "narrowphase_pairs: pair spans two worlds");
+    if (wa < prev) return fail(c, CBZ_ERR_ARG, "narrowphase_pairs: pairs not grouped by world");
+    prev = wa;
+    wq[wa]++;
+  }
+  std::vector<int32_t> wt0(c->nw + 1), tw;
+  int64_t nt = 0;
+  for (int32_t w = 0; w < c->nw; ++w) {
+    loff[w + 1] = loff[w] + wq[w];
+    wt0[w] = (int32_t)nt;
+    nt += (wq[w] + 63) / 64;
+  }
+  wt0[c->nw] = (int32_t)nt;
+  tw.resize((size_t)nt);
+  for (int32_t w = 0; w < c->nw; ++w)
+    for (int32_t t = wt0[w]; t < wt0[w + 1]; ++t) tw[t] = w;
+  if (nt > c->l_tiles_cap) {
+    if ((rc = dalloc(c, &c->l_tile_world, (size_t)nt))) return rc;
+    c->l_tiles_cap = (int32_t)nt;
+  }
+  if (n_pairs > c->l_pairs_cap) {
+    if ((rc = dalloc(c, &c->l_one, (size_t)n_pairs))) return rc;
+    if ((rc = dalloc(c, &c->l_two, (size_t)n_pairs))) return rc;
+    c->l_pairs_cap = (int32_t)n_pairs;
+  }
+  if (!c->l_world_tile0) {
+    if ((rc = dalloc(c, &c->l_world_tile0, (size_t)c->nw + 1))) return rc;
+    if ((rc = dalloc(c, &c->l_world_q, (size_t)c->nw))) return rc;
+    if ((rc = dalloc(c, &c->l_off, (size_t)c->nw + 1))) return rc;
+  }
+  if ((rc = ensure_tiles(c, (int32_t)std::max<int64_t>(nt, 1)))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (nt > 0) HIPCHK(c, hipMemcpy(c->l_tile_world, tw.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (n_pairs > 0) {
+    HIPCHK(c, hipMemcpy(c->l_one, one, n_pairs * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->l_two, two, n_pairs * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  HIPCHK(c, hipMemcpy(c->l_world_tile0, wt0.data(), wt0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->l_world_q, wq.data(), wq.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->l_off, loff.data(), loff.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  rc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, (int32_t)nt) : do_narrowphase<float>(c, (int32_t)nt);
+  if (rc) return rc;
+  if ((rc = check_overflow(c))) return rc;
+  if (total) HIPCHK(c, hipMemcpy(total, c->total, sizeof(int64_t), hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+int cbz_contact_counts(cbz_ctx* c, int64_t* counts) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (!counts) return fail(c, CBZ_ERR_ARG, "contact_counts: null output");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<int32_t> t((size_t)c->nw);
+  HIPCHK(c, hipMemcpy(t.data(), c->world_cnt, c->nw * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int32_t w = 0; w < c->nw; ++w) counts[w] = t[w];
+  return CBZ_OK;
+}
+
+int cbz_get_contacts(cbz_ctx* c, int32_t world, cbz_contact_view* out) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (!out || world < 0 || world >= c->nw) return fail(c, CBZ_ERR_ARG, "get_contacts: bad arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int32_t n = 0;
+  int64_t off = 0;
+  HIPCHK(c, hipMemcpy(&n, c->world_cnt + world, sizeof n, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&off, c->world_coff + world, sizeof off, hipMemcpyDeviceToHost));
+  if (n == 0) return CBZ_OK;
+  if (off + n > c->cap) return fail(c, CBZ_ERR_CAPACITY, "get_contacts: world's contacts exceed the capacity");
+  const size_t rs = real_size(c);
+  const char* base = (const char*)c->c_real;
+  const int64_t S = c->cap;
+  if (out->body0) HIPCHK(c, hipMemcpy(out->body0, (int32_t*)c->c_body0 + off, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (out->body1) HIPCHK(c, hipMemcpy(out->body1, (int32_t*)c->c_body1 + off, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  struct Part {
+    double* dst;
+    int plane0;
+    int k;
+  } parts[] = {{out->point, 0, 3}, {out->normal, 3, 3}, {out->pen, 6, 1}, {out->c2w, 7, 9},
+               {out->rel, 16, 6},  {out->cvel, 22, 3},  {out->ddv, 25, 1}};
+  for (const Part& p : parts) {
+    if (!p.dst) continue;
+    if ((rc = download_real(c, base + (size_t)p.plane0 * S * rs, S, off, n, p.k, p.dst))) return rc;
+  }
+  return CBZ_OK;
+}
+
+int cbz_set_contacts(cbz_ctx* c, const int64_t* counts, const int32_t* body0, const int32_t* body1,
+                     const double* point, const double* normal, const double* pen) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (!counts) return fail(c, CBZ_ERR_ARG, "set_contacts: null counts");
+  int64_t tot = 0;
+  std::vector<int32_t> cnt((size_t)c->nw);
+  std::vector<int64_t> off((size_t)c->nw);
+  for (int32_t w = 0; w < c->nw; ++w) {
+    if (counts[w] < 0 || counts[w] > INT32_MAX) return fail(c, CBZ_ERR_ARG, "set_contacts: bad count");
+    off[w] = tot;
+    cnt[w] = (int32_t)counts[w];
+    tot += counts[w];
+  }
+  if (tot > c->cap) return fail(c, CBZ_ERR_CAPACITY, "set_contacts: more contacts than slots");
+  if (tot > 0 && (!body0 || !body1 || !point || !normal || !pen)) return fail(c, CBZ_ERR_ARG, "set_contacts: null record");
+  for (int64_t k = 0; k < tot; ++k) {
+    if (body0[k] >= c->nb || body1[k] >= c->nb || (body0[k] < 0 && body1[k] < 0))
+      return fail(c, CBZ_ERR_ARG, "set_contacts: bad body index");
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t rs = real_size(c);
+  char* base = (char*)c->c_real;
+  const int64_t S = c->cap;
+  if (tot > 0) {
+    HIPCHK(c, hipMemcpy(c->c_body0, body0, tot * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->c_body1, body1, tot * sizeof(int32_t), hipMemcpyHostToDevice));
+    // upload_real writes [k][n] with stride n; write each component plane at stride S
+    for (int k = 0; k < 3; ++k) {
+      std::vector<double> t((size_t)tot);
+      for (int64_t i = 0; i < tot; ++i) t[i] = point[3 * i + k];
+      if ((rc = upload_real(c, base + ((size_t)0 + k) * S * rs, t.data(), tot, 1))) return rc;
+      for (int64_t i = 0; i < tot; ++i) t[i] = normal[3 * i + k];
+      if ((rc = upload_real(c, base + ((size_t)3 + k) * S * rs, t.data(), tot, 1))) return rc;
+    }
+    if ((rc = upload_real(c, base + (size_t)6 * S * rs, pen, tot, 1))) return rc;
+  }
+  HIPCHK(c, hipMemcpy(c->world_cnt, cnt.data(), cnt.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->world_coff, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->total, &tot, sizeof tot, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemset(c->overflow, 0, sizeof(int32_t)));
+  c->have_contacts = true;
+  return CBZ_OK;
+}
+
+int cbz_resolve(cbz_ctx* c, double dt, int64_t iter_per_contact, int64_t iter_fixed) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (!c->have_contacts) return fail(c, CBZ_ERR_STATE, "resolve: no contacts (run a narrowphase first)");
+  if (iter_per_contact < 0 || iter_fixed < 0) return fail(c, CBZ_ERR_ARG, "resolve: negative iteration count");
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = c->precision == CBZ_F64 ? do_resolve<double>(c, dt, iter_per_contact, iter_fixed)
+                               : do_resolve<float>(c, dt, iter_per_contact, iter_fixed);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CBZ_OK;
+}
+
+int cbz_resolver_iterations(cbz_ctx* c, int64_t* iters) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (!iters) return fail(c, CBZ_ERR_ARG, "resolver_iterations: null output");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(iters, c->world_iters, 2 * c->nw * sizeof(int64_t), hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+int cbz_step(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contact, int64_t* counts) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (n_steps < 0 || iter_per_contact < 0) return fail(c, CBZ_ERR_ARG, "step: bad arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  int32_t* hist = nullptr;
+  if (counts && n_steps > 0) HIPCHK(c, hipMalloc((void**)&hist, (size_t)n_steps * c->nw * sizeof(int32_t)));
+  HIPCHK(c, launch_fill_i32(c->overflow, 1, 0, c->stream));
+  for (int32_t s = 0; s < n_steps && rc == CBZ_OK; ++s) {
+    if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    rc = c->precision == CBZ_F64 ? do_integrate<double>(c, dt) : do_integrate<float>(c, dt);
+    if (rc) break;
+    if (c->prof) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    rc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, -1) : do_narrowphase<float>(c, -1);
+    if (rc) break;
+    if (hist)
+      HIPCHK(c, hipMemcpyAsync(hist + (size_t)s * c->nw, c->world_cnt, c->nw * sizeof(int32_t),
+                               hipMemcpyDeviceToDevice, c->stream));
+    if (c->prof) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    rc = c->precision == CBZ_F64 ? do_resolve<double>(c, dt, iter_per_contact, 0)
+                                 : do_resolve<float>(c, dt, iter_per_contact, 0);
+    if (rc) break;
+    if (c->prof) {
+      HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+      HIPCHK(c, hipEventSynchronize(c->ev[3]));
+      float a = 0, b = 0, d = 0;
+      HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+      HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+      HIPCHK(c, hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
+      c->acc_ms[0] += a;
+      c->acc_ms[1] += b;
+      c->acc_ms[2] += d;
+      c->acc_ms[3] += a + b + d;
+      c->acc_steps += 1;
+    }
+  }
+  if (rc == CBZ_OK) rc = check_overflow(c);
+  if (rc == CBZ_OK && hist) {
+    std::vector<int32_t> t((size_t)n_steps * c->nw);
+    hipError_t e = hipMemcpy(t.data(), hist, t.size() * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(c, CBZ_ERR_HIP, std::string("step: count download: ") + hipGetErrorString(e));
+    for (size_t k = 0; k < t.size(); ++k) counts[k] = t[k];
+  }
+  if (hist) (void)hipFree(hist);
+  return rc;
+}
+
+int cbz_synchronize(cbz_ctx* c) {
+  if (!c) return CBZ_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CBZ_OK;
+}
+
+int cbz_set_profiling(cbz_ctx* c, int32_t on) {
+  if (!c) return CBZ_ERR_ARG;
+  c->prof = on != 0;
+  return CBZ_OK;
+}
+
+int cbz_phase_times(cbz_ctx* c, double* ms4, int64_t* steps) {
+  if (!c) return CBZ_ERR_ARG;
+  if (ms4)
+    for (int k = 0; k < 4; ++k) ms4[k] = c->acc_ms[k];
+  if (steps) *steps = c->acc_steps;
+  for (double& v : c->acc_ms) v = 0;
+  c->acc_steps = 0;
+  return CBZ_OK;
+}
+
+int cbz_selftest_f64(cbz_ctx* c, int32_t op, int64_t n, const double* a, const double* b, double* out) {
+  if (!c || n < 0 || (n > 0 && (!a || !b || !out))) return fail(c, CBZ_ERR_ARG, "selftest: bad arguments");
+  if (n == 0) return CBZ_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  double *da = nullptr, *db = nullptr, *dout = nullptr;
+  HIPCHK(c, hipMalloc((void**)&da, n * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&db, n * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&dout, n * sizeof(double)));
+  hipError_t e = hipMemcpy(da, a, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(db, b, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_selftest(op, n, da, db, dout, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(da);
+  (void)hipFree(db);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(c, CBZ_ERR_HIP, std::string("selftest: ") + hipGetErrorString(e));
+  return CBZ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// cbz_launch.h — host-side launch entry points of the step kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cbz_types.h"
+
+namespace cbz {
+
+template <typename R>
+struct NarrowLaunch {
+  Bodies<R> b;
+  Planes<R> pl;
+  PairSpace ps;
+  Contacts<R> c;
+  const double* brad;
+  int32_t* tile_cnt;
+  uint64_t* tile_mask;
+  int32_t* tile_off;
+  int32_t* world_cnt;
+  int64_t* world_coff;
+  int64_t* total;
+  int32_t* overflow;
+  int32_t n_worlds;
+};
+
+template <typename R>
+struct ResolveLaunch {
+  Bodies<R> b;
+  Contacts<R> c;
+  const int32_t* world_cnt;
+  const int64_t* world_coff;
+  const int32_t* overflow;
+  const int32_t* order;
+  int64_t* world_iters;
+  int32_t n_worlds;
+  R dt;
+  int64_t iter_mult;
+  int64_t iter_fixed;
+};
+
+template <typename R>
+hipError_t launch_calculate_derived(const Bodies<R>& b, hipStream_t st);
+template <typename R>
+hipError_t launch_integrate(const Bodies<R>& b, R dt, R bias, hipStream_t st);
+template <typename R>
+hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st);
+template <typename R>
+hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st);
+
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+
+// Device self-test of the FP64 operations whose rounding the parity claim
+// depends on: op 0 = a/b, 1 = sqrt(a), 2 = a*b, 3 = a+b, 4 = a-b.
+hipError_t launch_selftest(int op, int64_t n, const double* a, const double* b, double* out, hipStream_t st);
+
+}  // namespace cbz

@@ -1,0 +1,471 @@
+// k_narrow.hip — K2: the narrowphase over ordered pairs, with contacts
+// compacted in the reference's emission order.
+//
+// Emission order (examples/cubedrop.go:47-64): for each body i of a world,
+// first its half-space checks (planes in order, up to 8 cube vertices each
+// in the `mults` order of colliders.go:319-327), then CheckForCollisions(i, j)
+// for every j != i in ascending order.  Pair slot q of world w is
+// (row r, column col) = (q / (np+n), q % (np+n)); columns < np are planes.
+// An explicit pair list (any caller loop) uses its own order instead.
+//
+// Four launches:
+//   count  — one lane per pair slot; 64-slot tiles record a hit bitmask and
+//            the tile's contact total.  Cube-cube and cube-sphere pairs whose
+//            bounding spheres are far apart are rejected before the SAT: the
+//            bound guarantees that the reference's own test reports no
+//            contact (see sat_reject below), so the result is unchanged.
+//   scan_tiles  — one wave per world: exclusive scan of tile totals.
+//   scan_worlds — one block: exclusive scan of world totals.
+//   emit   — one wave per tile with hits: recompute the hit pairs and write
+//            each contact at world_offset + tile_offset + lane prefix.
+// All geometry is evaluated in FP64 whatever the storage type R.
+#include "cbz_devmath.h"
+#include "cbz_launch.h"
+#include "cbz_types.h"
+#include "cbz_wave.h"
+
+namespace cbz {
+
+using D3 = V3<double>;
+
+struct Col {
+  double T[12];
+  double h[3];
+  int shape;
+};
+
+template <typename R>
+__device__ __forceinline__ void load_col(const Bodies<R>& b, int i, Col& c) {
+  const int64_t S = b.stride;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) c.T[k] = double(b.ctransform[k * S + i]);
+  c.h[0] = double(b.half[i]);
+  c.h[1] = double(b.half[S + i]);
+  c.h[2] = double(b.half[2 * S + i]);
+  c.shape = b.shape[i];
+}
+
+template <typename R>
+struct NP {
+  Bodies<R> b;
+  Planes<R> pl;
+  PairSpace ps;
+  Contacts<R> c;
+  const double* brad;  // conservative bounding radius per body
+  int32_t* tile_cnt;
+  uint64_t* tile_mask;
+  int32_t* tile_off;
+  int32_t* world_cnt;
+  int64_t* world_coff;
+  int64_t* total;
+  int32_t* overflow;
+  int32_t n_worlds;
+};
+
+template <typename R>
+__device__ __forceinline__ void put_contact(const Contacts<R>& c, int64_t slot, int b0, int b1, const D3& p,
+                                            const D3& n, double pen) {
+  if (slot >= c.cap) return;
+  const int64_t S = c.cap;
+  c.body0[slot] = b0;
+  c.body1[slot] = b1;
+  c.point[slot] = R(p.x);
+  c.point[S + slot] = R(p.y);
+  c.point[2 * S + slot] = R(p.z);
+  c.normal[slot] = R(n.x);
+  c.normal[S + slot] = R(n.y);
+  c.normal[2 * S + slot] = R(n.z);
+  c.pen[slot] = R(pen);
+}
+
+// transformToAxis, colliders.go:762-770
+__device__ __forceinline__ double transform_to_axis(const Col& c, const D3& a) {
+  return c.h[0] * dabs(dot(a, axis(c.T, 0))) + c.h[1] * dabs(dot(a, axis(c.T, 1))) +
+         c.h[2] * dabs(dot(a, axis(c.T, 2)));
+}
+
+// CollisionSphere.CheckAgainstHalfSpace, colliders.go:180-207
+template <typename R, bool EMIT>
+__device__ int sphere_half_space(const NP<R>& a, const Col& s, int si, const D3& pn, double po, int64_t slot) {
+  D3 positionAxis = axis(s.T, 3);
+  double r = s.h[0];
+  double distance = dot(pn, positionAxis) - r;
+  if ((distance <= po) == false) return 0;
+  if (EMIT) {
+    D3 p = add(mul(pn, distance + r * -1.0), positionAxis);
+    put_contact(a.c, slot, si, -1, p, pn, -distance);
+  }
+  return 1;
+}
+
+// CollisionCube.CheckAgainstHalfSpace, colliders.go:308-366, with
+// intersectCubeAndHalfSpace (:750-760) as the early-out.
+template <typename R, bool EMIT>
+__device__ int cube_half_space(const NP<R>& a, const Col& c, int ci, const D3& pn, double po, int64_t slot) {
+  double projectedRadius = transform_to_axis(c, pn);
+  double cubeDistance = dot(pn, axis(c.T, 3)) - projectedRadius;
+  if (!(cubeDistance <= po)) return 0;
+  int k = 0;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    // mults[v] = (+-1, +-1, +-1): x flips on odd v, y on v&2, z on v&4
+    D3 vert{(v & 1) ? -c.h[0] : c.h[0], (v & 2) ? -c.h[1] : c.h[1], (v & 4) ? -c.h[2] : c.h[2]};
+    D3 vertexPos = m34v(c.T, vert);
+    double vertexDistance = dot(vertexPos, pn);
+    if (vertexDistance <= po) {
+      if (EMIT) {
+        D3 p = add(mul(pn, vertexDistance - po), vertexPos);
+        put_contact(a.c, slot + k, ci, -1, p, pn, po - vertexDistance);
+      }
+      ++k;
+    }
+  }
+  return k;
+}
+
+// CollisionSphere.CheckAgainstSphere, colliders.go:216-254
+template <typename R, bool EMIT>
+__device__ int sphere_sphere(const NP<R>& a, const Col& s1, int i1, const Col& s2, int i2, int64_t slot) {
+  D3 p1 = axis(s1.T, 3), p2 = axis(s2.T, 3);
+  D3 midline = sub(p1, p2);
+  double size = mag(midline);
+  if (size <= 0.0 || size >= s1.h[0] + s2.h[0]) return 0;
+  if (EMIT) {
+    D3 p = add(mul(midline, 0.5), p1);
+    D3 n = mul(midline, 1.0 / size);
+    put_contact(a.c, slot, i1, i2, p, n, s1.h[0] + s2.h[0] - size);
+  }
+  return 1;
+}
+
+// CollisionCube.CheckAgainstSphere, colliders.go:369-441
+template <typename R, bool EMIT>
+__device__ int cube_sphere(const NP<R>& a, const Col& c, int ci, const Col& s, int si, int64_t slot) {
+  const double r = s.h[0];
+  D3 position = axis(s.T, 3);
+  D3 rc = m34_tinv(c.T, position);
+  if (dabs(rc.x) - r > c.h[0] || dabs(rc.y) - r > c.h[1] || dabs(rc.z) - r > c.h[2]) return 0;
+  double cp[3];
+  const double rcv[3] = {rc.x, rc.y, rc.z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double dist = rcv[k];
+    if (dist > c.h[k]) {
+      dist = c.h[k];
+    } else if (dist < -c.h[k]) {
+      dist = -c.h[k];
+    }
+    cp[k] = dist;
+  }
+  D3 closest{cp[0], cp[1], cp[2]};
+  double dist = sqmag(sub(closest, rc));
+  if (dist > r * r) return 0;
+  if (EMIT) {
+    D3 cw = m34v(c.T, closest);
+    D3 n = sub(cw, position);
+    if (real_equal(mag(n), 0.0)) {
+      const int64_t S = a.b.stride;
+      n = D3{double(a.b.vel[si]), double(a.b.vel[S + si]), double(a.b.vel[2 * S + si])};
+    }
+    n = normalize(n);
+    double pen = r;
+    if (!real_equal(dist, 0.0)) {
+      pen -= dsqrt(dist);
+    } else {
+      pen = 0.0;
+    }
+    put_contact(a.c, slot, ci, si, cw, n, pen);
+  }
+  return 1;
+}
+
+// tryAxis, colliders.go:458-477 with penetrationOnAxis (:445-456)
+__device__ __forceinline__ bool try_axis(const Col& one, const Col& two, D3 ax, const D3& toCenter, int index,
+                                         double& smallest, int& best) {
+  if (sqmag(ax) < Lim<double>::eps) return true;
+  ax = normalize(ax);
+  double oneProject = transform_to_axis(one, ax);
+  double twoProject = transform_to_axis(two, ax);
+  double distance = dabs(dot(toCenter, ax));
+  double penetration = oneProject + twoProject - distance;
+  if (penetration < 0) return false;
+  if (penetration < smallest) {
+    smallest = penetration;
+    best = index;
+  }
+  return true;
+}
+
+// fillPointFaceBoxBox, colliders.go:481-517
+__device__ __forceinline__ void point_face(const Col& one, const Col& two, const D3& toCenter, int best, D3& n,
+                                           D3& p) {
+  n = axis(one.T, best);
+  if (dot(n, toCenter) > 0) n = mul(n, -1.0);
+  D3 v{two.h[0], two.h[1], two.h[2]};
+  if (dot(axis(two.T, 0), n) < 0) v.x = -v.x;
+  if (dot(axis(two.T, 1), n) < 0) v.y = -v.y;
+  if (dot(axis(two.T, 2), n) < 0) v.z = -v.z;
+  p = m34v(two.T, v);
+}
+
+// contactPoint, colliders.go:519-573
+__device__ __forceinline__ D3 contact_point(const D3& pOne, const D3& dOne, double oneSize, const D3& pTwo,
+                                            const D3& dTwo, double twoSize, bool useOne) {
+  double smOne = sqmag(dOne);
+  double smTwo = sqmag(dTwo);
+  double dpOneTwo = dot(dTwo, dOne);
+  D3 toSt = sub(pOne, pTwo);
+  double dpStaOne = dot(dOne, toSt);
+  double dpStaTwo = dot(dTwo, toSt);
+  double denom = smOne * smTwo - dpOneTwo * dpOneTwo;
+  if (dabs(denom) < Lim<double>::eps) return useOne ? pOne : pTwo;
+  double mua = (dpOneTwo * dpStaTwo - smTwo * dpStaOne) / denom;
+  double mub = (smOne * dpStaTwo - dpOneTwo * dpStaOne) / denom;
+  if (mua > oneSize || mua < -oneSize || mub > twoSize || mub < -twoSize) return useOne ? pOne : pTwo;
+  D3 cOne = add(mul(dOne, mua), pOne);
+  D3 cTwo = add(mul(dTwo, mub), pTwo);
+  cOne = mul(cOne, 0.5);
+  cTwo = mul(cTwo, 0.5);
+  return add(cOne, cTwo);
+}
+
+// CollisionCube.CheckAgainstCube, colliders.go:576-710
+template <typename R, bool EMIT>
+__device__ int cube_cube(const NP<R>& a, const Col& one, int i1, const Col& two, int i2, int64_t slot) {
+  D3 toCenter = sub(axis(two.T, 3), axis(one.T, 3));
+  double pen = Lim<double>::max_value;
+  int best = 0xffffff;
+  for (int i = 0; i <= 2; ++i)
+    if (!try_axis(one, two, axis(one.T, i), toCenter, i, pen, best)) return 0;
+  for (int i = 0; i <= 2; ++i)
+    if (!try_axis(one, two, axis(two.T, i), toCenter, i + 3, pen, best)) return 0;
+  const int bestSingleAxis = best;
+  for (int i = 0; i <= 2; ++i)
+    for (int k = 0; k <= 2; ++k)
+      if (!try_axis(one, two, cross(axis(one.T, i), axis(two.T, k)), toCenter, i * 3 + 6 + k, pen, best)) return 0;
+  if (!EMIT) return 1;
+  D3 n, p;
+  if (best < 3) {
+    point_face(one, two, toCenter, best, n, p);
+    put_contact(a.c, slot, i1, i2, p, n, pen);
+  } else if (best < 6) {
+    point_face(two, one, mul(toCenter, -1.0), best - 3, n, p);
+    put_contact(a.c, slot, i2, i1, p, n, pen);
+  } else {
+    best -= 6;
+    const int oneAxisIndex = best / 3;
+    const int twoAxisIndex = best % 3;
+    D3 oneAxis = axis(one.T, oneAxisIndex);
+    D3 twoAxis = axis(two.T, twoAxisIndex);
+    D3 ax = normalize(cross(oneAxis, twoAxis));
+    if (dot(ax, toCenter) > 0) ax = mul(ax, -1.0);
+    double p1[3] = {one.h[0], one.h[1], one.h[2]};
+    double p2[3] = {two.h[0], two.h[1], two.h[2]};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (i == oneAxisIndex) {
+        p1[i] = 0;
+      } else if (dot(axis(one.T, i), ax) > 0) {
+        p1[i] = -p1[i];
+      }
+      if (i == twoAxisIndex) {
+        p2[i] = 0;
+      } else if (dot(axis(two.T, i), ax) < 0) {
+        p2[i] = -p2[i];
+      }
+    }
+    D3 ptOnOneEdge = m34v(one.T, D3{p1[0], p1[1], p1[2]});
+    D3 ptOnTwoEdge = m34v(two.T, D3{p2[0], p2[1], p2[2]});
+    const bool useOne = bestSingleAxis > 2;
+    // an index past 2 only arises from a NaN state; GetAxis clamps it to 3
+    const double oneSize = oneAxisIndex < 3 ? one.h[oneAxisIndex] : 0.0;
+    const double twoSize = twoAxisIndex < 3 ? two.h[twoAxisIndex] : 0.0;
+    D3 cv = contact_point(ptOnOneEdge, oneAxis, oneSize, ptOnTwoEdge, twoAxis, twoSize, useOne);
+    put_contact(a.c, slot, i1, i2, cv, ax, pen);
+  }
+  return 1;
+}
+
+// Conservative rejection for body pairs.  For orthonormal box axes a_k,
+// max_k |t.a_k| >= |t|/sqrt(3), while the projections of the two shapes on
+// a_k are bounded by their bounding radii; so |t|^2 > 3 (rA+rB)^2 makes a
+// face axis of A separate (tryAxis returns false) and a sphere centre
+// farther than rA + rB from a box centre fails the closest-point test.
+// The factors carry a 1e-3 margin for rounding and for the 1e-7 slack the
+// quaternion normalisation leaves in the axes.
+template <typename R>
+__device__ __forceinline__ bool far_apart(const NP<R>& a, int i, int j, int si, int sj) {
+  const int64_t S = a.b.stride;
+  double dx = double(a.b.ctransform[9 * S + j]) - double(a.b.ctransform[9 * S + i]);
+  double dy = double(a.b.ctransform[10 * S + j]) - double(a.b.ctransform[10 * S + i]);
+  double dz = double(a.b.ctransform[11 * S + j]) - double(a.b.ctransform[11 * S + i]);
+  double d2 = dx * dx + dy * dy + dz * dz;
+  double rr = a.brad[i] + a.brad[j];
+  double k = (si == SHAPE_CUBE && sj == SHAPE_CUBE) ? 3.003 : 1.001;
+  return d2 > k * rr * rr;
+}
+
+// CheckForCollisions, colliders.go:720-747, including the plane delegations
+// (:116-125) and the sphere-to-cube delegation (:210-213).
+template <typename R, bool EMIT>
+__device__ int pair_eval(const NP<R>& a, int one, int two, int64_t slot) {
+  if (one < 0 && two < 0) return 0;
+  if (one < 0 || two < 0) {
+    const int bi = one < 0 ? two : one;
+    const int pk = one < 0 ? -one - 1 : -two - 1;
+    const int64_t PS = a.pl.stride;
+    D3 pn{double(a.pl.normal[pk]), double(a.pl.normal[PS + pk]), double(a.pl.normal[2 * PS + pk])};
+    double po = double(a.pl.offset[pk]);
+    Col c;
+    load_col(a.b, bi, c);
+    if (c.shape == SHAPE_SPHERE) return sphere_half_space<R, EMIT>(a, c, bi, pn, po, slot);
+    return cube_half_space<R, EMIT>(a, c, bi, pn, po, slot);
+  }
+  const int s1 = a.b.shape[one], s2 = a.b.shape[two];
+  if (!(s1 == SHAPE_SPHERE && s2 == SHAPE_SPHERE) && far_apart(a, one, two, s1, s2)) return 0;
+  Col c1, c2;
+  load_col(a.b, one, c1);
+  load_col(a.b, two, c2);
+  if (s2 == SHAPE_SPHERE) {
+    if (s1 == SHAPE_SPHERE) return sphere_sphere<R, EMIT>(a, c1, one, c2, two, slot);
+    return cube_sphere<R, EMIT>(a, c1, one, c2, two, slot);
+  }
+  if (s1 == SHAPE_SPHERE) return cube_sphere<R, EMIT>(a, c2, two, c1, one, slot);
+  return cube_cube<R, EMIT>(a, c1, one, c2, two, slot);
+}
+
+// Pair slot q of world w -> (one, two).  Returns false for the j == i
+// diagonal of the all-pairs space (no check is made there).
+__device__ __forceinline__ bool pair_of(const PairSpace& ps, int w, int q, int& one, int& two) {
+  if (ps.list_one) {
+    const int p = ps.list_off[w] + q;
+    one = ps.list_one[p];
+    two = ps.list_two[p];
+    return true;
+  }
+  const int b0 = ps.body_off[w], n = ps.body_off[w + 1] - b0;
+  const int p0 = ps.plane_off[w], np = ps.plane_off[w + 1] - p0;
+  const int width = np + n;
+  const int r = q / width, col = q - r * width;
+  one = b0 + r;
+  if (col < np) {
+    two = -(p0 + col) - 1;
+    return true;
+  }
+  const int j = col - np;
+  two = b0 + j;
+  return j != r;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_np_count(NP<R> a) {
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= a.ps.n_tiles) return;
+  const int w = a.ps.tile_world[t];
+  const int q = (t - a.ps.world_tile0[w]) * 64 + lane;
+  int cnt = 0;
+  if (q < a.ps.world_q[w]) {
+    int one, two;
+    if (pair_of(a.ps, w, q, one, two)) cnt = pair_eval<R, false>(a, one, two, 0);
+  }
+  const uint64_t m = __ballot(cnt > 0);
+  const int tot = wave_sum_i32(cnt);
+  if (lane == 0) {
+    a.tile_cnt[t] = tot;
+    a.tile_mask[t] = m;
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(64) k_np_scan_tiles(NP<R> a) {
+  const int w = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int t0 = a.ps.world_tile0[w], nt = a.ps.world_tile0[w + 1] - t0;
+  int carry = 0;
+  for (int base = 0; base < nt; base += 64) {
+    const int k = base + lane;
+    const int v = k < nt ? a.tile_cnt[t0 + k] : 0;
+    const int inc = wave_incl_scan_i32(v);
+    if (k < nt) a.tile_off[t0 + k] = carry + inc - v;
+    carry += __shfl(inc, 63, 64);
+  }
+  if (lane == 0) a.world_cnt[w] = carry;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(1024) k_np_scan_worlds(NP<R> a) {
+  __shared__ int64_t wsum[16];
+  __shared__ int64_t s_carry;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < a.n_worlds; base += 1024) {
+    const int k = base + tid;
+    const int64_t v = k < a.n_worlds ? int64_t(a.world_cnt[k]) : 0;
+    const int64_t inc = wave_incl_scan_i64(v);
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    int64_t before = 0;
+    for (int u = 0; u < wv; ++u) before += wsum[u];
+    const int64_t carry = s_carry;
+    if (k < a.n_worlds) a.world_coff[k] = carry + before + inc - v;
+    __syncthreads();
+    if (tid == 1023) s_carry = carry + before + inc;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    a.total[0] = s_carry;
+    if (s_carry > a.c.cap) a.overflow[0] = 1;
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_np_emit(NP<R> a) {
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= a.ps.n_tiles) return;
+  const uint64_t m = a.tile_mask[t];
+  if (m == 0) return;
+  const int w = a.ps.tile_world[t];
+  const int q = (t - a.ps.world_tile0[w]) * 64 + lane;
+  const bool hit = (m >> lane) & 1ull;
+  int one = 0, two = 0, cnt = 0;
+  if (hit) {
+    pair_of(a.ps, w, q, one, two);
+    cnt = pair_eval<R, false>(a, one, two, 0);
+  }
+  const int inc = wave_incl_scan_i32(cnt);
+  const int64_t slot = a.world_coff[w] + a.tile_off[t] + (inc - cnt);
+  if (hit) pair_eval<R, true>(a, one, two, slot);
+}
+
+template <typename R>
+hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st) {
+  NP<R> a;
+  a.b = L.b;
+  a.pl = L.pl;
+  a.ps = L.ps;
+  a.c = L.c;
+  a.brad = L.brad;
+  a.tile_cnt = L.tile_cnt;
+  a.tile_mask = L.tile_mask;
+  a.tile_off = L.tile_off;
+  a.world_cnt = L.world_cnt;
+  a.world_coff = L.world_coff;
+  a.total = L.total;
+  a.overflow = L.overflow;
+  a.n_worlds = L.n_worlds;
+  if (a.n_worlds <= 0) return hipSuccess;
+  const int nb = (a.ps.n_tiles + 3) / 4;
+  if (nb > 0) k_np_count<R><<<nb, 256, 0, st>>>(a);
+  k_np_scan_tiles<R><<<a.n_worlds, 64, 0, st>>>(a);
+  k_np_scan_worlds<R><<<1, 1024, 0, st>>>(a);
+  if (nb > 0) k_np_emit<R><<<nb, 256, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+template hipError_t launch_narrowphase<double>(const NarrowLaunch<double>&, hipStream_t);
+template hipError_t launch_narrowphase<float>(const NarrowLaunch<float>&, hipStream_t);
+
+}  // namespace cbz

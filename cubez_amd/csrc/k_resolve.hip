@@ -1,0 +1,529 @@
+// k_resolve.hip — K3: ResolveContacts (contact.go:208-222), one wavefront
+// per world.
+//
+// The reference's resolver is sequential: each iteration picks the worst
+// contact (largest penetration, then largest desired velocity change; ties
+// go to the lowest index because the scan uses a strict '>'), applies a
+// correction to its one or two bodies, and updates every contact that
+// shares one of those bodies.  A world therefore runs on one wave: the
+// argmax is a lane-strided scan plus a butterfly reduction, the correction
+// is computed redundantly by all 64 lanes (lane 0 stores it), and the
+// update scans the world's contacts lane-parallel, each contact applying
+// its matches in the reference's (b, d) order.  Worlds run concurrently.
+//
+// The selection keys and the body indices of the world's contacts live in
+// LDS when the world has at most kKeyCap contacts, otherwise in HBM.
+#include <climits>
+
+#include "cbz_devmath.h"
+#include "cbz_launch.h"
+#include "cbz_types.h"
+#include "cbz_wave.h"
+
+namespace cbz {
+
+constexpr int kKeyCap = 2048;
+
+template <typename R>
+struct RS {
+  Bodies<R> b;
+  Contacts<R> c;
+  const int32_t* world_cnt;
+  const int64_t* world_coff;
+  const int32_t* overflow;
+  const int32_t* order;
+  int64_t* world_iters;  // [2 * n_worlds]: position / velocity iterations
+  int32_t n_worlds;
+  R dt;
+  int64_t iter_mult;
+  int64_t iter_fixed;
+};
+
+template <typename R>
+__device__ __forceinline__ V3<R> ld3(const R* a, int64_t s, int64_t i) {
+  return V3<R>{a[i], a[s + i], a[2 * s + i]};
+}
+template <typename R>
+__device__ __forceinline__ void st3(R* a, int64_t s, int64_t i, const V3<R>& v) {
+  a[i] = v.x;
+  a[s + i] = v.y;
+  a[2 * s + i] = v.z;
+}
+template <typename R>
+__device__ __forceinline__ void ld9(const R* a, int64_t s, int64_t i, R* o) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) o[k] = a[k * s + i];
+}
+
+template <typename R>
+__device__ __forceinline__ bool awake(const Bodies<R>& b, int i) {
+  return (b.flags[i] & F_AWAKE) != 0;
+}
+
+// calculateLocalVelocity, contact.go:159-182
+template <typename R>
+__device__ __forceinline__ V3<R> local_velocity(const Bodies<R>& b, int body, const V3<R>& rel, const R* c2w,
+                                                R dt) {
+  const int64_t S = b.stride;
+  V3<R> velocity = add(cross(ld3(b.rot, S, body), rel), ld3(b.vel, S, body));
+  V3<R> cv = m3tv(c2w, velocity);
+  V3<R> accv = m3tv(c2w, mul(ld3(b.last_acc, S, body), dt));
+  accv.x = R(0);
+  return add(cv, accv);
+}
+
+// calculateDesiredDeltaVelocity, contact.go:87-112
+template <typename R>
+__device__ __forceinline__ R desired_dv(const Bodies<R>& b, int b0, int b1, const V3<R>& n, R cvx, R dt) {
+  const int64_t S = b.stride;
+  R vfa = R(0);
+  if (awake(b, b0)) vfa += dot(mul(ld3(b.last_acc, S, b0), dt), n);
+  if (b1 >= 0 && awake(b, b1)) vfa -= dot(mul(ld3(b.last_acc, S, b1), dt), n);
+  R restitution = R(kRestitution);
+  if (dabs(cvx) < R(0.25)) restitution = R(0);
+  return -cvx - restitution * (cvx - vfa);
+}
+
+// calculateContactBasis, contact.go:118-156
+template <typename R>
+__device__ __forceinline__ void contact_basis(const V3<R>& n, R* m) {
+  V3<R> ty, tz;
+  if (dabs(n.x) > dabs(n.y)) {
+    R s = R(1) / dsqrt(n.z * n.z + n.x * n.x);
+    ty = V3<R>{n.z * s, R(0), n.x * -s};
+    tz = V3<R>{n.y * ty.x, n.z * ty.x - n.x * ty.z, -n.y * ty.x};
+  } else {
+    R s = R(1) / dsqrt(n.z * n.z + n.y * n.y);
+    ty = V3<R>{R(0), -n.z * s, n.y * s};
+    tz = V3<R>{n.y * ty.z - n.z * ty.y, -n.x * ty.z, n.x * ty.y};
+  }
+  // SetComponents(normal, tangentY, tangentZ), math/matrix.go:52-56
+  m[0] = n.x; m[1] = n.y; m[2] = n.z;
+  m[3] = ty.x; m[4] = ty.y; m[5] = ty.z;
+  m[6] = tz.x; m[7] = tz.y; m[8] = tz.z;
+}
+
+// calculateInternals, contact.go:59-85, for contact g.
+template <typename R>
+__device__ void prepare_contact(const RS<R>& a, int64_t g) {
+  const Bodies<R>& b = a.b;
+  const Contacts<R>& c = a.c;
+  const int64_t S = b.stride, CS = c.cap;
+  int b0 = c.body0[g], b1 = c.body1[g];
+  V3<R> n = ld3(c.normal, CS, g);
+  if (b0 < 0) {  // contact.go:61-65 (not produced by the narrowphase)
+    n = mul(n, R(-1));
+    b0 = b1;
+    b1 = -1;
+    c.body0[g] = b0;
+    c.body1[g] = b1;
+    st3(c.normal, CS, g, n);
+  }
+  R m[9];
+  contact_basis(n, m);
+  const V3<R> p = ld3(c.point, CS, g);
+  const V3<R> r0 = sub(p, ld3(b.pos, S, b0));
+  V3<R> cv = local_velocity(b, b0, r0, m, a.dt);
+  V3<R> r1{R(0), R(0), R(0)};
+  if (b1 >= 0) {
+    r1 = sub(p, ld3(b.pos, S, b1));
+    cv = sub(cv, local_velocity(b, b1, r1, m, a.dt));
+  }
+  const R ddv = desired_dv(b, b0, b1, n, cv.x, a.dt);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) c.c2w[k * CS + g] = m[k];
+  st3(c.rel, CS, g, r0);
+  st3(c.rel + 3 * CS, CS, g, r1);
+  st3(c.cvel, CS, g, cv);
+  c.ddv[g] = ddv;
+}
+
+// matchAwakeState, contact.go:185-202.  Returns the awake states after the
+// call in a0/a1 (lane 0 stores the change).
+template <typename R>
+__device__ __forceinline__ void match_awake(const RS<R>& a, int k0, int k1, bool& a0, bool& a1, int lane) {
+  a0 = awake(a.b, k0);
+  a1 = k1 >= 0 ? awake(a.b, k1) : false;
+  if (k1 < 0) return;
+  if ((a0 || a1) && !(a0 && a1)) {
+    const int t = a0 ? k1 : k0;
+    if (lane == 0) {  // SetAwake(true), rigidbody.go:183-186
+      a.b.flags[t] = uint8_t(a.b.flags[t] | F_AWAKE);
+      a.b.motion[t] = R(kSleepEpsilon * 2.0);
+    }
+    a0 = a1 = true;
+  }
+}
+
+// CalculateDerivedData (rigidbody.go:268-272) for a sleeping body moved by
+// applyPositionChange (contact.go:380-382).
+template <typename R>
+__device__ __forceinline__ void rederive(const Bodies<R>& b, int i, const V3<R>& pos, Q4<R>& q, int lane) {
+  const int64_t S = b.stride;
+  q = q_normalize(q);
+  R T[12], IB[9], IW[9];
+  set_as_transform(pos, q, T);
+  ld9(b.iit_body, S, i, IB);
+  transform_inertia(IB, T, IW);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) b.transform[k * S + i] = T[k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) b.iit_world[k * S + i] = IW[k];
+  }
+}
+
+// applyPositionChange, contact.go:286-386
+template <typename R>
+__device__ void apply_position_change(const RS<R>& a, const int kb[2], const bool aw[2], const V3<R>& n,
+                                      const V3<R> rel[2], R penetration, V3<R> lin[2], V3<R> ang[2], int lane) {
+  const Bodies<R>& b = a.b;
+  const int64_t S = b.stride;
+  const R angularLimit = R(0.2);
+  R angI[2] = {R(0), R(0)}, linI[2] = {R(0), R(0)};
+  R total = R(0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    lin[i] = V3<R>{R(0), R(0), R(0)};
+    ang[i] = V3<R>{R(0), R(0), R(0)};
+    if (kb[i] < 0) continue;
+    R iw[9];
+    ld9(b.iit_world, S, kb[i], iw);
+    V3<R> aiw = cross(rel[i], n);
+    aiw = m3v(iw, aiw);
+    aiw = cross(aiw, rel[i]);
+    angI[i] = dot(aiw, n);
+    linI[i] = b.inv_mass[kb[i]];
+    total += linI[i] + angI[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int bi = kb[i];
+    if (bi < 0) continue;
+    R sign = R(1);
+    if (i != 0) sign = R(-1);
+    R angM = sign * penetration * (angI[i] / total);
+    R linM = sign * penetration * (linI[i] / total);
+    V3<R> projection = add_scaled(rel[i], n, -dot(rel[i], n));
+    R maxMagnitude = angularLimit * mag(projection);
+    if (angM < -maxMagnitude) {
+      R totalMove = angM + linM;
+      angM = -maxMagnitude;
+      linM = totalMove - angM;
+    } else if (angM > maxMagnitude) {
+      R totalMove = angM + linM;
+      angM = maxMagnitude;
+      linM = totalMove - angM;
+    }
+    if (angM == R(0)) {
+      ang[i] = V3<R>{R(0), R(0), R(0)};
+    } else {
+      V3<R> tad = cross(rel[i], n);
+      R iw[9];
+      ld9(b.iit_world, S, bi, iw);
+      ang[i] = mul(m3v(iw, tad), angM / angI[i]);
+    }
+    lin[i] = mul(n, linM);
+    V3<R> pos = add_scaled(ld3(b.pos, S, bi), n, linM);
+    Q4<R> q{b.quat[bi], b.quat[S + bi], b.quat[2 * S + bi], b.quat[3 * S + bi]};
+    q = q_add_scaled_vector(q, ang[i], R(1));
+    q = q_normalize(q);
+    if (!aw[i]) rederive(b, bi, pos, q, lane);
+    if (lane == 0) {
+      st3(b.pos, S, bi, pos);
+      b.quat[bi] = q.w;
+      b.quat[S + bi] = q.x;
+      b.quat[2 * S + bi] = q.y;
+      b.quat[3 * S + bi] = q.z;
+    }
+  }
+}
+
+// setSkewSymmetric, contact.go:612-616
+template <typename R>
+__device__ __forceinline__ void skew(const V3<R>& v, R* m) {
+  m[0] = R(0); m[3] = -v.z; m[6] = v.y;
+  m[1] = v.z;  m[4] = R(0); m[7] = -v.x;
+  m[2] = -v.y; m[5] = v.x;  m[8] = R(0);
+}
+
+// calculateFrictionImpulse, contact.go:535-606.  (Every contact carries
+// Friction 0.9, so applyVelocityChange never takes the frictionless branch
+// of contact.go:459-461.)
+template <typename R>
+__device__ __forceinline__ V3<R> friction_impulse(const Bodies<R>& b, const int kb[2], const R* iit0, const R* iit1,
+                                                  const V3<R> rel[2], const R* c2w, R ddv, const V3<R>& cvel) {
+  const R friction = R(kFriction);
+  R inverseMass = b.inv_mass[kb[0]];
+  R sk[9], t[9], dvw[9];
+  skew(rel[0], sk);
+  m3mul(sk, iit0, t);
+  m3mul(t, sk, dvw);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dvw[k] *= R(-1);
+  if (kb[1] >= 0) {
+    R d2[9];
+    skew(rel[1], sk);
+    m3mul(sk, iit1, t);
+    m3mul(t, sk, d2);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) d2[k] *= R(-1);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dvw[k] += d2[k];
+    inverseMass += b.inv_mass[kb[1]];
+  }
+  R ct[9] = {c2w[0], c2w[3], c2w[6], c2w[1], c2w[4], c2w[7], c2w[2], c2w[5], c2w[8]};
+  R dv[9];
+  m3mul(ct, dvw, t);
+  m3mul(t, c2w, dv);
+  dv[0] += inverseMass;
+  dv[4] += inverseMass;
+  dv[8] += inverseMass;
+  R im[9];
+  m3inv(dv, im);
+  V3<R> velKill{ddv, -cvel.y, -cvel.z};
+  V3<R> ic = m3v(im, velKill);
+  R planar = dsqrt(ic.y * ic.y + ic.z * ic.z);
+  if (planar > ic.x * friction) {
+    ic.y /= planar;
+    ic.z /= planar;
+    ic.x = dv[0] + dv[3] * friction * ic.y + dv[6] * friction * ic.z;
+    ic.x = ddv / ic.x;
+    ic.y *= friction * ic.x;
+    ic.z *= friction * ic.x;
+  }
+  return ic;
+}
+
+// applyVelocityChange, contact.go:448-494
+template <typename R>
+__device__ void apply_velocity_change(const RS<R>& a, const int kb[2], const V3<R> rel[2], const R* c2w, R ddv,
+                                      const V3<R>& cvel, V3<R> vc[2], V3<R> rc[2], int lane) {
+  const Bodies<R>& b = a.b;
+  const int64_t S = b.stride;
+  R iit0[9], iit1[9];
+  ld9(b.iit_world, S, kb[0], iit0);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) iit1[k] = R(0);
+  if (kb[1] >= 0) ld9(b.iit_world, S, kb[1], iit1);
+  const V3<R> ic = friction_impulse(b, kb, iit0, iit1, rel, c2w, ddv, cvel);
+  const V3<R> impulse = m3v(c2w, ic);
+  const V3<R> zero{R(0), R(0), R(0)};
+  V3<R> torque = cross(rel[0], impulse);
+  rc[0] = m3v(iit0, torque);
+  vc[0] = add_scaled(zero, impulse, b.inv_mass[kb[0]]);
+  rc[1] = zero;
+  vc[1] = zero;
+  const V3<R> v0 = add(ld3(b.vel, S, kb[0]), vc[0]);
+  const V3<R> w0 = add(ld3(b.rot, S, kb[0]), rc[0]);
+  if (lane == 0) {
+    st3(b.vel, S, kb[0], v0);
+    st3(b.rot, S, kb[0], w0);
+  }
+  if (kb[1] >= 0) {
+    torque = cross(impulse, rel[1]);
+    rc[1] = m3v(iit1, torque);
+    vc[1] = add_scaled(zero, impulse, -b.inv_mass[kb[1]]);
+    const V3<R> v1 = add(ld3(b.vel, S, kb[1]), vc[1]);
+    const V3<R> w1 = add(ld3(b.rot, S, kb[1]), rc[1]);
+    if (lane == 0) {
+      st3(b.vel, S, kb[1], v1);
+      st3(b.rot, S, kb[1], w1);
+    }
+  }
+}
+
+template <typename R, bool LDS>
+__device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key, int32_t* kb0, int32_t* kb1,
+                              int lane) {
+  const Bodies<R>& b = a.b;
+  const Contacts<R>& c = a.c;
+  const int64_t CS = c.cap;
+  const int64_t maxIter = a.iter_fixed > 0 ? a.iter_fixed : a.iter_mult * int64_t(C);
+
+  // prepareContacts, contact.go:225-229
+  for (int j = lane; j < C; j += 64) {
+    prepare_contact(a, base + j);
+    if (LDS) {
+      key[j] = c.pen[base + j];
+      kb0[j] = c.body0[base + j];
+      kb1[j] = c.body1[base + j];
+    }
+  }
+  __syncthreads();
+
+  // adjustPositions, contact.go:233-283
+  int64_t used = 0;
+  while (used < maxIter) {
+    R best = R(kPositionEpsilon);
+    int bi = INT_MAX;
+    for (int j = lane; j < C; j += 64) {
+      const R p = key[j];
+      if (p > best) {
+        best = p;
+        bi = j;
+      }
+    }
+    wave_argmax(best, bi);
+    if (bi == INT_MAX) break;
+    const int64_t g = base + bi;
+    const int kb[2] = {kb0[bi], kb1[bi]};
+    const V3<R> n = ld3(c.normal, CS, g);
+    const V3<R> rel[2] = {ld3(c.rel, CS, g), ld3(c.rel + 3 * CS, CS, g)};
+    bool aw[2];
+    match_awake(a, kb[0], kb[1], aw[0], aw[1], lane);
+    V3<R> lin[2], ang[2];
+    apply_position_change(a, kb, aw, n, rel, best, lin, ang, lane);
+    for (int j = lane; j < C; j += 64) {
+      const int jb[2] = {kb0[j], kb1[j]};
+      bool any = false;
+#pragma unroll
+      for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) any |= (jb[bb] >= 0 && jb[bb] == kb[d]);
+      if (!any) continue;
+      R p = key[j];
+      const V3<R> nj = ld3(c.normal, CS, base + j);
+#pragma unroll
+      for (int bb = 0; bb < 2; ++bb) {
+        if (jb[bb] < 0) continue;
+        const V3<R> rj = ld3(c.rel + 3 * bb * CS, CS, base + j);
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          if (jb[bb] == kb[d]) {
+            V3<R> dp = add(cross(ang[d], rj), lin[d]);
+            R sign = R(1);
+            if (bb == 0) sign = R(-1);
+            p += dot(dp, nj) * sign;
+          }
+        }
+      }
+      key[j] = p;
+    }
+    __syncthreads();
+    ++used;
+  }
+  const int64_t pos_iters = used;
+  if (LDS) {
+    for (int j = lane; j < C; j += 64) {
+      c.pen[base + j] = key[j];
+      key[j] = c.ddv[base + j];
+    }
+    __syncthreads();
+  }
+  R* dkey = LDS ? key : c.ddv + base;
+
+  // adjustVelocities, contact.go:390-445
+  used = 0;
+  while (used < maxIter) {
+    R best = R(kVelocityEpsilon);
+    int bi = INT_MAX;
+    for (int j = lane; j < C; j += 64) {
+      const R p = dkey[j];
+      if (p > best) {
+        best = p;
+        bi = j;
+      }
+    }
+    wave_argmax(best, bi);
+    if (bi == INT_MAX) break;
+    const int64_t g = base + bi;
+    const int kb[2] = {kb0[bi], kb1[bi]};
+    const V3<R> rel[2] = {ld3(c.rel, CS, g), ld3(c.rel + 3 * CS, CS, g)};
+    R m[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = c.c2w[k * CS + g];
+    const V3<R> cv = ld3(c.cvel, CS, g);
+    bool aw[2];
+    match_awake(a, kb[0], kb[1], aw[0], aw[1], lane);
+    V3<R> vc[2], rc[2];
+    apply_velocity_change(a, kb, rel, m, best, cv, vc, rc, lane);
+    for (int j = lane; j < C; j += 64) {
+      const int jb[2] = {kb0[j], kb1[j]};
+      bool any = false;
+#pragma unroll
+      for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) any |= (jb[bb] >= 0 && jb[bb] == kb[d]);
+      if (!any) continue;
+      const int64_t gj = base + j;
+      R mj[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) mj[k] = c.c2w[k * CS + gj];
+      V3<R> cvj = ld3(c.cvel, CS, gj);
+#pragma unroll
+      for (int bb = 0; bb < 2; ++bb) {
+        if (jb[bb] < 0) continue;
+        const V3<R> rj = ld3(c.rel + 3 * bb * CS, CS, gj);
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          if (jb[bb] == kb[d]) {
+            V3<R> dv = add(cross(rc[d], rj), vc[d]);
+            R sign = R(1);
+            if (bb == 1) sign = R(-1);
+            cvj = add(cvj, mul(m3tv(mj, dv), sign));
+          }
+        }
+      }
+      st3(c.cvel, CS, gj, cvj);
+      const V3<R> nj = ld3(c.normal, CS, gj);
+      dkey[j] = desired_dv(b, jb[0], jb[1], nj, cvj.x, a.dt);
+    }
+    __syncthreads();
+    ++used;
+  }
+  if (LDS) {
+    for (int j = lane; j < C; j += 64) c.ddv[base + j] = dkey[j];
+  }
+  if (lane == 0 && a.world_iters) {
+    a.world_iters[2 * w] = pos_iters;
+    a.world_iters[2 * w + 1] = used;
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(64) k_resolve(RS<R> a) {
+  __shared__ R s_key[kKeyCap];
+  __shared__ int32_t s_b0[kKeyCap];
+  __shared__ int32_t s_b1[kKeyCap];
+  const int lane = threadIdx.x;
+  const int w = a.order ? a.order[blockIdx.x] : int(blockIdx.x);
+  if (w >= a.n_worlds) return;
+  const int C = a.world_cnt[w];
+  if (a.world_iters && lane == 0) {
+    a.world_iters[2 * w] = 0;
+    a.world_iters[2 * w + 1] = 0;
+  }
+  // ResolveContacts' guard, contact.go:210-212
+  if (a.overflow[0] != 0 || C <= 0 || a.dt <= R(0)) return;
+  const int64_t base = a.world_coff[w];
+  if (C <= kKeyCap) {
+    resolve_world<R, true>(a, w, C, base, s_key, s_b0, s_b1, lane);
+  } else {
+    resolve_world<R, false>(a, w, C, base, a.c.pen + base, a.c.body0 + base, a.c.body1 + base, lane);
+  }
+}
+
+template <typename R>
+hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
+  RS<R> a;
+  a.b = L.b;
+  a.c = L.c;
+  a.world_cnt = L.world_cnt;
+  a.world_coff = L.world_coff;
+  a.overflow = L.overflow;
+  a.order = L.order;
+  a.world_iters = L.world_iters;
+  a.n_worlds = L.n_worlds;
+  a.dt = L.dt;
+  a.iter_mult = L.iter_mult;
+  a.iter_fixed = L.iter_fixed;
+  if (a.n_worlds <= 0) return hipSuccess;
+  k_resolve<R><<<a.n_worlds, 64, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+template hipError_t launch_resolve<double>(const ResolveLaunch<double>&, hipStream_t);
+template hipError_t launch_resolve<float>(const ResolveLaunch<float>&, hipStream_t);
+
+}  // namespace cbz

@@ -1,0 +1,167 @@
+"""Batched worlds on one GPU, over the C ABI (include/cubez_amd.h).
+
+``World`` holds many independent scenes (worlds) in HBM and runs the
+reference's step on all of them: Integrate + CalculateDerivedData
+(rigidbody.go:213-272), the ordered-pair narrowphase (colliders.go:720) and
+ResolveContacts (contact.go:208), each phase one launch over every world.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class World:
+    def __init__(self, scene=None, precision="f64", device=0, contact_capacity=0):
+        self._lib = N.load()
+        cfg = N.Config(N.CBZ_F64 if precision == "f64" else N.CBZ_F32, device, int(contact_capacity))
+        h = ctypes.c_void_p()
+        rc = self._lib.cbz_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise N.CubezError(rc, "cbz_create failed (no HIP device, or bad configuration)")
+        self._h = h
+        self.precision = precision
+        self.scene = None
+        if scene is not None:
+            self.load(scene)
+
+    # -- plumbing -----------------------------------------------------------
+    def _chk(self, rc):
+        if rc != 0:
+            msg = self._lib.cbz_last_error(self._h)
+            raise N.CubezError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.cbz_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- scene ---------------------------------------------------------------
+    def load(self, scene, derive=True):
+        """Upload a Scene and (by default) compute the derived data of every
+        body, as the reference's set-up calls CalculateDerivedData."""
+        bo = np.ascontiguousarray(scene.body_off, np.int32)
+        po = np.ascontiguousarray(scene.plane_off, np.int32)
+        pn = np.ascontiguousarray(scene.plane_normal, np.float64).reshape(-1)
+        pof = np.ascontiguousarray(scene.plane_offset, np.float64)
+        self._chk(self._lib.cbz_set_scene(self._h, len(bo) - 1, _ptr(bo), _ptr(po), _ptr(pn), _ptr(pof)))
+        self.scene = scene
+        self.n_worlds = len(bo) - 1
+        self.n_bodies = int(bo[-1])
+        for name, arr in scene.fields.items():
+            self.set(name, arr)
+        if derive:
+            self.calculate_derived()
+
+    def set(self, name, arr):
+        fid, width, kind = N.FIELDS[name]
+        dt = {"f": np.float64, "i": np.int32, "u": np.uint8}[kind]
+        a = np.ascontiguousarray(arr, dtype=dt).reshape(self.n_bodies * width)
+        self._chk(self._lib.cbz_set_field(self._h, fid, _ptr(a)))
+
+    def get(self, name):
+        fid, width, kind = N.FIELDS[name]
+        dt = {"f": np.float64, "i": np.int32, "u": np.uint8}[kind]
+        a = np.empty(self.n_bodies * width, dtype=dt)
+        self._chk(self._lib.cbz_get_field(self._h, fid, _ptr(a)))
+        return a.reshape(self.n_bodies, width) if width > 1 else a
+
+    def state(self, names=("pos", "quat", "vel", "rot", "iit_world", "transform", "ctransform",
+                           "last_acc", "motion", "flags")):
+        return {k: self.get(k) for k in names}
+
+    # -- phases -----------------------------------------------------------------
+    def calculate_derived(self):
+        self._chk(self._lib.cbz_calculate_derived(self._h))
+
+    def integrate(self, dt):
+        self._chk(self._lib.cbz_integrate(self._h, float(dt)))
+
+    def narrowphase(self):
+        t = ctypes.c_int64(0)
+        self._chk(self._lib.cbz_narrowphase(self._h, ctypes.byref(t)))
+        return t.value
+
+    def narrowphase_pairs(self, one, two):
+        one = np.ascontiguousarray(one, np.int32)
+        two = np.ascontiguousarray(two, np.int32)
+        t = ctypes.c_int64(0)
+        self._chk(self._lib.cbz_narrowphase_pairs(self._h, len(one), _ptr(one), _ptr(two), ctypes.byref(t)))
+        return t.value
+
+    def contact_counts(self):
+        c = np.empty(self.n_worlds, np.int64)
+        self._chk(self._lib.cbz_contact_counts(self._h, _ptr(c)))
+        return c
+
+    def contacts(self, world=0, internals=True):
+        n = int(self.contact_counts()[world])
+        out = {
+            "body0": np.empty(n, np.int32), "body1": np.empty(n, np.int32),
+            "point": np.empty((n, 3)), "normal": np.empty((n, 3)), "pen": np.empty(n),
+        }
+        if internals:
+            out.update({"c2w": np.empty((n, 9)), "rel": np.empty((n, 6)), "cvel": np.empty((n, 3)),
+                        "ddv": np.empty(n)})
+        v = N.ContactView(*[out[k].ctypes.data if k in out else None for k in
+                            ("body0", "body1", "point", "normal", "pen", "c2w", "rel", "cvel", "ddv")])
+        self._chk(self._lib.cbz_get_contacts(self._h, int(world), ctypes.byref(v)))
+        return out
+
+    def set_contacts(self, counts, body0, body1, point, normal, pen):
+        counts = np.ascontiguousarray(counts, np.int64)
+        arrs = [np.ascontiguousarray(body0, np.int32), np.ascontiguousarray(body1, np.int32),
+                np.ascontiguousarray(point, np.float64).reshape(-1),
+                np.ascontiguousarray(normal, np.float64).reshape(-1), np.ascontiguousarray(pen, np.float64)]
+        self._chk(self._lib.cbz_set_contacts(self._h, _ptr(counts), *[_ptr(a) for a in arrs]))
+
+    def resolve(self, dt, iter_per_contact=4, iter_fixed=0):
+        self._chk(self._lib.cbz_resolve(self._h, float(dt), int(iter_per_contact), int(iter_fixed)))
+
+    def resolver_iterations(self):
+        it = np.empty(2 * self.n_worlds, np.int64)
+        self._chk(self._lib.cbz_resolver_iterations(self._h, _ptr(it)))
+        return it.reshape(self.n_worlds, 2)
+
+    def step(self, dt, n_steps=1, iter_per_contact=4, counts=False):
+        """n_steps fused steps; with counts=True returns [n_steps, n_worlds]
+        contact counts."""
+        buf = np.empty((n_steps, self.n_worlds), np.int64) if counts else None
+        self._chk(self._lib.cbz_step(self._h, float(dt), int(n_steps), int(iter_per_contact), _ptr(buf)))
+        return buf
+
+    def synchronize(self):
+        self._chk(self._lib.cbz_synchronize(self._h))
+
+    def set_profiling(self, on=True):
+        self._chk(self._lib.cbz_set_profiling(self._h, 1 if on else 0))
+
+    def phase_times(self):
+        ms = np.zeros(4)
+        st = ctypes.c_int64(0)
+        self._chk(self._lib.cbz_phase_times(self._h, _ptr(ms), ctypes.byref(st)))
+        return ms, st.value
+
+    def selftest_f64(self, op, a, b):
+        a = np.ascontiguousarray(a, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        out = np.empty_like(a)
+        self._chk(self._lib.cbz_selftest_f64(self._h, int(op), len(a), _ptr(a), _ptr(b), _ptr(out)))
+        return out

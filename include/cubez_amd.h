@@ -1,0 +1,184 @@
+/*
+ * cubez_amd.h — C ABI of the MI355X implementation of the cubez simulation
+ * step (Integrate + CalculateDerivedData, the narrowphase, ResolveContacts).
+ *
+ * The reference (tbogdala/cubez, pure Go) has no FFI: its boundary is the
+ * exported Go API.  Each entry point below names the Go function(s) whose
+ * behaviour it provides, as file:line of the reference.  A Go package keeps
+ * those names and calls these functions through cgo, one call per phase per
+ * step (or one per N steps via cbz_step); see INTEGRATION.md.
+ *
+ * Conventions
+ *  - Every call returns 0 (CBZ_OK) or a negative CBZ_ERR_* code; the message
+ *    of the last failure is cbz_last_error(ctx).
+ *  - Host buffers are caller-owned and body-major (body i's k components at
+ *    [i*k .. i*k+k)); device buffers are library-owned.  Downloads are
+ *    synchronous; other calls are ordered on the context's HIP stream.
+ *  - A context serves one host thread.
+ *  - Bodies of all worlds are concatenated: world w owns bodies
+ *    [body_off[w], body_off[w+1]) and planes [plane_off[w], plane_off[w+1]).
+ *    Each world is one independent scene: one ResolveContacts call per world.
+ */
+#ifndef CUBEZ_AMD_H
+#define CUBEZ_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CBZ_OK 0
+#define CBZ_ERR_ARG (-1)
+#define CBZ_ERR_HIP (-2)
+#define CBZ_ERR_CAPACITY (-3)
+#define CBZ_ERR_STATE (-4)
+
+/* Arithmetic of the stored state.  CBZ_F64 is the exact path (the Go
+ * reference's float64 Real, math/math.go:23).  CBZ_F32 stores and resolves in
+ * float32 and keeps FP64 for the narrowphase and the contact basis. */
+#define CBZ_F64 0
+#define CBZ_F32 1
+
+/* Shape codes (CollisionCube colliders.go:39, CollisionSphere :57). */
+#define CBZ_CUBE 0
+#define CBZ_SPHERE 1
+
+/* Body flag bits (CBZ_FLAGS field, one byte per body). */
+#define CBZ_FLAG_AWAKE 1     /* RigidBody.IsAwake,  rigidbody.go:62 */
+#define CBZ_FLAG_CAN_SLEEP 2 /* RigidBody.CanSleep, rigidbody.go:67 */
+#define CBZ_FLAG_STATIC 4    /* a body the caller never integrates (examples/ballistic.go:27-44 backboard) */
+
+/* Per-body fields: id, components, element type.  Names follow RigidBody
+ * (rigidbody.go:23-101) and the colliders (colliders.go:29-71). */
+enum cbz_field {
+    CBZ_SHAPE = 0,       /* 1 int32  */
+    CBZ_HALF = 1,        /* 3 double CollisionCube.HalfSize; CollisionSphere.Radius in [0] */
+    CBZ_POS = 2,         /* 3 double Position */
+    CBZ_QUAT = 3,        /* 4 double Orientation (w,x,y,z) */
+    CBZ_VEL = 4,         /* 3 double Velocity */
+    CBZ_ROT = 5,         /* 3 double Rotation (angular velocity) */
+    CBZ_ACC = 6,         /* 3 double Acceleration */
+    CBZ_INV_MASS = 7,    /* 1 double inverseMass (SetMass rigidbody.go:124) */
+    CBZ_IIT_BODY = 8,    /* 9 double InverseInertiaTensor, column-major */
+    CBZ_IIT_WORLD = 9,   /* 9 double inverseInertiaTensorWorld */
+    CBZ_TRANSFORM = 10,  /* 12 double body transform (Matrix3x4, column-major) */
+    CBZ_CTRANSFORM = 11, /* 12 double collider transform */
+    CBZ_OFFSET = 12,     /* 12 double collider Offset */
+    CBZ_LAST_ACC = 13,   /* 3 double lastFrameAccelleration */
+    CBZ_MOTION = 14,     /* 1 double motion */
+    CBZ_LIN_DAMP = 15,   /* 1 double LinearDamping */
+    CBZ_ANG_DAMP = 16,   /* 1 double AngularDamping */
+    CBZ_FLAGS = 17,      /* 1 uint8  CBZ_FLAG_* */
+    CBZ_NUM_FIELDS = 18
+};
+
+typedef struct cbz_config {
+    int32_t precision;        /* CBZ_F64 or CBZ_F32 */
+    int32_t device;           /* HIP device ordinal */
+    int64_t contact_capacity; /* contact slots; 0 = 16 per body + 4096 */
+} cbz_config;
+
+/* Contact readback (Contact, contact.go:17-51).  Any pointer may be NULL.
+ * body0/body1 are global body indices, body1 = -1 for a half-space contact.
+ * The resolver internals (c2w .. ddv) are valid after cbz_resolve/cbz_step. */
+typedef struct cbz_contact_view {
+    int32_t *body0;  /* [n]   Bodies[0] */
+    int32_t *body1;  /* [n]   Bodies[1] */
+    double *point;   /* [n*3] ContactPoint */
+    double *normal;  /* [n*3] ContactNormal */
+    double *pen;     /* [n]   Penetration */
+    double *c2w;     /* [n*9] contactToWorld */
+    double *rel;     /* [n*6] relativeContactPosition[0], [1] */
+    double *cvel;    /* [n*3] contactVelocity */
+    double *ddv;     /* [n]   desiredDeltaVelocity */
+} cbz_contact_view;
+
+typedef struct cbz_context cbz_ctx;
+
+/* --- lifecycle ----------------------------------------------------------- */
+int cbz_create(const cbz_config *cfg, cbz_ctx **out);
+int cbz_destroy(cbz_ctx *ctx);
+const char *cbz_last_error(const cbz_ctx *ctx);
+/* components of a field, or -1 for an unknown id */
+int cbz_field_width(int32_t field);
+
+/* --- scene --------------------------------------------------------------- */
+/* Allocates the state of body_off[n_worlds] bodies and plane_off[n_worlds]
+ * planes.  Every body starts as NewRigidBody() leaves it (rigidbody.go:104-114:
+ * identity orientation, damping 0.95/0.95, Acceleration (0,-9.78,0),
+ * world inverse inertia identity, CanSleep, awake with motion 0.6) inside a
+ * cube collider with identity Offset (NewCollisionCube colliders.go:265-274).
+ * Planes: NewCollisionPlane(normal, offset), colliders.go:81-86. */
+int cbz_set_scene(cbz_ctx *ctx, int32_t n_worlds, const int32_t *body_off, const int32_t *plane_off,
+                  const double *plane_normal, const double *plane_offset);
+/* Upload / download one field for every body (host layout [n_bodies][width]).
+ * Setting a field is assigning the exported Go field (or calling SetMass /
+ * SetInertiaTensor / SetAwake with their results). */
+int cbz_set_field(cbz_ctx *ctx, int32_t field, const void *host);
+int cbz_get_field(cbz_ctx *ctx, int32_t field, void *host);
+int32_t cbz_num_worlds(const cbz_ctx *ctx);
+int64_t cbz_num_bodies(const cbz_ctx *ctx);
+
+/* --- the step, phase by phase -------------------------------------------- */
+/* RigidBody.CalculateDerivedData (rigidbody.go:268-272) followed by the
+ * collider's CalculateDerivedData (colliders.go:173-176, :302-304), for every
+ * body: the set-up step of examples/cubedrop.go:171-172. */
+int cbz_calculate_derived(cbz_ctx *ctx);
+/* RigidBody.Integrate(dt) (rigidbody.go:213-259) then the collider's
+ * CalculateDerivedData, for every non-static body: updateObjects of
+ * examples/cubedrop.go:29-39.  The three math.Pow values (rigidbody.go:233,
+ * 234, 250) are evaluated on the host with the C library's pow(). */
+int cbz_integrate(cbz_ctx *ctx, double dt);
+/* generateContacts of examples/cubedrop.go:42-67 for every world: for each
+ * body i, CheckAgainstHalfSpace against each plane, then
+ * CheckForCollisions(i, j) (colliders.go:720-747) for every j != i, with the
+ * contacts of a world in exactly that order.  *total receives the contact
+ * count over all worlds (may be NULL). */
+int cbz_narrowphase(cbz_ctx *ctx, int64_t *total);
+/* CheckForCollisions(one[k], two[k]) for an explicit ordered pair list (any
+ * caller loop, e.g. examples/ballistic.go:47-96).  Indices are global body
+ * indices; -(p+1) names plane p.  Pairs must be grouped by world in
+ * non-decreasing world order; both operands of a pair in the same world. */
+int cbz_narrowphase_pairs(cbz_ctx *ctx, int64_t n_pairs, const int32_t *one, const int32_t *two,
+                          int64_t *total);
+/* contact count of every world after the last narrowphase ([n_worlds]) */
+int cbz_contact_counts(cbz_ctx *ctx, int64_t *counts);
+/* all contacts of one world, in emission order */
+int cbz_get_contacts(cbz_ctx *ctx, int32_t world, cbz_contact_view *out);
+/* Replace the contact lists (counts[n_worlds], then the records of all
+ * worlds concatenated): contacts built by the caller, as when a Go caller
+ * appends its own *Contact values before ResolveContacts. */
+int cbz_set_contacts(cbz_ctx *ctx, const int64_t *counts, const int32_t *body0, const int32_t *body1,
+                     const double *point, const double *normal, const double *pen);
+/* ResolveContacts(maxIterations, contacts, dt) (contact.go:208-222) on every
+ * world's list, with maxIterations = iter_per_contact * C_w when iter_fixed
+ * is 0 (the examples pass 8*len(contacts), examples/cubedrop.go:73), else
+ * iter_fixed. */
+int cbz_resolve(cbz_ctx *ctx, double dt, int64_t iter_per_contact, int64_t iter_fixed);
+/* iterations the last resolve used per world: [2*n_worlds] = positions,
+ * velocities (the loop counters of contact.go:235 and :392) */
+int cbz_resolver_iterations(cbz_ctx *ctx, int64_t *iters);
+
+/* --- fused stepping --------------------------------------------------------
+ * n_steps of updateCallback (examples/cubedrop.go:69-75): integrate, the
+ * all-pairs narrowphase and ResolveContacts(iter_per_contact * C_w) per
+ * world, with no host synchronisation between steps.  counts (may be NULL)
+ * receives [n_steps][n_worlds] contact counts. */
+int cbz_step(cbz_ctx *ctx, double dt, int32_t n_steps, int64_t iter_per_contact, int64_t *counts);
+int cbz_synchronize(cbz_ctx *ctx);
+
+/* --- measurement ----------------------------------------------------------
+ * When enabled, cbz_step records HIP events around each phase on the
+ * context's stream; cbz_phase_times returns the accumulated milliseconds
+ * [integrate, narrowphase, resolve, whole steps] and the step count, and
+ * resets the accumulators. */
+int cbz_set_profiling(cbz_ctx *ctx, int32_t on);
+int cbz_phase_times(cbz_ctx *ctx, double *ms4, int64_t *steps);
+/* Device FP64 arithmetic self-test (the rounding the exact path relies on):
+ * op 0: a/b, 1: sqrt(a), 2: a*b, 3: a+b, 4: a-b, elementwise over n. */
+int cbz_selftest_f64(cbz_ctx *ctx, int32_t op, int64_t n, const double *a, const double *b, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
