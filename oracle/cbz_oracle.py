@@ -84,7 +84,7 @@ def load():
 
 
 def _p(a):
-    return ctypes.c_void_p(a.ctypes.data)
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 def pow_inputs(lin_damp, ang_damp, dt):
