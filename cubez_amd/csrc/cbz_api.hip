@@ -95,9 +95,11 @@ struct cbz_context {
 
   // profiling
   bool prof = false;
-  hipEvent_t ev[4] = {};
-  double acc_ms[4] = {0, 0, 0, 0};
-  int64_t acc_steps = 0;
+  hipEvent_t ev[8] = {};
+  double acc_ms[6] = {0, 0, 0, 0, 0, 0};
+  int64_t acc_cnt[4] = {0, 0, 0, 0};  // steps, bodies integrated, contacts, resolver iterations
+  uint32_t* d_awake = nullptr;
+  bool count_awake = false;
 };
 
 namespace {
@@ -214,6 +216,7 @@ void free_scene(cbz_ctx* c) {
   dfree(c->overflow);
   dfree(c->world_iters);
   dfree(c->order);
+  dfree(c->d_awake);
   c->l_tiles_cap = c->l_pairs_cap = c->tiles_cap = 0;
   c->have_contacts = false;
 }
@@ -241,6 +244,7 @@ Bodies<R> bodies_view(cbz_ctx* c) {
   b.pow_lin = (const R*)c->pow_lin;
   b.pow_ang = (const R*)c->pow_ang;
   b.flags = (uint8_t*)c->field[CBZ_FLAGS];
+  b.n_awake = c->count_awake ? c->d_awake : nullptr;
   return b;
 }
 
@@ -370,7 +374,7 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles) {
   L.overflow = c->overflow;
   L.n_worlds = c->nw;
   HIPCHK(c, launch_fill_i32(c->overflow, 1, 0, c->stream));
-  HIPCHK(c, launch_narrowphase<R>(L, c->stream));
+  HIPCHK(c, launch_narrowphase<R>(L, c->stream, c->prof ? &c->ev[1] : nullptr));
   c->have_contacts = true;
   return CBZ_OK;
 }
@@ -556,6 +560,7 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc(c, &c->overflow, 1))) return rc;
   if ((rc = dalloc(c, &c->world_iters, 2 * (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->order, (size_t)n_worlds))) return rc;
+  if ((rc = dalloc(c, &c->d_awake, 1))) return rc;
   HIPCHK(c, hipMemset(c->world_cnt, 0, n_worlds * sizeof(int32_t)));
   HIPCHK(c, hipMemset(c->world_coff, 0, n_worlds * sizeof(int64_t)));
   HIPCHK(c, hipMemset(c->world_iters, 0, 2 * n_worlds * sizeof(int64_t)));
@@ -917,34 +922,53 @@ int cbz_step(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contact, i
   int32_t* hist = nullptr;
   if (counts && n_steps > 0) HIPCHK(c, hipMalloc((void**)&hist, (size_t)n_steps * c->nw * sizeof(int32_t)));
   HIPCHK(c, launch_fill_i32(c->overflow, 1, 0, c->stream));
+  // profiling events: ev[0] step start, ev[1..4] narrowphase (before count,
+  // after count, after scans, after emit), ev[5] after resolve
+  std::vector<int64_t> it_host;
+  if (c->prof) it_host.resize(2 * (size_t)c->nw);
+  c->count_awake = c->prof;
   for (int32_t s = 0; s < n_steps && rc == CBZ_OK; ++s) {
-    if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    if (c->prof) {
+      HIPCHK(c, launch_fill_i32((int32_t*)c->d_awake, 1, 0, c->stream));
+      HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    }
     rc = c->precision == CBZ_F64 ? do_integrate<double>(c, dt) : do_integrate<float>(c, dt);
     if (rc) break;
-    if (c->prof) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     rc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, -1) : do_narrowphase<float>(c, -1);
     if (rc) break;
     if (hist)
       HIPCHK(c, hipMemcpyAsync(hist + (size_t)s * c->nw, c->world_cnt, c->nw * sizeof(int32_t),
                                hipMemcpyDeviceToDevice, c->stream));
-    if (c->prof) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    if (c->prof) HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
     rc = c->precision == CBZ_F64 ? do_resolve<double>(c, dt, iter_per_contact, 0)
                                  : do_resolve<float>(c, dt, iter_per_contact, 0);
     if (rc) break;
     if (c->prof) {
-      HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-      HIPCHK(c, hipEventSynchronize(c->ev[3]));
-      float a = 0, b = 0, d = 0;
-      HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-      HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
-      HIPCHK(c, hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
-      c->acc_ms[0] += a;
-      c->acc_ms[1] += b;
-      c->acc_ms[2] += d;
-      c->acc_ms[3] += a + b + d;
-      c->acc_steps += 1;
+      HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+      HIPCHK(c, hipEventSynchronize(c->ev[5]));
+      float t[5] = {0, 0, 0, 0, 0}, whole = 0;
+      HIPCHK(c, hipEventElapsedTime(&t[0], c->ev[0], c->ev[1]));  // integrate (+ fill)
+      HIPCHK(c, hipEventElapsedTime(&t[1], c->ev[1], c->ev[2]));  // count
+      HIPCHK(c, hipEventElapsedTime(&t[2], c->ev[2], c->ev[3]));  // scans
+      HIPCHK(c, hipEventElapsedTime(&t[3], c->ev[3], c->ev[4]));  // emit
+      HIPCHK(c, hipEventElapsedTime(&t[4], c->ev[6], c->ev[5]));  // resolve
+      HIPCHK(c, hipEventElapsedTime(&whole, c->ev[0], c->ev[5]));
+      for (int k = 0; k < 5; ++k) c->acc_ms[k] += t[k];
+      c->acc_ms[5] += whole;
+      uint32_t awake = 0;
+      int64_t tot = 0;
+      HIPCHK(c, hipMemcpy(&awake, c->d_awake, sizeof awake, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(&tot, c->total, sizeof tot, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(it_host.data(), c->world_iters, it_host.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+      int64_t its = 0;
+      for (int64_t v : it_host) its += v;
+      c->acc_cnt[0] += 1;
+      c->acc_cnt[1] += awake;
+      c->acc_cnt[2] += tot;
+      c->acc_cnt[3] += its;
     }
   }
+  c->count_awake = false;
   if (rc == CBZ_OK) rc = check_overflow(c);
   if (rc == CBZ_OK && hist) {
     std::vector<int32_t> t((size_t)n_steps * c->nw);
@@ -969,13 +993,14 @@ int cbz_set_profiling(cbz_ctx* c, int32_t on) {
   return CBZ_OK;
 }
 
-int cbz_phase_times(cbz_ctx* c, double* ms4, int64_t* steps) {
+int cbz_phase_times(cbz_ctx* c, double* ms6, int64_t* counts4) {
   if (!c) return CBZ_ERR_ARG;
-  if (ms4)
-    for (int k = 0; k < 4; ++k) ms4[k] = c->acc_ms[k];
-  if (steps) *steps = c->acc_steps;
+  if (ms6)
+    for (int k = 0; k < 6; ++k) ms6[k] = c->acc_ms[k];
+  if (counts4)
+    for (int k = 0; k < 4; ++k) counts4[k] = c->acc_cnt[k];
   for (double& v : c->acc_ms) v = 0;
-  c->acc_steps = 0;
+  for (int64_t& v : c->acc_cnt) v = 0;
   return CBZ_OK;
 }
 

@@ -44,7 +44,7 @@ hipError_t launch_calculate_derived(const Bodies<R>& b, hipStream_t st);
 template <typename R>
 hipError_t launch_integrate(const Bodies<R>& b, R dt, R bias, hipStream_t st);
 template <typename R>
-hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st);
+hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st, hipEvent_t* ev = nullptr);
 template <typename R>
 hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st);
 

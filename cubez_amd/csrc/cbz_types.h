@@ -49,6 +49,7 @@ struct Bodies {
   const R* pow_lin;   // [1] Pow(LinearDamping, dt), host-evaluated
   const R* pow_ang;   // [1] Pow(AngularDamping, dt), host-evaluated
   uint8_t* flags;     // [1]
+  uint32_t* n_awake;  // optional counter of bodies integrated (profiling)
 };
 
 template <typename R>

@@ -80,9 +80,13 @@ __global__ void __launch_bounds__(256) k_calculate_derived(Bodies<R> b) {
 template <typename R>
 __global__ void __launch_bounds__(256) k_integrate(Bodies<R> b, R dt, R bias) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint8_t f = i < b.n ? b.flags[i] : uint8_t(F_STATIC);
+  if (b.n_awake) {
+    const uint64_t m = __ballot(!(f & F_STATIC) && (f & F_AWAKE));
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(b.n_awake, uint32_t(__popcll(m)));
+  }
   if (i >= b.n) return;
   const int64_t S = b.stride;
-  uint8_t f = b.flags[i];
   if (f & F_STATIC) return;
   R T[12];
   if (f & F_AWAKE) {

@@ -13,7 +13,7 @@
 //            the tile's contact total.  Cube-cube and cube-sphere pairs whose
 //            bounding spheres are far apart are rejected before the SAT: the
 //            bound guarantees that the reference's own test reports no
-//            contact (see sat_reject below), so the result is unchanged.
+//            contact (see far_apart below), so the result is unchanged.
 //   scan_tiles  — one wave per world: exclusive scan of tile totals.
 //   scan_worlds — one block: exclusive scan of world totals.
 //   emit   — one wave per tile with hits: recompute the hit pairs and write
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(256) k_np_emit(NP<R> a) {
 }
 
 template <typename R>
-hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st) {
+hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st, hipEvent_t* ev) {
   NP<R> a;
   a.b = L.b;
   a.pl = L.pl;
@@ -457,15 +457,21 @@ hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st) {
   a.overflow = L.overflow;
   a.n_worlds = L.n_worlds;
   if (a.n_worlds <= 0) return hipSuccess;
+  // ev (optional): recorded before the count, after it, after the scans
+  // and after the emit, for per-kernel timing on this stream.
   const int nb = (a.ps.n_tiles + 3) / 4;
+  if (ev) (void)hipEventRecord(ev[0], st);
   if (nb > 0) k_np_count<R><<<nb, 256, 0, st>>>(a);
+  if (ev) (void)hipEventRecord(ev[1], st);
   k_np_scan_tiles<R><<<a.n_worlds, 64, 0, st>>>(a);
   k_np_scan_worlds<R><<<1, 1024, 0, st>>>(a);
+  if (ev) (void)hipEventRecord(ev[2], st);
   if (nb > 0) k_np_emit<R><<<nb, 256, 0, st>>>(a);
+  if (ev) (void)hipEventRecord(ev[3], st);
   return hipGetLastError();
 }
 
-template hipError_t launch_narrowphase<double>(const NarrowLaunch<double>&, hipStream_t);
-template hipError_t launch_narrowphase<float>(const NarrowLaunch<float>&, hipStream_t);
+template hipError_t launch_narrowphase<double>(const NarrowLaunch<double>&, hipStream_t, hipEvent_t*);
+template hipError_t launch_narrowphase<float>(const NarrowLaunch<float>&, hipStream_t, hipEvent_t*);
 
 }  // namespace cbz

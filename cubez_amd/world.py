@@ -13,7 +13,7 @@ from . import _native as N
 
 
 def _ptr(a):
-    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
 
 class World:
@@ -154,10 +154,13 @@ class World:
         self._chk(self._lib.cbz_set_profiling(self._h, 1 if on else 0))
 
     def phase_times(self):
-        ms = np.zeros(4)
-        st = ctypes.c_int64(0)
-        self._chk(self._lib.cbz_phase_times(self._h, _ptr(ms), ctypes.byref(st)))
-        return ms, st.value
+        """Accumulated ms [integrate, np count, np scans, np emit, resolve,
+        whole step] and counters [steps, bodies integrated, contacts, resolver
+        iterations] of the profiled steps since the last call."""
+        ms = np.zeros(6)
+        cnt = np.zeros(4, np.int64)
+        self._chk(self._lib.cbz_phase_times(self._h, _ptr(ms), _ptr(cnt)))
+        return ms, cnt
 
     def selftest_f64(self, op, a, b):
         a = np.ascontiguousarray(a, np.float64)

@@ -168,12 +168,14 @@ int cbz_step(cbz_ctx *ctx, double dt, int32_t n_steps, int64_t iter_per_contact,
 int cbz_synchronize(cbz_ctx *ctx);
 
 /* --- measurement ----------------------------------------------------------
- * When enabled, cbz_step records HIP events around each phase on the
- * context's stream; cbz_phase_times returns the accumulated milliseconds
- * [integrate, narrowphase, resolve, whole steps] and the step count, and
+ * When enabled, cbz_step records HIP events between its kernels on the
+ * context's stream (and waits for each step); cbz_phase_times returns the
+ * accumulated milliseconds [integrate, narrowphase count, narrowphase scans,
+ * narrowphase emit, resolve, whole steps] and the counters [steps, bodies
+ * integrated (awake at Integrate), contacts, resolver iterations], and
  * resets the accumulators. */
 int cbz_set_profiling(cbz_ctx *ctx, int32_t on);
-int cbz_phase_times(cbz_ctx *ctx, double *ms4, int64_t *steps);
+int cbz_phase_times(cbz_ctx *ctx, double *ms6, int64_t *counts4);
 /* Device FP64 arithmetic self-test (the rounding the exact path relies on):
  * op 0: a/b, 1: sqrt(a), 2: a*b, 3: a+b, 4: a-b, elementwise over n. */
 int cbz_selftest_f64(cbz_ctx *ctx, int32_t op, int64_t n, const double *a, const double *b, double *out);
