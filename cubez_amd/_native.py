@@ -8,7 +8,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcubez_amd.so")
+LIB_PATH = os.environ.get("CUBEZ_AMD_LIB") or os.path.join(_HERE, "libcubez_amd.so")
 
 CBZ_OK = 0
 CBZ_ERR_ARG = -1

@@ -388,6 +388,7 @@ int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed) {
   L.world_coff = c->world_coff;
   L.overflow = c->overflow;
   L.order = c->order;
+  L.body_off = c->d_body_off;
   L.world_iters = c->world_iters;
   L.n_worlds = c->nw;
   L.dt = (R)dt;

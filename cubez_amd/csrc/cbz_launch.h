@@ -32,6 +32,7 @@ struct ResolveLaunch {
   const int64_t* world_coff;
   const int32_t* overflow;
   const int32_t* order;
+  const int32_t* body_off;
   int64_t* world_iters;
   int32_t n_worlds;
   R dt;

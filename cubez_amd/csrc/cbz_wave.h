@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <climits>
+
 namespace cbz {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
@@ -47,6 +49,64 @@ __device__ __forceinline__ void wave_argmax(R& key, int& idx) {
       idx = i2;
     }
   }
+}
+
+}  // namespace cbz
+
+namespace cbz {
+
+// DPP wave reductions (gfx9 DPP: row_shr within 16-lane rows, then
+// row_bcast15 / row_bcast31 across rows).  The inclusive scan leaves the
+// result in lane 63, which readlane broadcasts.  Lanes whose DPP source is
+// outside the row read `ident`.
+namespace dpp {
+constexpr int kShr1 = 0x111, kShr2 = 0x112, kShr4 = 0x114, kShr8 = 0x118;
+constexpr int kBcast15 = 0x142, kBcast31 = 0x143;
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int mov(int ident, int v) {
+  return __builtin_amdgcn_update_dpp(ident, v, CTRL, ROW_MASK, 0xF, false);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double mov(double ident, double v) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(ident), __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(ident), __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float mov(float ident, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(ident), __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+__device__ __forceinline__ int bcast63(int v) { return __builtin_amdgcn_readlane(v, 63); }
+__device__ __forceinline__ double bcast63(double v) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                          __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+__device__ __forceinline__ float bcast63(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+template <typename T, typename OP>
+__device__ __forceinline__ T allreduce(T v, T ident, OP op) {
+  v = op(v, mov<kShr1, 0xF>(ident, v));
+  v = op(v, mov<kShr2, 0xF>(ident, v));
+  v = op(v, mov<kShr4, 0xF>(ident, v));
+  v = op(v, mov<kShr8, 0xF>(ident, v));
+  v = op(v, mov<kBcast15, 0xA>(ident, v));
+  v = op(v, mov<kBcast31, 0xC>(ident, v));
+  return bcast63(v);
+}
+}  // namespace dpp
+
+// Argmax with the reference's tie rule (larger key; equal keys -> lower
+// index), as a max over keys followed by a min over the indices of the
+// lanes holding that max.  Every lane's (key, idx) must already be its own
+// lowest-index maximum; a lane without a candidate carries idx = INT_MAX.
+template <typename R>
+__device__ __forceinline__ void wave_argmax_dpp(R& key, int& idx) {
+  const R m = dpp::allreduce(key, R(-__builtin_inf()), [](R x, R y) { return y > x ? y : x; });
+  const int cand = key == m ? idx : INT_MAX;
+  idx = dpp::allreduce(cand, INT_MAX, [](int x, int y) { return y < x ? y : x; });
+  key = m;
 }
 
 }  // namespace cbz

@@ -78,6 +78,21 @@ __device__ __forceinline__ void put_contact(const Contacts<R>& c, int64_t slot, 
   c.pen[slot] = R(pen);
 }
 
+// GetAxis / HalfSize[i] with a run-time index, re-read from the body arrays
+// (a dynamic index into a register array would put the whole collider in
+// scratch memory).
+template <typename R>
+__device__ __forceinline__ D3 axis_g(const NP<R>& a, int body, int col) {
+  const int c = col > 3 ? 3 : col;
+  const int64_t S = a.b.stride;
+  return D3{double(a.b.ctransform[(3 * c) * S + body]), double(a.b.ctransform[(3 * c + 1) * S + body]),
+            double(a.b.ctransform[(3 * c + 2) * S + body])};
+}
+template <typename R>
+__device__ __forceinline__ double half_g(const NP<R>& a, int body, int k) {
+  return double(a.b.half[k * a.b.stride + body]);
+}
+
 // transformToAxis, colliders.go:762-770
 __device__ __forceinline__ double transform_to_axis(const Col& c, const D3& a) {
   return c.h[0] * dabs(dot(a, axis(c.T, 0))) + c.h[1] * dabs(dot(a, axis(c.T, 1))) +
@@ -86,7 +101,7 @@ __device__ __forceinline__ double transform_to_axis(const Col& c, const D3& a) {
 
 // CollisionSphere.CheckAgainstHalfSpace, colliders.go:180-207
 template <typename R, bool EMIT>
-__device__ int sphere_half_space(const NP<R>& a, const Col& s, int si, const D3& pn, double po, int64_t slot) {
+__device__ __forceinline__ int sphere_half_space(const NP<R>& a, const Col& s, int si, const D3& pn, double po, int64_t slot) {
   D3 positionAxis = axis(s.T, 3);
   double r = s.h[0];
   double distance = dot(pn, positionAxis) - r;
@@ -101,7 +116,7 @@ __device__ int sphere_half_space(const NP<R>& a, const Col& s, int si, const D3&
 // CollisionCube.CheckAgainstHalfSpace, colliders.go:308-366, with
 // intersectCubeAndHalfSpace (:750-760) as the early-out.
 template <typename R, bool EMIT>
-__device__ int cube_half_space(const NP<R>& a, const Col& c, int ci, const D3& pn, double po, int64_t slot) {
+__device__ __forceinline__ int cube_half_space(const NP<R>& a, const Col& c, int ci, const D3& pn, double po, int64_t slot) {
   double projectedRadius = transform_to_axis(c, pn);
   double cubeDistance = dot(pn, axis(c.T, 3)) - projectedRadius;
   if (!(cubeDistance <= po)) return 0;
@@ -125,7 +140,7 @@ __device__ int cube_half_space(const NP<R>& a, const Col& c, int ci, const D3& p
 
 // CollisionSphere.CheckAgainstSphere, colliders.go:216-254
 template <typename R, bool EMIT>
-__device__ int sphere_sphere(const NP<R>& a, const Col& s1, int i1, const Col& s2, int i2, int64_t slot) {
+__device__ __forceinline__ int sphere_sphere(const NP<R>& a, const Col& s1, int i1, const Col& s2, int i2, int64_t slot) {
   D3 p1 = axis(s1.T, 3), p2 = axis(s2.T, 3);
   D3 midline = sub(p1, p2);
   double size = mag(midline);
@@ -140,7 +155,7 @@ __device__ int sphere_sphere(const NP<R>& a, const Col& s1, int i1, const Col& s
 
 // CollisionCube.CheckAgainstSphere, colliders.go:369-441
 template <typename R, bool EMIT>
-__device__ int cube_sphere(const NP<R>& a, const Col& c, int ci, const Col& s, int si, int64_t slot) {
+__device__ __forceinline__ int cube_sphere(const NP<R>& a, const Col& c, int ci, const Col& s, int si, int64_t slot) {
   const double r = s.h[0];
   D3 position = axis(s.T, 3);
   D3 rc = m34_tinv(c.T, position);
@@ -196,10 +211,11 @@ __device__ __forceinline__ bool try_axis(const Col& one, const Col& two, D3 ax, 
   return true;
 }
 
-// fillPointFaceBoxBox, colliders.go:481-517
-__device__ __forceinline__ void point_face(const Col& one, const Col& two, const D3& toCenter, int best, D3& n,
-                                           D3& p) {
-  n = axis(one.T, best);
+// fillPointFaceBoxBox, colliders.go:481-517 (i1 = one's body index)
+template <typename R>
+__device__ __forceinline__ void point_face(const NP<R>& a, const Col& one, int i1, const Col& two,
+                                           const D3& toCenter, int best, D3& n, D3& p) {
+  n = axis_g(a, i1, best);
   if (dot(n, toCenter) > 0) n = mul(n, -1.0);
   D3 v{two.h[0], two.h[1], two.h[2]};
   if (dot(axis(two.T, 0), n) < 0) v.x = -v.x;
@@ -231,7 +247,7 @@ __device__ __forceinline__ D3 contact_point(const D3& pOne, const D3& dOne, doub
 
 // CollisionCube.CheckAgainstCube, colliders.go:576-710
 template <typename R, bool EMIT>
-__device__ int cube_cube(const NP<R>& a, const Col& one, int i1, const Col& two, int i2, int64_t slot) {
+__device__ __forceinline__ int cube_cube(const NP<R>& a, const Col& one, int i1, const Col& two, int i2, int64_t slot) {
   D3 toCenter = sub(axis(two.T, 3), axis(one.T, 3));
   double pen = Lim<double>::max_value;
   int best = 0xffffff;
@@ -246,17 +262,17 @@ __device__ int cube_cube(const NP<R>& a, const Col& one, int i1, const Col& two,
   if (!EMIT) return 1;
   D3 n, p;
   if (best < 3) {
-    point_face(one, two, toCenter, best, n, p);
+    point_face(a, one, i1, two, toCenter, best, n, p);
     put_contact(a.c, slot, i1, i2, p, n, pen);
   } else if (best < 6) {
-    point_face(two, one, mul(toCenter, -1.0), best - 3, n, p);
+    point_face(a, two, i2, one, mul(toCenter, -1.0), best - 3, n, p);
     put_contact(a.c, slot, i2, i1, p, n, pen);
   } else {
     best -= 6;
     const int oneAxisIndex = best / 3;
     const int twoAxisIndex = best % 3;
-    D3 oneAxis = axis(one.T, oneAxisIndex);
-    D3 twoAxis = axis(two.T, twoAxisIndex);
+    D3 oneAxis = axis_g(a, i1, oneAxisIndex);
+    D3 twoAxis = axis_g(a, i2, twoAxisIndex);
     D3 ax = normalize(cross(oneAxis, twoAxis));
     if (dot(ax, toCenter) > 0) ax = mul(ax, -1.0);
     double p1[3] = {one.h[0], one.h[1], one.h[2]};
@@ -278,8 +294,8 @@ __device__ int cube_cube(const NP<R>& a, const Col& one, int i1, const Col& two,
     D3 ptOnTwoEdge = m34v(two.T, D3{p2[0], p2[1], p2[2]});
     const bool useOne = bestSingleAxis > 2;
     // an index past 2 only arises from a NaN state; GetAxis clamps it to 3
-    const double oneSize = oneAxisIndex < 3 ? one.h[oneAxisIndex] : 0.0;
-    const double twoSize = twoAxisIndex < 3 ? two.h[twoAxisIndex] : 0.0;
+    const double oneSize = oneAxisIndex < 3 ? half_g(a, i1, oneAxisIndex) : 0.0;
+    const double twoSize = twoAxisIndex < 3 ? half_g(a, i2, twoAxisIndex) : 0.0;
     D3 cv = contact_point(ptOnOneEdge, oneAxis, oneSize, ptOnTwoEdge, twoAxis, twoSize, useOne);
     put_contact(a.c, slot, i1, i2, cv, ax, pen);
   }
@@ -308,7 +324,7 @@ __device__ __forceinline__ bool far_apart(const NP<R>& a, int i, int j, int si, 
 // CheckForCollisions, colliders.go:720-747, including the plane delegations
 // (:116-125) and the sphere-to-cube delegation (:210-213).
 template <typename R, bool EMIT>
-__device__ int pair_eval(const NP<R>& a, int one, int two, int64_t slot) {
+__device__ __forceinline__ int pair_eval(const NP<R>& a, int one, int two, int64_t slot) {
   if (one < 0 && two < 0) return 0;
   if (one < 0 || two < 0) {
     const int bi = one < 0 ? two : one;
@@ -433,7 +449,9 @@ __global__ void __launch_bounds__(256) k_np_emit(NP<R> a) {
   int one = 0, two = 0, cnt = 0;
   if (hit) {
     pair_of(a.ps, w, q, one, two);
-    cnt = pair_eval<R, false>(a, one, two, 0);
+    // a body pair that hit holds exactly one contact; only a half-space
+    // check can hold several (up to 8 cube vertices)
+    cnt = (one < 0 || two < 0) ? pair_eval<R, false>(a, one, two, 0) : 1;
   }
   const int inc = wave_incl_scan_i32(cnt);
   const int64_t slot = a.world_coff[w] + a.tile_off[t] + (inc - cnt);

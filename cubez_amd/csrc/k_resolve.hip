@@ -32,6 +32,7 @@ struct RS {
   const int64_t* world_coff;
   const int32_t* overflow;
   const int32_t* order;
+  const int32_t* body_off;
   int64_t* world_iters;  // [2 * n_worlds]: position / velocity iterations
   int32_t n_worlds;
   R dt;
@@ -84,16 +85,22 @@ __device__ __forceinline__ R desired_dv(const Bodies<R>& b, int b0, int b1, cons
   return -cvx - restitution * (cvx - vfa);
 }
 
-// calculateContactBasis, contact.go:118-156
+// calculateContactBasis, contact.go:118-156, split in two: the scale
+// s = 1/sqrt(...) (the only square root and division), and the frame built
+// from the normal and s.  Rebuilding the frame from a stored s gives the
+// same bits as the full computation.
 template <typename R>
-__device__ __forceinline__ void contact_basis(const V3<R>& n, R* m) {
+__device__ __forceinline__ R basis_scale(const V3<R>& n) {
+  if (dabs(n.x) > dabs(n.y)) return R(1) / dsqrt(n.z * n.z + n.x * n.x);
+  return R(1) / dsqrt(n.z * n.z + n.y * n.y);
+}
+template <typename R>
+__device__ __forceinline__ void basis_from_scale(const V3<R>& n, R s, R* m) {
   V3<R> ty, tz;
   if (dabs(n.x) > dabs(n.y)) {
-    R s = R(1) / dsqrt(n.z * n.z + n.x * n.x);
     ty = V3<R>{n.z * s, R(0), n.x * -s};
     tz = V3<R>{n.y * ty.x, n.z * ty.x - n.x * ty.z, -n.y * ty.x};
   } else {
-    R s = R(1) / dsqrt(n.z * n.z + n.y * n.y);
     ty = V3<R>{R(0), -n.z * s, n.y * s};
     tz = V3<R>{n.y * ty.z - n.z * ty.y, -n.x * ty.z, n.x * ty.y};
   }
@@ -101,6 +108,10 @@ __device__ __forceinline__ void contact_basis(const V3<R>& n, R* m) {
   m[0] = n.x; m[1] = n.y; m[2] = n.z;
   m[3] = ty.x; m[4] = ty.y; m[5] = ty.z;
   m[6] = tz.x; m[7] = tz.y; m[8] = tz.z;
+}
+template <typename R>
+__device__ __forceinline__ void contact_basis(const V3<R>& n, R* m) {
+  basis_from_scale(n, basis_scale(n), m);
 }
 
 // calculateInternals, contact.go:59-85, for contact g.
@@ -364,7 +375,7 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
         bi = j;
       }
     }
-    wave_argmax(best, bi);
+    wave_argmax_dpp(best, bi);
     if (bi == INT_MAX) break;
     const int64_t g = base + bi;
     const int kb[2] = {kb0[bi], kb1[bi]};
@@ -425,7 +436,7 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
         bi = j;
       }
     }
-    wave_argmax(best, bi);
+    wave_argmax_dpp(best, bi);
     if (bi == INT_MAX) break;
     const int64_t g = base + bi;
     const int kb[2] = {kb0[bi], kb1[bi]};
@@ -481,15 +492,15 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
   }
 }
 
+constexpr int kRegBodies = 256;   // bodies per world for the register-resident kernel
+constexpr int kRegMaxSlots = 16;  // contacts per lane (64 * 16 = 1024 per world)
+
+__device__ __forceinline__ bool fits_registers(int C, int n) { return C <= 64 * kRegMaxSlots && n <= kRegBodies; }
+
+// HBM-resident path (worlds too large for the register kernel): selection
+// keys and body ids in LDS when the world has at most kKeyCap contacts.
 template <typename R>
-__global__ void __launch_bounds__(64) k_resolve(RS<R> a) {
-  __shared__ R s_key[kKeyCap];
-  __shared__ int32_t s_b0[kKeyCap];
-  __shared__ int32_t s_b1[kKeyCap];
-  const int lane = threadIdx.x;
-  const int w = a.order ? a.order[blockIdx.x] : int(blockIdx.x);
-  if (w >= a.n_worlds) return;
-  const int C = a.world_cnt[w];
+__device__ void resolve_hbm(const RS<R>& a, int w, int C, char* lds, int lane) {
   if (a.world_iters && lane == 0) {
     a.world_iters[2 * w] = 0;
     a.world_iters[2 * w + 1] = 0;
@@ -498,9 +509,401 @@ __global__ void __launch_bounds__(64) k_resolve(RS<R> a) {
   if (a.overflow[0] != 0 || C <= 0 || a.dt <= R(0)) return;
   const int64_t base = a.world_coff[w];
   if (C <= kKeyCap) {
+    R* s_key = (R*)lds;
+    int32_t* s_b0 = (int32_t*)(lds + kKeyCap * sizeof(R));
+    int32_t* s_b1 = s_b0 + kKeyCap;
     resolve_world<R, true>(a, w, C, base, s_key, s_b0, s_b1, lane);
   } else {
     resolve_world<R, false>(a, w, C, base, a.c.pen + base, a.c.body0 + base, a.c.body1 + base, lane);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident resolver: worlds with at most 64*SLOTS contacts and
+// kRegBodies bodies.  Lane l holds contacts j = 64*s + l (s < SLOTS) in
+// VGPRs -- selection key, body ids (world-local, packed 16+16 bits), normal,
+// point and (velocity phase) contact velocity -- so the argmax and the
+// shared-body update read no memory.  LDS holds per body the position at
+// prepare time (relativeContactPosition = point - that position, recomputed
+// bit for bit where the reference stores it), lastFrameAccelleration * dt
+// and the flag byte (matchAwakeState writes it).  contactToWorld is rebuilt
+// from the normal where needed (calculateContactBasis reads nothing else).
+// The two bodies a correction moves are read from and written to HBM, one
+// round trip per iteration.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoBody = 0xFFFFu;
+
+template <typename R>
+__device__ __forceinline__ V3<R> lds3(const R* a, int i) {
+  return V3<R>{a[i], a[kRegBodies + i], a[2 * kRegBodies + i]};
+}
+
+// calculateDesiredDeltaVelocity (contact.go:87-112) with the awake flags and
+// lastFrameAccelleration*dt from LDS.
+template <typename R>
+__device__ __forceinline__ R desired_dv_lds(const uint8_t* fl, const R* lad, int l0, int l1, const V3<R>& n,
+                                            R cvx) {
+  R vfa = R(0);
+  if (fl[l0] & F_AWAKE) vfa += dot(lds3(lad, l0), n);
+  if (l1 >= 0 && (fl[l1] & F_AWAKE)) vfa -= dot(lds3(lad, l1), n);
+  R restitution = R(kRestitution);
+  if (dabs(cvx) < R(0.25)) restitution = R(0);
+  return -cvx - restitution * (cvx - vfa);
+}
+
+// calculateLocalVelocity (contact.go:159-182) with lastFrameAccelleration*dt
+// from LDS.
+template <typename R>
+__device__ __forceinline__ V3<R> local_velocity_lds(const Bodies<R>& b, int body, const V3<R>& lad,
+                                                    const V3<R>& rel, const R* c2w) {
+  const int64_t S = b.stride;
+  V3<R> velocity = add(cross(ld3(b.rot, S, body), rel), ld3(b.vel, S, body));
+  V3<R> cv = m3tv(c2w, velocity);
+  V3<R> accv = m3tv(c2w, lad);
+  accv.x = R(0);
+  return add(cv, accv);
+}
+
+__device__ __forceinline__ int id_lo(uint32_t id) { return int(id & 0xFFFFu); }
+__device__ __forceinline__ int id_hi(uint32_t id) { return (id >> 16) == kNoBody ? -1 : int(id >> 16); }
+
+template <typename R, int SLOTS>
+struct RegLds {
+  // the widest variants keep contact points in LDS (in VGPRs they would spill)
+  static constexpr bool PT_LDS = SLOTS > 12;
+  static constexpr int CAP = 64 * SLOTS;
+  static constexpr size_t pos0 = 0;                                   // R[3][kRegBodies]
+  static constexpr size_t lad = pos0 + 3 * kRegBodies * sizeof(R);    // R[3][kRegBodies]
+  static constexpr size_t bs = lad + 3 * kRegBodies * sizeof(R);      // R[CAP] basis scale
+  static constexpr size_t bc = bs + CAP * sizeof(R);                  // R[9] broadcast
+  static constexpr size_t pt = bc + 16 * sizeof(R);                   // R[3][CAP] (PT_LDS)
+  static constexpr size_t flags = pt + (PT_LDS ? 3 * CAP * sizeof(R) : 0);  // uint8[kRegBodies]
+  static constexpr size_t bcid = flags + kRegBodies;                  // uint32
+  static constexpr size_t bytes = bcid + 16;
+};
+
+template <typename R, int SLOTS>
+__device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* lds, int lane) {
+  using L = RegLds<R, SLOTS>;
+  constexpr bool PT_LDS = L::PT_LDS;
+  constexpr int CAP = L::CAP;
+  R* s_pos0 = (R*)(lds + L::pos0);
+  R* s_lad = (R*)(lds + L::lad);
+  R* s_bs = (R*)(lds + L::bs);
+  R* s_bc = (R*)(lds + L::bc);
+  R* s_pt = (R*)(lds + L::pt);
+  uint8_t* s_flags = (uint8_t*)(lds + L::flags);
+  uint32_t& s_bcid = *(uint32_t*)(lds + L::bcid);
+  if (a.world_iters && lane == 0) {
+    a.world_iters[2 * w] = 0;
+    a.world_iters[2 * w + 1] = 0;
+  }
+  // ResolveContacts' guard, contact.go:210-212
+  if (a.overflow[0] != 0 || C <= 0 || a.dt <= R(0)) return;
+
+  const Bodies<R>& b = a.b;
+  const Contacts<R>& c = a.c;
+  const int64_t S = b.stride, CS = c.cap;
+  const int64_t base = a.world_coff[w];
+  const int64_t maxIter = a.iter_fixed > 0 ? a.iter_fixed : a.iter_mult * int64_t(C);
+
+  for (int i = lane; i < n; i += 64) {
+    const int g = b0w + i;
+    const V3<R> p = ld3(b.pos, S, g);
+    const V3<R> la = mul(ld3(b.last_acc, S, g), a.dt);
+    s_pos0[i] = p.x;
+    s_pos0[kRegBodies + i] = p.y;
+    s_pos0[2 * kRegBodies + i] = p.z;
+    s_lad[i] = la.x;
+    s_lad[kRegBodies + i] = la.y;
+    s_lad[2 * kRegBodies + i] = la.z;
+    s_flags[i] = b.flags[g];
+  }
+  __syncthreads();
+
+  R key[SLOTS];
+  uint32_t ids[SLOTS];
+  V3<R> nrm[SLOTS];
+  V3<R> ptr[PT_LDS ? 1 : SLOTS];
+  auto get_pt = [&](int s) -> V3<R> {
+    if constexpr (PT_LDS) {
+      const int j = 64 * s + lane;
+      return V3<R>{s_pt[j], s_pt[CAP + j], s_pt[2 * CAP + j]};
+    } else {
+      return ptr[s];
+    }
+  };
+
+  // prepareContacts / calculateInternals, contact.go:225-229, :59-85
+#pragma unroll
+  for (int s = 0; s < SLOTS; ++s) {
+    const int j = 64 * s + lane;
+    key[s] = R(0);
+    ids[s] = 0xFFFFFFFFu;
+    nrm[s] = V3<R>{R(0), R(0), R(0)};
+    if constexpr (!PT_LDS) ptr[s] = nrm[s];
+    if (j < C) {
+      const int64_t g = base + j;
+      int c0 = c.body0[g], c1 = c.body1[g];
+      V3<R> nn = ld3(c.normal, CS, g);
+      if (c0 < 0) {  // contact.go:61-65
+        nn = mul(nn, R(-1));
+        c0 = c1;
+        c1 = -1;
+        c.body0[g] = c0;
+        c.body1[g] = c1;
+        st3(c.normal, CS, g, nn);
+      }
+      const int l0 = c0 - b0w, l1 = c1 >= 0 ? c1 - b0w : -1;
+      R m[9];
+      const R sc = basis_scale(nn);
+      basis_from_scale(nn, sc, m);
+      s_bs[j] = sc;
+      const V3<R> p = ld3(c.point, CS, g);
+      const V3<R> r0 = sub(p, lds3(s_pos0, l0));
+      V3<R> cv = local_velocity_lds(b, c0, lds3(s_lad, l0), r0, m);
+      V3<R> r1{R(0), R(0), R(0)};
+      if (c1 >= 0) {
+        r1 = sub(p, lds3(s_pos0, l1));
+        cv = sub(cv, local_velocity_lds(b, c1, lds3(s_lad, l1), r1, m));
+      }
+      const R ddv = desired_dv_lds(s_flags, s_lad, l0, l1, nn, cv.x);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) c.c2w[k * CS + g] = m[k];
+      st3(c.rel, CS, g, r0);
+      st3(c.rel + 3 * CS, CS, g, r1);
+      st3(c.cvel, CS, g, cv);
+      c.ddv[g] = ddv;
+      key[s] = c.pen[g];
+      ids[s] = uint32_t(l0) | (uint32_t(l1 >= 0 ? l1 : int(kNoBody)) << 16);
+      nrm[s] = nn;
+      if constexpr (PT_LDS) {
+        s_pt[j] = p.x;
+        s_pt[CAP + j] = p.y;
+        s_pt[2 * CAP + j] = p.z;
+      } else {
+        ptr[s] = p;
+      }
+    }
+  }
+  __syncthreads();
+
+  // adjustPositions, contact.go:233-283
+  int64_t used = 0;
+  while (used < maxIter) {
+    R best = R(kPositionEpsilon);
+    int bi = INT_MAX;
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+      if (64 * s + lane < C && key[s] > best) {
+        best = key[s];
+        bi = 64 * s + lane;
+      }
+    }
+    wave_argmax_dpp(best, bi);
+    if (bi == INT_MAX) break;
+    if (lane == (bi & 63)) {
+#pragma unroll
+      for (int s = 0; s < SLOTS; ++s) {
+        if (s == (bi >> 6)) {
+          const V3<R> q = get_pt(s);
+          s_bc[0] = nrm[s].x; s_bc[1] = nrm[s].y; s_bc[2] = nrm[s].z;
+          s_bc[3] = q.x;      s_bc[4] = q.y;      s_bc[5] = q.z;
+          s_bcid = ids[s];
+        }
+      }
+    }
+    __syncthreads();
+    const V3<R> n{s_bc[0], s_bc[1], s_bc[2]};
+    const V3<R> p{s_bc[3], s_bc[4], s_bc[5]};
+    const uint32_t kid = s_bcid;
+    const int kl[2] = {id_lo(kid), id_hi(kid)};
+    const int kb[2] = {b0w + kl[0], kl[1] >= 0 ? b0w + kl[1] : -1};
+    V3<R> rel[2] = {sub(p, lds3(s_pos0, kl[0])), V3<R>{R(0), R(0), R(0)}};
+    if (kl[1] >= 0) rel[1] = sub(p, lds3(s_pos0, kl[1]));
+    // matchAwakeState, contact.go:185-202
+    bool aw[2] = {(s_flags[kl[0]] & F_AWAKE) != 0, kl[1] >= 0 && (s_flags[kl[1]] & F_AWAKE) != 0};
+    if (kl[1] >= 0 && (aw[0] || aw[1]) && !(aw[0] && aw[1])) {
+      const int t = aw[0] ? kl[1] : kl[0];
+      if (lane == 0) {
+        s_flags[t] = uint8_t(s_flags[t] | F_AWAKE);
+        b.motion[b0w + t] = R(kSleepEpsilon * 2.0);
+      }
+      aw[0] = aw[1] = true;
+    }
+    V3<R> lin[2], ang[2];
+    apply_position_change(a, kb, aw, n, rel, best, lin, ang, lane);
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+      if (64 * s + lane >= C) continue;
+      const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+      bool any = false;
+#pragma unroll
+      for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) any |= (jb[bb] >= 0 && jb[bb] == kl[d]);
+      if (!any) continue;
+      R pen = key[s];
+#pragma unroll
+      for (int bb = 0; bb < 2; ++bb) {
+        if (jb[bb] < 0) continue;
+        const V3<R> rj = sub(get_pt(s), lds3(s_pos0, jb[bb]));
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          if (jb[bb] == kl[d]) {
+            V3<R> dp = add(cross(ang[d], rj), lin[d]);
+            R sign = R(1);
+            if (bb == 0) sign = R(-1);
+            pen += dot(dp, nrm[s]) * sign;
+          }
+        }
+      }
+      key[s] = pen;
+    }
+    __syncthreads();
+    ++used;
+  }
+  const int64_t pos_iters = used;
+
+  // adjustVelocities, contact.go:390-445
+  V3<R> cvl[SLOTS];
+#pragma unroll
+  for (int s = 0; s < SLOTS; ++s) {
+    const int j = 64 * s + lane;
+    cvl[s] = V3<R>{R(0), R(0), R(0)};
+    if (j < C) {
+      c.pen[base + j] = key[s];
+      key[s] = c.ddv[base + j];
+      cvl[s] = ld3(c.cvel, CS, base + j);
+    }
+  }
+  used = 0;
+  while (used < maxIter) {
+    R best = R(kVelocityEpsilon);
+    int bi = INT_MAX;
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+      if (64 * s + lane < C && key[s] > best) {
+        best = key[s];
+        bi = 64 * s + lane;
+      }
+    }
+    wave_argmax_dpp(best, bi);
+    if (bi == INT_MAX) break;
+    if (lane == (bi & 63)) {
+#pragma unroll
+      for (int s = 0; s < SLOTS; ++s) {
+        if (s == (bi >> 6)) {
+          const V3<R> q = get_pt(s);
+          s_bc[0] = nrm[s].x; s_bc[1] = nrm[s].y; s_bc[2] = nrm[s].z;
+          s_bc[3] = q.x;      s_bc[4] = q.y;      s_bc[5] = q.z;
+          s_bc[6] = cvl[s].x; s_bc[7] = cvl[s].y; s_bc[8] = cvl[s].z;
+          s_bcid = ids[s];
+        }
+      }
+    }
+    __syncthreads();
+    const V3<R> n{s_bc[0], s_bc[1], s_bc[2]};
+    const V3<R> p{s_bc[3], s_bc[4], s_bc[5]};
+    const V3<R> cvk{s_bc[6], s_bc[7], s_bc[8]};
+    const uint32_t kid = s_bcid;
+    const int kl[2] = {id_lo(kid), id_hi(kid)};
+    const int kb[2] = {b0w + kl[0], kl[1] >= 0 ? b0w + kl[1] : -1};
+    V3<R> rel[2] = {sub(p, lds3(s_pos0, kl[0])), V3<R>{R(0), R(0), R(0)}};
+    if (kl[1] >= 0) rel[1] = sub(p, lds3(s_pos0, kl[1]));
+    bool aw0 = (s_flags[kl[0]] & F_AWAKE) != 0;
+    bool aw1 = kl[1] >= 0 && (s_flags[kl[1]] & F_AWAKE) != 0;
+    if (kl[1] >= 0 && (aw0 || aw1) && !(aw0 && aw1)) {
+      const int t = aw0 ? kl[1] : kl[0];
+      if (lane == 0) {
+        s_flags[t] = uint8_t(s_flags[t] | F_AWAKE);
+        b.motion[b0w + t] = R(kSleepEpsilon * 2.0);
+      }
+    }
+    __syncthreads();
+    R m[9];
+    basis_from_scale(n, s_bs[bi], m);
+    V3<R> vc[2], rc[2];
+    apply_velocity_change(a, kb, rel, m, best, cvk, vc, rc, lane);
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+      if (64 * s + lane >= C) continue;
+      const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+      bool any = false;
+#pragma unroll
+      for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) any |= (jb[bb] >= 0 && jb[bb] == kl[d]);
+      if (!any) continue;
+      R mj[9];
+      basis_from_scale(nrm[s], s_bs[64 * s + lane], mj);
+      V3<R> cvj = cvl[s];
+#pragma unroll
+      for (int bb = 0; bb < 2; ++bb) {
+        if (jb[bb] < 0) continue;
+        const V3<R> rj = sub(get_pt(s), lds3(s_pos0, jb[bb]));
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          if (jb[bb] == kl[d]) {
+            V3<R> dv = add(cross(rc[d], rj), vc[d]);
+            R sign = R(1);
+            if (bb == 1) sign = R(-1);
+            cvj = add(cvj, mul(m3tv(mj, dv), sign));
+          }
+        }
+      }
+      cvl[s] = cvj;
+      key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x);
+    }
+    __syncthreads();
+    ++used;
+  }
+#pragma unroll
+  for (int s = 0; s < SLOTS; ++s) {
+    const int j = 64 * s + lane;
+    if (j < C) {
+      c.ddv[base + j] = key[s];
+      st3(c.cvel, CS, base + j, cvl[s]);
+    }
+  }
+  for (int i = lane; i < n; i += 64) b.flags[b0w + i] = s_flags[i];
+  if (lane == 0 && a.world_iters) {
+    a.world_iters[2 * w] = pos_iters;
+    a.world_iters[2 * w + 1] = used;
+  }
+}
+
+constexpr size_t cmax(size_t x, size_t y) { return x > y ? x : y; }
+template <typename R>
+constexpr size_t resolve_lds_bytes() {
+  return cmax(cmax(RegLds<R, 16>::bytes, RegLds<R, 12>::bytes),
+              cmax(RegLds<R, 8>::bytes, kKeyCap * (sizeof(R) + 2 * sizeof(int32_t))));
+}
+
+// One launch for every world: each workgroup takes the world order[blockIdx]
+// (larger worlds first) and runs the variant for its contact count.
+template <typename R>
+__global__ void __launch_bounds__(64) k_resolve(RS<R> a) {
+  __shared__ __align__(16) char lds[resolve_lds_bytes<R>()];
+  const int lane = threadIdx.x;
+  const int w = a.order ? a.order[blockIdx.x] : int(blockIdx.x);
+  if (w >= a.n_worlds) return;
+  const int C = a.world_cnt[w];
+  const int b0w = a.body_off[w], n = a.body_off[w + 1] - b0w;
+  if (!fits_registers(C, n)) {
+    resolve_hbm(a, w, C, lds, lane);
+  } else if (C <= 64) {
+    resolve_reg<R, 1>(a, w, C, b0w, n, lds, lane);
+  } else if (C <= 128) {
+    resolve_reg<R, 2>(a, w, C, b0w, n, lds, lane);
+  } else if (C <= 256) {
+    resolve_reg<R, 4>(a, w, C, b0w, n, lds, lane);
+  } else if (C <= 512) {
+    resolve_reg<R, 8>(a, w, C, b0w, n, lds, lane);
+  } else if (C <= 768) {
+    resolve_reg<R, 12>(a, w, C, b0w, n, lds, lane);
+  } else {
+    resolve_reg<R, 16>(a, w, C, b0w, n, lds, lane);
   }
 }
 
@@ -513,6 +916,7 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   a.world_coff = L.world_coff;
   a.overflow = L.overflow;
   a.order = L.order;
+  a.body_off = L.body_off;
   a.world_iters = L.world_iters;
   a.n_worlds = L.n_worlds;
   a.dt = L.dt;

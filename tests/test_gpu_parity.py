@@ -137,3 +137,44 @@ def test_full_step_batched_worlds():
             assert np.array_equal(gcounts[:, wi], oc), f"world {wi} contact counts"
             sl = sc.world_slice(wi)
             assert_state_equal(w, o, sl.start, sl.stop - sl.start)
+
+
+@pytest.mark.parametrize("n,steps", [(200, 400), (256, 400), (300, 240)])
+def test_full_step_large_world(n, steps):
+    """A world whose resting contacts exceed 512 (the widest register class)
+    and one with more than 256 bodies (the HBM-resident resolver)."""
+    sc = scenes.cubedrop(5, n)
+    with World(sc) as w:
+        o = OracleWorld(sc)
+        o.calculate_derived()
+        gcounts = w.step(sc.dt, steps, 4, counts=True)[:, 0]
+        ocounts = o.step(sc.dt, steps, 4)
+        assert np.array_equal(gcounts, ocounts)
+        assert_state_equal(w, o, 0, sc.n_bodies)
+        assert ocounts.max() > 512
+
+
+def test_resolver_iterations_match_oracle():
+    sc = scenes.batched(7, 12, 8, 200)
+    with World(sc) as w:
+        w.step(sc.dt, 150, 4)
+        w.integrate(sc.dt)
+        w.narrowphase()
+        w.resolve(sc.dt, 4)
+        it = w.resolver_iterations()
+        for wi in range(sc.n_worlds):
+            o = OracleWorld(sc, wi)
+            o.calculate_derived()
+            o.step(sc.dt, 150, 4)
+            o.integrate(sc.dt)
+            cs = o.narrowphase()
+            before = dict(o.counters())
+            out = o.resolve(cs, sc.dt, 4 * len(cs))
+            after = o.counters()
+            assert it[wi, 0] == after["pos_iters"] - before["pos_iters"], wi
+            assert it[wi, 1] == after["vel_iters"] - before["vel_iters"], wi
+            gc = w.contacts(wi)
+            for k in ("pen", "cvel", "ddv", "c2w", "rel"):
+                assert same(gc[k], out[k]), (wi, k)
+            sl = sc.world_slice(wi)
+            assert_state_equal(w, o, sl.start, sl.stop - sl.start)
