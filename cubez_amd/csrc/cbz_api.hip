@@ -88,10 +88,15 @@ struct cbz_context {
   int32_t* world_cnt = nullptr;
   int64_t* world_coff = nullptr;
   int64_t* total = nullptr;
-  int32_t* overflow = nullptr;
+  int32_t* overflow = nullptr;  // [0] this narrowphase, [1] sticky since the host cleared it
   int64_t* world_iters = nullptr;
   int32_t* order = nullptr;
   bool have_contacts = false;
+
+  // captured step graph (cbz_step without counts or profiling)
+  hipGraphExec_t gexec = nullptr;
+  double g_dt = 0;
+  int64_t g_mult = -1;
 
   // profiling
   bool prof = false;
@@ -186,7 +191,14 @@ int download_real(cbz_ctx* c, const void* srcbase, int64_t stride, int64_t first
   return CBZ_OK;
 }
 
+void drop_graph(cbz_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+  c->g_mult = -1;
+}
+
 void free_scene(cbz_ctx* c) {
+  drop_graph(c);
   for (auto& f : c->field) dfree(f);
   dfree(c->pow_lin);
   dfree(c->pow_ang);
@@ -401,7 +413,7 @@ int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed) {
 int check_overflow(cbz_ctx* c) {
   int32_t of = 0;
   int64_t tot = 0;
-  HIPCHK(c, hipMemcpyAsync(&of, c->overflow, sizeof of, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&of, c->overflow + 1, sizeof of, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(&tot, c->total, sizeof tot, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (of)
@@ -558,14 +570,14 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc(c, &c->world_cnt, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->world_coff, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->total, 1))) return rc;
-  if ((rc = dalloc(c, &c->overflow, 1))) return rc;
+  if ((rc = dalloc(c, &c->overflow, 2))) return rc;
   if ((rc = dalloc(c, &c->world_iters, 2 * (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->order, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->d_awake, 1))) return rc;
   HIPCHK(c, hipMemset(c->world_cnt, 0, n_worlds * sizeof(int32_t)));
   HIPCHK(c, hipMemset(c->world_coff, 0, n_worlds * sizeof(int64_t)));
   HIPCHK(c, hipMemset(c->world_iters, 0, 2 * n_worlds * sizeof(int64_t)));
-  HIPCHK(c, hipMemset(c->overflow, 0, sizeof(int32_t)));
+  HIPCHK(c, hipMemset(c->overflow, 0, 2 * sizeof(int32_t)));
   HIPCHK(c, hipMemset(c->total, 0, sizeof(int64_t)));
   std::vector<int32_t> ord;
   build_order(c, ord);
@@ -717,6 +729,7 @@ int cbz_narrowphase(cbz_ctx* c, int64_t* total) {
   int rc = need_scene(c);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, launch_fill_i32(c->overflow, 2, 0, c->stream));
   rc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, -1) : do_narrowphase<float>(c, -1);
   if (rc) return rc;
   if ((rc = check_overflow(c))) return rc;
@@ -797,6 +810,7 @@ int cbz_narrowphase_pairs(cbz_ctx* c, int64_t n_pairs, const int32_t* one, const
   HIPCHK(c, hipMemcpy(c->l_world_tile0, wt0.data(), wt0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->l_world_q, wq.data(), wq.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->l_off, loff.data(), loff.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, launch_fill_i32(c->overflow, 2, 0, c->stream));
   rc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, (int32_t)nt) : do_narrowphase<float>(c, (int32_t)nt);
   if (rc) return rc;
   if ((rc = check_overflow(c))) return rc;
@@ -887,7 +901,7 @@ int cbz_set_contacts(cbz_ctx* c, const int64_t* counts, const int32_t* body0, co
   HIPCHK(c, hipMemcpy(c->world_cnt, cnt.data(), cnt.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->world_coff, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->total, &tot, sizeof tot, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemset(c->overflow, 0, sizeof(int32_t)));
+  HIPCHK(c, hipMemset(c->overflow, 0, 2 * sizeof(int32_t)));
   c->have_contacts = true;
   return CBZ_OK;
 }
@@ -920,9 +934,45 @@ int cbz_step(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contact, i
   if (rc) return rc;
   if (n_steps < 0 || iter_per_contact < 0) return fail(c, CBZ_ERR_ARG, "step: bad arguments");
   HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, launch_fill_i32(c->overflow, 2, 0, c->stream));
+  if (!counts && !c->prof && n_steps > 0) {
+    // Replay a captured graph of one step: the host-side inputs (the Pow
+    // values, the bounding radii) are refreshed first, so the captured
+    // launches do no host work and read them from device memory.
+    rc = refresh_pow(c, dt);
+    if (rc == CBZ_OK) rc = refresh_brad(c);
+    if (rc) return rc;
+    if (!c->gexec || c->g_dt != dt || c->g_mult != iter_per_contact) {
+      drop_graph(c);
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int crc = c->precision == CBZ_F64 ? do_integrate<double>(c, dt) : do_integrate<float>(c, dt);
+      if (crc == CBZ_OK)
+        crc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, -1) : do_narrowphase<float>(c, -1);
+      if (crc == CBZ_OK)
+        crc = c->precision == CBZ_F64 ? do_resolve<double>(c, dt, iter_per_contact, 0)
+                                      : do_resolve<float>(c, dt, iter_per_contact, 0);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (crc) {
+        if (g) (void)hipGraphDestroy(g);
+        return crc;
+      }
+      if (e != hipSuccess) return fail(c, CBZ_ERR_HIP, std::string("step: capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(c, CBZ_ERR_HIP, std::string("step: instantiate: ") + hipGetErrorString(e));
+      }
+      c->g_dt = dt;
+      c->g_mult = iter_per_contact;
+    }
+    for (int32_t s = 0; s < n_steps; ++s) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    c->have_contacts = true;
+    return check_overflow(c);
+  }
   int32_t* hist = nullptr;
   if (counts && n_steps > 0) HIPCHK(c, hipMalloc((void**)&hist, (size_t)n_steps * c->nw * sizeof(int32_t)));
-  HIPCHK(c, launch_fill_i32(c->overflow, 1, 0, c->stream));
   // profiling events: ev[0] step start, ev[1..4] narrowphase (before count,
   // after count, after scans, after emit), ev[5] after resolve
   std::vector<int64_t> it_host;

@@ -432,7 +432,10 @@ __global__ void __launch_bounds__(1024) k_np_scan_worlds(NP<R> a) {
   }
   if (tid == 0) {
     a.total[0] = s_carry;
-    if (s_carry > a.c.cap) a.overflow[0] = 1;
+    if (s_carry > a.c.cap) {
+      a.overflow[0] = 1;  // this narrowphase (read by emit and the resolver)
+      a.overflow[1] = 1;  // sticky until the host clears it
+    }
   }
 }
 

@@ -88,26 +88,30 @@ __device__ __forceinline__ R desired_dv(const Bodies<R>& b, int b0, int b1, cons
 // calculateContactBasis, contact.go:118-156, split in two: the scale
 // s = 1/sqrt(...) (the only square root and division), and the frame built
 // from the normal and s.  Rebuilding the frame from a stored s gives the
-// same bits as the full computation.
+// same bits as the full computation.  Both run in FP64 whatever the storage
+// type R (the FP32 variant keeps the basis in FP64; for R = double nothing
+// is converted).
 template <typename R>
-__device__ __forceinline__ R basis_scale(const V3<R>& n) {
-  if (dabs(n.x) > dabs(n.y)) return R(1) / dsqrt(n.z * n.z + n.x * n.x);
-  return R(1) / dsqrt(n.z * n.z + n.y * n.y);
+__device__ __forceinline__ double basis_scale(const V3<R>& nr) {
+  const V3<double> n{double(nr.x), double(nr.y), double(nr.z)};
+  if (dabs(n.x) > dabs(n.y)) return 1.0 / dsqrt(n.z * n.z + n.x * n.x);
+  return 1.0 / dsqrt(n.z * n.z + n.y * n.y);
 }
 template <typename R>
-__device__ __forceinline__ void basis_from_scale(const V3<R>& n, R s, R* m) {
-  V3<R> ty, tz;
+__device__ __forceinline__ void basis_from_scale(const V3<R>& nr, double s, R* m) {
+  const V3<double> n{double(nr.x), double(nr.y), double(nr.z)};
+  V3<double> ty, tz;
   if (dabs(n.x) > dabs(n.y)) {
-    ty = V3<R>{n.z * s, R(0), n.x * -s};
-    tz = V3<R>{n.y * ty.x, n.z * ty.x - n.x * ty.z, -n.y * ty.x};
+    ty = V3<double>{n.z * s, 0.0, n.x * -s};
+    tz = V3<double>{n.y * ty.x, n.z * ty.x - n.x * ty.z, -n.y * ty.x};
   } else {
-    ty = V3<R>{R(0), -n.z * s, n.y * s};
-    tz = V3<R>{n.y * ty.z - n.z * ty.y, -n.x * ty.z, n.x * ty.y};
+    ty = V3<double>{0.0, -n.z * s, n.y * s};
+    tz = V3<double>{n.y * ty.z - n.z * ty.y, -n.x * ty.z, n.x * ty.y};
   }
   // SetComponents(normal, tangentY, tangentZ), math/matrix.go:52-56
-  m[0] = n.x; m[1] = n.y; m[2] = n.z;
-  m[3] = ty.x; m[4] = ty.y; m[5] = ty.z;
-  m[6] = tz.x; m[7] = tz.y; m[8] = tz.z;
+  m[0] = nr.x; m[1] = nr.y; m[2] = nr.z;
+  m[3] = R(ty.x); m[4] = R(ty.y); m[5] = R(ty.z);
+  m[6] = R(tz.x); m[7] = R(tz.y); m[8] = R(tz.z);
 }
 template <typename R>
 __device__ __forceinline__ void contact_basis(const V3<R>& n, R* m) {
@@ -574,8 +578,8 @@ struct RegLds {
   static constexpr int CAP = 64 * SLOTS;
   static constexpr size_t pos0 = 0;                                   // R[3][kRegBodies]
   static constexpr size_t lad = pos0 + 3 * kRegBodies * sizeof(R);    // R[3][kRegBodies]
-  static constexpr size_t bs = lad + 3 * kRegBodies * sizeof(R);      // R[CAP] basis scale
-  static constexpr size_t bc = bs + CAP * sizeof(R);                  // R[9] broadcast
+  static constexpr size_t bs = lad + 3 * kRegBodies * sizeof(R);      // double[CAP] basis scale
+  static constexpr size_t bc = bs + CAP * sizeof(double);             // R[9] broadcast
   static constexpr size_t pt = bc + 16 * sizeof(R);                   // R[3][CAP] (PT_LDS)
   static constexpr size_t flags = pt + (PT_LDS ? 3 * CAP * sizeof(R) : 0);  // uint8[kRegBodies]
   static constexpr size_t bcid = flags + kRegBodies;                  // uint32
@@ -589,7 +593,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
   constexpr int CAP = L::CAP;
   R* s_pos0 = (R*)(lds + L::pos0);
   R* s_lad = (R*)(lds + L::lad);
-  R* s_bs = (R*)(lds + L::bs);
+  double* s_bs = (double*)(lds + L::bs);
   R* s_bc = (R*)(lds + L::bc);
   R* s_pt = (R*)(lds + L::pt);
   uint8_t* s_flags = (uint8_t*)(lds + L::flags);
@@ -656,7 +660,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
       }
       const int l0 = c0 - b0w, l1 = c1 >= 0 ? c1 - b0w : -1;
       R m[9];
-      const R sc = basis_scale(nn);
+      const double sc = basis_scale(nn);
       basis_from_scale(nn, sc, m);
       s_bs[j] = sc;
       const V3<R> p = ld3(c.point, CS, g);

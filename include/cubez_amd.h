@@ -163,7 +163,9 @@ int cbz_resolver_iterations(cbz_ctx *ctx, int64_t *iters);
  * n_steps of updateCallback (examples/cubedrop.go:69-75): integrate, the
  * all-pairs narrowphase and ResolveContacts(iter_per_contact * C_w) per
  * world, with no host synchronisation between steps.  counts (may be NULL)
- * receives [n_steps][n_worlds] contact counts. */
+ * receives [n_steps][n_worlds] contact counts.  Without counts and with
+ * profiling off, the step is captured once as a HIP graph (per dt and
+ * iteration multiplier) and replayed n_steps times. */
 int cbz_step(cbz_ctx *ctx, double dt, int32_t n_steps, int64_t iter_per_contact, int64_t *counts);
 int cbz_synchronize(cbz_ctx *ctx);
 
