@@ -28,12 +28,12 @@ sys.path.insert(0, ROOT)
 METRIC = "body-steps/s, 4096 worlds: surrogate vs exact GPU resolver at ≤1e-3 m error"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes of K1 per body-step (FP64): an integrated body reads 29
-# doubles + its flag byte and writes 50 doubles + the flag byte; a sleeping
-# one reads its flag byte and transform (12) and writes the collider
-# transform (12).
-K1_AWAKE_BYTES = 29 * 8 + 1 + 50 * 8 + 1
-K1_SLEEP_BYTES = 1 + 12 * 8 + 12 * 8
+# Algorithmic bytes of K1 per body-step: an integrated body reads 29 reals
+# (pos 3, quat 4, vel 3, rot 3, acc 3, inv_mass 1, iit_body 9, motion 1, the
+# two Pow factors) + its flag byte and writes 50 reals (pos, quat, vel, rot,
+# last_acc, transform 12, iit_world 9, collider transform 12, motion) + the
+# flag byte; a sleeping body reads its flag byte and transform (12) and
+# writes the collider transform (12).  Reals are 8 bytes (FP64) or 4 (FP32).
 
 
 _SCENE = None
@@ -86,6 +86,38 @@ def cpu_baseline(scene, steps, budget_s):
     }
 
 
+def run_path(World, sc, precision, steps, warmup):
+    """Warm up, then time `steps` steps from the initial state (graph
+    replay), then the same steps again with HIP events between kernels."""
+    w = World(sc, precision=precision)
+    if warmup > 0:
+        w.step(sc.dt, warmup, sc.iter_mult)
+    w.load(sc)
+    w.synchronize()
+    t0 = time.perf_counter()
+    w.step(sc.dt, steps, sc.iter_mult)
+    w.synchronize()
+    wall = time.perf_counter() - t0
+    w.load(sc)
+    w.set_profiling(True)
+    w.step(sc.dt, steps, sc.iter_mult)
+    w.set_profiling(False)
+    ms, cnt = w.phase_times()
+    w.close()
+    sp, integrated, contacts, iters = (int(x) for x in cnt)
+    names = ("integrate", "np_count", "np_scans", "np_emit", "resolve", "step_profiled")
+    return {
+        "value": sc.n_bodies * steps / wall,
+        "ms_per_step": 1000.0 * wall / steps,
+        "kernels": {k: round(ms[i] / sp, 5) for i, k in enumerate(names)},
+        "steps_profiled": sp,
+        "integrated": integrated,
+        "contacts_per_step": contacts / sp,
+        "contacts_per_s": contacts / (ms[5] / 1000.0) if ms[5] > 0 else None,
+        "iters_per_step": iters / sp,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,7 +125,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--precision", choices=("f64", "f32"), default="f64")
+    ap.add_argument("--precision", choices=("f64", "f32"), default="f32",
+                    help="state precision of the timed path (the configuration names FP32)")
+    ap.add_argument("--single", action="store_true", help="skip the other precision's companion run")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle wall time")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -105,49 +139,29 @@ def main():
 
     from cubez_amd import World, scenes
 
-    t_gen = time.perf_counter()
     sc = scenes.batched(args.seed, args.worlds)
-    t_gen = time.perf_counter() - t_gen
     nb = sc.n_bodies
-    w = World(sc, precision=args.precision)
-    if args.warmup > 0:
-        w.step(sc.dt, args.warmup, sc.iter_mult)
-    # timed run: the first K steps of the trajectory
-    w.load(sc)
-    w.synchronize()
-    t0 = time.perf_counter()
-    w.step(sc.dt, args.steps, sc.iter_mult)
-    w.synchronize()
-    wall = time.perf_counter() - t0
-    # same K steps again with HIP events between the kernels
-    w.load(sc)
-    w.set_profiling(True)
-    w.step(sc.dt, args.steps, sc.iter_mult)
-    w.set_profiling(False)
-    ms, cnt = w.phase_times()
-    w.close()
-    steps_p, integrated, contacts, iters = (int(x) for x in cnt)
+    main_path = run_path(World, sc, args.precision, args.steps, args.warmup)
+    other = "f64" if args.precision == "f32" else "f32"
+    side = run_path(World, sc, other, args.steps, args.warmup) if not args.single else None
 
-    ms_per_step = 1000.0 * wall / args.steps
-    value = nb * args.steps / wall
-    kern = {
-        "integrate": ms[0] / steps_p, "np_count": ms[1] / steps_p, "np_scans": ms[2] / steps_p,
-        "np_emit": ms[3] / steps_p, "resolve": ms[4] / steps_p, "step_profiled": ms[5] / steps_p,
-    }
+    kern = main_path["kernels"]
+    rs = 8 if args.precision == "f64" else 4
     # K1 roofline: algorithmic bytes per launch / average launch time
-    n_dyn = nb  # every body of this workload is dynamic
-    k1_bytes = (integrated * K1_AWAKE_BYTES + (n_dyn * steps_p - integrated) * K1_SLEEP_BYTES) / steps_p
-    k1_s = kern["integrate"] / 1000.0
-    achieved = k1_bytes / k1_s / 1e9
-    dominant = max(("integrate", "np_count", "np_scans", "np_emit", "resolve"), key=lambda k: kern[k])
+    k1_awake = 29 * rs + 1 + 50 * rs + 1
+    k1_sleep = 1 + 12 * rs + 12 * rs
+    steps_p = main_path["steps_profiled"]
+    integrated = main_path["integrated"]
+    k1_bytes = (integrated * k1_awake + (nb * steps_p - integrated) * k1_sleep) / steps_p
+    achieved = k1_bytes / (kern["integrate"] / 1000.0) / 1e9
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": main_path["value"],
         "unit": "body-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": main_path["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -156,15 +170,17 @@ def main():
         "config": {
             "workload": f"batched worlds: {args.worlds} worlds, plane + n~U{{8..256}} cubes each "
                         f"({nb} bodies), first {args.steps} steps at dt=1/60, all-pairs narrowphase, "
-                        f"exact resolver with maxIterations=4*C",
+                        f"exact resolver with maxIterations=4*C, "
+                        + ("FP32 state with FP64 SAT and contact basis" if args.precision == "f32"
+                           else "FP64 throughout"),
             "worlds": args.worlds, "bodies": nb, "seed": args.seed, "resolver": "exact",
-            "precision": args.precision,
+            "precision": args.precision, "step": "hipGraph replay of one captured step",
         },
-        "contacts_per_s": contacts / (ms[5] / 1000.0) if ms[5] > 0 else None,
-        "contacts_per_step": contacts / steps_p,
-        "resolver_iterations_per_step": iters / steps_p,
-        "kernels_ms_per_step": {k: round(v, 5) for k, v in kern.items()},
-        "dominant_kernel": dominant,
+        "contacts_per_s": main_path["contacts_per_s"],
+        "contacts_per_step": main_path["contacts_per_step"],
+        "resolver_iterations_per_step": main_path["iters_per_step"],
+        "kernels_ms_per_step": kern,
+        "dominant_kernel": max(("integrate", "np_count", "np_scans", "np_emit", "resolve"), key=lambda k: kern[k]),
         "roofline": {
             "kernel": "k_integrate (K1: Integrate + CalculateDerivedData)",
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -172,6 +188,8 @@ def main():
             "bytes_per_launch": k1_bytes,
         },
     }
+    if side is not None:
+        out["plain_" + other] = {k: side[k] for k in ("value", "ms_per_step", "kernels")}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.steps, args.cpu_budget)
     print(json.dumps(out), flush=True)

@@ -985,6 +985,7 @@ int cbz_step(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contact, i
     }
     rc = c->precision == CBZ_F64 ? do_integrate<double>(c, dt) : do_integrate<float>(c, dt);
     if (rc) break;
+    if (c->prof) HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
     rc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, -1) : do_narrowphase<float>(c, -1);
     if (rc) break;
     if (hist)
@@ -998,7 +999,7 @@ int cbz_step(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contact, i
       HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
       HIPCHK(c, hipEventSynchronize(c->ev[5]));
       float t[5] = {0, 0, 0, 0, 0}, whole = 0;
-      HIPCHK(c, hipEventElapsedTime(&t[0], c->ev[0], c->ev[1]));  // integrate (+ fill)
+      HIPCHK(c, hipEventElapsedTime(&t[0], c->ev[0], c->ev[7]));  // integrate
       HIPCHK(c, hipEventElapsedTime(&t[1], c->ev[1], c->ev[2]));  // count
       HIPCHK(c, hipEventElapsedTime(&t[2], c->ev[2], c->ev[3]));  // scans
       HIPCHK(c, hipEventElapsedTime(&t[3], c->ev[3], c->ev[4]));  // emit
