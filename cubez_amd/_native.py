@@ -54,6 +54,9 @@ EXPORTS = (
     "cbz_contact_counts", "cbz_get_contacts", "cbz_set_contacts", "cbz_resolve",
     "cbz_resolver_iterations", "cbz_step", "cbz_synchronize", "cbz_set_profiling",
     "cbz_phase_times", "cbz_selftest_f64",
+    "cbz_sur_num_params", "cbz_sur_set_params", "cbz_sur_get_params", "cbz_sur_set_norm", "cbz_sur_get_norm",
+    "cbz_sur_reserve", "cbz_sur_collect", "cbz_sur_dataset", "cbz_sur_load_dataset", "cbz_sur_init",
+    "cbz_sur_train", "cbz_sur_grad", "cbz_sur_predict", "cbz_sur_resolve", "cbz_sur_world_mask", "cbz_step_sur",
 )
 
 
@@ -112,6 +115,22 @@ def load():
         "cbz_set_profiling": ([P, i32], i32),
         "cbz_phase_times": ([P, P, P], i32),
         "cbz_selftest_f64": ([P, i32, i64, P, P, P], i32),
+        "cbz_sur_num_params": ([], i64),
+        "cbz_sur_set_params": ([P, P], i32),
+        "cbz_sur_get_params": ([P, P], i32),
+        "cbz_sur_set_norm": ([P, P], i32),
+        "cbz_sur_get_norm": ([P, P], i32),
+        "cbz_sur_reserve": ([P, i64], i32),
+        "cbz_sur_collect": ([P, f64, i64, P], i32),
+        "cbz_sur_dataset": ([P, P, P, P], i32),
+        "cbz_sur_load_dataset": ([P, i64, P, P], i32),
+        "cbz_sur_init": ([P, ctypes.c_uint64], i32),
+        "cbz_sur_train": ([P, i32, i32, ctypes.c_float, ctypes.c_uint64, P], i32),
+        "cbz_sur_grad": ([P, i32, P, P, P, P], i32),
+        "cbz_sur_predict": ([P, i32, P, P], i32),
+        "cbz_sur_resolve": ([P, f64, i64, f64, P], i32),
+        "cbz_sur_world_mask": ([P, P], i32),
+        "cbz_step_sur": ([P, f64, i32, i64, f64, P, P], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "cbz_launch.h"
+#include "cbz_mlp.h"
 #include "cbz_types.h"
 #include "cubez_amd.h"
 
@@ -93,6 +94,46 @@ struct cbz_context {
   int32_t* order = nullptr;
   bool have_contacts = false;
 
+  // resolver surrogate (see cbz_sur_*)
+  struct Sur {
+    bool have_model = false;
+    float* params = nullptr;  // FP32 master weights: per layer W [N x K] then b [N]
+    float* grad = nullptr;
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    int64_t adam_t = 0;
+    __bf16* Wnk[cbz::kLayers] = {};  // [Npad x Kpad] forward operand
+    __bf16* Wkn[cbz::kLayers] = {};  // [Kpad x Npad] backward operand
+    float* norm = nullptr;  // in_mean 40, in_inv_std 40, out_mean 12, out_std 12, out_inv_std 12
+    // dataset of (raw features, labels)
+    float* dsX = nullptr;
+    float* dsY = nullptr;
+    int64_t ds_rows = 0, ds_cap = 0;
+    // inference buffers
+    int64_t inf_cap = 0;
+    __bf16* X = nullptr;
+    __bf16* H[2] = {};
+    float* Yp = nullptr;
+    int32_t* deg = nullptr;
+    uint8_t* mask = nullptr;
+    int32_t* nfall = nullptr;
+    // training buffers
+    int batch_cap = 0;
+    int64_t* idx = nullptr;
+    __bf16* bX = nullptr;
+    float* bT = nullptr;
+    float* Z[cbz::kLayers] = {};
+    __bf16* Hb[cbz::kLayers - 1] = {};
+    __bf16* At[cbz::kLayers] = {};  // transposed layer inputs: X^T, H0^T, H1^T, H2^T
+    float* Yout = nullptr;
+    __bf16* dZ[cbz::kLayers] = {};
+    __bf16* dZt[cbz::kLayers] = {};
+    float* dA = nullptr;
+    float* row_loss = nullptr;
+    float* losses = nullptr;
+    int losses_cap = 0;
+  } sur;
+
   // captured step graph (cbz_step without counts or profiling)
   hipGraphExec_t gexec = nullptr;
   double g_dt = 0;
@@ -105,6 +146,7 @@ struct cbz_context {
   int64_t acc_cnt[4] = {0, 0, 0, 0};  // steps, bodies integrated, contacts, resolver iterations
   uint32_t* d_awake = nullptr;
   bool count_awake = false;
+  void* pos0 = nullptr;  // body positions at the last prepareContacts
 };
 
 namespace {
@@ -197,8 +239,21 @@ void drop_graph(cbz_ctx* c) {
   c->g_mult = -1;
 }
 
+void free_sur_buffers(cbz_ctx* c) {
+  auto& u = c->sur;
+  dfree(u.X);
+  dfree(u.H[0]);
+  dfree(u.H[1]);
+  dfree(u.Yp);
+  dfree(u.deg);
+  dfree(u.mask);
+  dfree(u.nfall);
+  u.inf_cap = 0;
+}
+
 void free_scene(cbz_ctx* c) {
   drop_graph(c);
+  free_sur_buffers(c);
   for (auto& f : c->field) dfree(f);
   dfree(c->pow_lin);
   dfree(c->pow_ang);
@@ -229,6 +284,7 @@ void free_scene(cbz_ctx* c) {
   dfree(c->world_iters);
   dfree(c->order);
   dfree(c->d_awake);
+  dfree(c->pos0);
   c->l_tiles_cap = c->l_pairs_cap = c->tiles_cap = 0;
   c->have_contacts = false;
 }
@@ -392,8 +448,13 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles) {
 }
 
 template <typename R>
-int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed) {
+int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode = 3,
+               const uint8_t* mask = nullptr, float* labels = nullptr) {
   ResolveLaunch<R> L;
+  L.mode = mode;
+  L.world_mask = mask;
+  L.labels = labels;
+  L.pos0 = (R*)c->pos0;
   L.b = bodies_view<R>(c);
   L.c = contacts_view<R>(c);
   L.world_cnt = c->world_cnt;
@@ -431,6 +492,261 @@ void build_order(cbz_ctx* c, std::vector<int32_t>& ord) {
   std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
     return (c->body_off[a + 1] - c->body_off[a]) > (c->body_off[b + 1] - c->body_off[b]);
   });
+}
+
+
+// ----------------------------------------------------------------------
+// resolver surrogate: host orchestration
+// ----------------------------------------------------------------------
+struct LayerOff {
+  int64_t w[kLayers], b[kLayers], total;
+};
+LayerOff layer_offsets() {
+  LayerOff o{};
+  int64_t off = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    o.w[l] = off;
+    off += int64_t(layer_out(l)) * layer_in(l);
+    o.b[l] = off;
+    off += layer_out(l);
+  }
+  o.total = off;
+  return o;
+}
+constexpr int kNorm = kFeat + kFeat + kOut + kOut + kOut;
+int64_t round32(int64_t n) { return (n + 31) / 32 * 32; }
+
+int sur_alloc_model(cbz_ctx* c) {
+  auto& u = c->sur;
+  if (u.params) return CBZ_OK;
+  const LayerOff o = layer_offsets();
+  int rc;
+  if ((rc = dalloc(c, &u.params, (size_t)o.total))) return rc;
+  if ((rc = dalloc(c, &u.grad, (size_t)o.total))) return rc;
+  if ((rc = dalloc(c, &u.adam_m, (size_t)o.total))) return rc;
+  if ((rc = dalloc(c, &u.adam_v, (size_t)o.total))) return rc;
+  if ((rc = dalloc(c, &u.norm, (size_t)kNorm))) return rc;
+  HIPCHK(c, hipMemset(u.adam_m, 0, o.total * sizeof(float)));
+  HIPCHK(c, hipMemset(u.adam_v, 0, o.total * sizeof(float)));
+  HIPCHK(c, hipMemset(u.grad, 0, o.total * sizeof(float)));
+  for (int l = 0; l < kLayers; ++l) {
+    const size_t sz = (size_t)layer_out_pad(l) * layer_in_pad(l);
+    if ((rc = dalloc(c, &u.Wnk[l], sz))) return rc;
+    if ((rc = dalloc(c, &u.Wkn[l], sz))) return rc;
+    HIPCHK(c, hipMemset(u.Wnk[l], 0, sz * sizeof(__bf16)));
+    HIPCHK(c, hipMemset(u.Wkn[l], 0, sz * sizeof(__bf16)));
+  }
+  return CBZ_OK;
+}
+
+void sur_free_model(cbz_ctx* c) {
+  auto& u = c->sur;
+  dfree(u.params);
+  dfree(u.grad);
+  dfree(u.adam_m);
+  dfree(u.adam_v);
+  dfree(u.norm);
+  for (int l = 0; l < kLayers; ++l) {
+    dfree(u.Wnk[l]);
+    dfree(u.Wkn[l]);
+  }
+  dfree(u.dsX);
+  dfree(u.dsY);
+  dfree(u.idx);
+  dfree(u.bX);
+  dfree(u.bT);
+  for (auto& p : u.Z) dfree(p);
+  for (auto& p : u.Hb) dfree(p);
+  for (auto& p : u.At) dfree(p);
+  for (auto& p : u.dZ) dfree(p);
+  for (auto& p : u.dZt) dfree(p);
+  dfree(u.Yout);
+  dfree(u.dA);
+  dfree(u.row_loss);
+  dfree(u.losses);
+  u = cbz_context::Sur{};
+}
+
+int sur_pack(cbz_ctx* c) {
+  auto& u = c->sur;
+  const LayerOff o = layer_offsets();
+  for (int l = 0; l < kLayers; ++l)
+    HIPCHK(c, launch_pack_weights(u.params + o.w[l], layer_out(l), layer_in(l), u.Wnk[l], layer_in_pad(l), u.Wkn[l],
+                                  layer_out_pad(l), c->stream));
+  return CBZ_OK;
+}
+
+// Forward pass over M rows (Mpad = M rounded up to 32) of BF16 inputs X.
+// train: keep pre-activations, activations and their transposes for the
+// backward pass and write normalised outputs to Yout; else write
+// denormalised predictions to Yp.
+int sur_forward(cbz_ctx* c, const __bf16* X, int64_t M, int64_t Mpad, bool train) {
+  auto& u = c->sur;
+  const LayerOff o = layer_offsets();
+  const __bf16* in = X;
+  for (int l = 0; l < kLayers; ++l) {
+    GemmArgs g{};
+    g.M = (int)Mpad;
+    g.N = layer_out_pad(l);
+    g.K = layer_in_pad(l);
+    g.A = in;
+    g.lda = layer_in_pad(l);
+    g.Bt = u.Wnk[l];
+    g.ldb = layer_in_pad(l);
+    g.bias = u.params + o.b[l];
+    g.act = l < kLayers - 1;
+    g.m_valid = (int)M;
+    g.n_valid = layer_out(l);
+    if (l < kLayers - 1) {
+      __bf16* out = train ? u.Hb[l] : u.H[l % 2];
+      g.H = out;
+      g.ldh = layer_out_pad(l);
+      if (train) {
+        g.Z = u.Z[l];
+        g.ldz = layer_out_pad(l);
+        g.Ht = u.At[l + 1];
+        g.ldht = Mpad;
+      }
+      in = out;
+    } else if (train) {
+      g.Y = u.Yout;
+      g.ldy = kOut;
+    } else {
+      g.Y = u.Yp;
+      g.ldy = kOut;
+      g.y_scale = u.norm + 2 * kFeat + kOut;  // out_std
+      g.y_shift = u.norm + 2 * kFeat;         // out_mean
+    }
+    HIPCHK(c, launch_gemm(g, c->stream));
+  }
+  return CBZ_OK;
+}
+
+// Backward pass of the mean-squared error against bT; gradients into grad.
+int sur_backward(cbz_ctx* c, int64_t M, int64_t Mpad) {
+  auto& u = c->sur;
+  const LayerOff o = layer_offsets();
+  HIPCHK(c, launch_mse(u.Yout, u.bT, (int)Mpad, (int)M, u.row_loss, u.dZ[kLayers - 1], u.dZt[kLayers - 1],
+                       (int)Mpad, c->stream));
+  for (int l = kLayers - 1; l >= 0; --l) {
+    const int N = layer_out(l), K = layer_in(l), Np = layer_out_pad(l), Kp = layer_in_pad(l);
+    // dW = dZ^T * A_in : [Np x Mpad] * [Mpad x Kp]
+    GemmArgs g{};
+    g.M = Np;
+    g.N = Kp;
+    g.K = (int)Mpad;
+    g.A = u.dZt[l];
+    g.lda = Mpad;
+    g.Bt = u.At[l];
+    g.ldb = Mpad;
+    g.Y = u.grad + o.w[l];
+    g.ldy = K;
+    g.m_valid = N;
+    g.n_valid = K;
+    HIPCHK(c, launch_gemm(g, c->stream));
+    HIPCHK(c, launch_colsum(u.dZ[l], (int)Mpad, N, Np, u.grad + o.b[l], c->stream));
+    if (l > 0) {
+      // dA_in = dZ * W : [Mpad x Np] * [Np x Kp], B given as Wkn [Kp x Np]
+      GemmArgs h{};
+      h.M = (int)Mpad;
+      h.N = Kp;
+      h.K = Np;
+      h.A = u.dZ[l];
+      h.lda = Np;
+      h.Bt = u.Wkn[l];
+      h.ldb = Np;
+      h.Y = u.dA;
+      h.ldy = Kp;
+      h.m_valid = (int)M;
+      h.n_valid = Kp;
+      HIPCHK(c, launch_gemm(h, c->stream));
+      HIPCHK(c, launch_silu_bwd(u.dA, u.Z[l - 1], (int)Mpad, (int)M, Kp, Kp, u.dZ[l - 1], u.dZt[l - 1], (int)Mpad,
+                                c->stream));
+    }
+  }
+  return CBZ_OK;
+}
+
+int sur_ensure_batch(cbz_ctx* c, int rows) {
+  auto& u = c->sur;
+  const int Mpad = (int)round32(rows);
+  if (Mpad <= u.batch_cap) return CBZ_OK;
+  int rc;
+  if ((rc = dalloc(c, &u.idx, (size_t)Mpad))) return rc;
+  if ((rc = dalloc(c, &u.bX, (size_t)Mpad * kFeatPad))) return rc;
+  if ((rc = dalloc(c, &u.bT, (size_t)Mpad * kOut))) return rc;
+  if ((rc = dalloc(c, &u.At[0], (size_t)Mpad * kFeatPad))) return rc;
+  for (int l = 0; l < kLayers; ++l) {
+    const size_t w = (size_t)Mpad * layer_out_pad(l);
+    if ((rc = dalloc(c, &u.Z[l], w))) return rc;
+    if ((rc = dalloc(c, &u.dZ[l], w))) return rc;
+    if ((rc = dalloc(c, &u.dZt[l], w))) return rc;
+    if (l < kLayers - 1) {
+      if ((rc = dalloc(c, &u.Hb[l], w))) return rc;
+      if ((rc = dalloc(c, &u.At[l + 1], w))) return rc;
+    }
+  }
+  if ((rc = dalloc(c, &u.Yout, (size_t)Mpad * kOut))) return rc;
+  if ((rc = dalloc(c, &u.dA, (size_t)Mpad * kHidden))) return rc;
+  if ((rc = dalloc(c, &u.row_loss, (size_t)Mpad))) return rc;
+  u.batch_cap = Mpad;
+  return CBZ_OK;
+}
+
+int sur_ensure_inference(cbz_ctx* c, int64_t rows) {
+  auto& u = c->sur;
+  int64_t Mpad = round32(std::max<int64_t>(rows, 32));
+  if (Mpad <= u.inf_cap && u.mask) return CBZ_OK;
+  Mpad = std::max(Mpad, u.inf_cap + u.inf_cap / 4);  // grow geometrically
+  int rc;
+  if ((rc = dalloc(c, &u.X, (size_t)Mpad * kFeatPad))) return rc;
+  if ((rc = dalloc(c, &u.H[0], (size_t)Mpad * kHidden))) return rc;
+  if ((rc = dalloc(c, &u.H[1], (size_t)Mpad * kHidden))) return rc;
+  if ((rc = dalloc(c, &u.Yp, (size_t)Mpad * kOut))) return rc;
+  if ((rc = dalloc(c, &u.deg, (size_t)std::max<int64_t>(c->nb, 1)))) return rc;
+  if ((rc = dalloc(c, &u.mask, (size_t)std::max(c->nw, 1)))) return rc;
+  if ((rc = dalloc(c, &u.nfall, 1))) return rc;
+  u.inf_cap = Mpad;
+  return CBZ_OK;
+}
+
+// One resolve with the surrogate: exact positions, predicted velocity phase
+// per world with its residual check, exact velocity phase where it fails.
+template <typename R>
+int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int32_t* nfall_dst) {
+  auto& u = c->sur;
+  int rc = do_resolve<R>(c, dt, mult, 0, RM_POS);
+  if (rc) return rc;
+  const int64_t cap_rows = c->cap;  // contacts never exceed the slots
+  // the GEMM grid follows the contact count, read once per resolve
+  int64_t tot = 0;
+  HIPCHK(c, hipMemcpyAsync(&tot, c->total, sizeof tot, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  tot = std::min(std::min(tot, c->cap), cap_rows);
+  if ((rc = sur_ensure_inference(c, tot))) return rc;
+  const int64_t Mpad = round32(std::max<int64_t>(tot, 1));
+  // rows tot..Mpad-1 of X are padding: GEMM rows are independent and those
+  // outputs are never read
+  HIPCHK(c, launch_features<R>(bodies_view<R>(c), contacts_view<R>(c), tot, u.deg, u.norm, u.norm + kFeat, u.X,
+                               nullptr, c->stream));
+  if ((rc = sur_forward(c, u.X, tot, Mpad, false))) return rc;
+  SurApply<R> a;
+  a.b = bodies_view<R>(c);
+  a.c = contacts_view<R>(c);
+  a.world_cnt = c->world_cnt;
+  a.world_coff = c->world_coff;
+  a.body_off = c->d_body_off;
+  a.overflow = c->overflow;
+  a.Y = u.Yp;
+  a.mask = u.mask;
+  a.dt = (R)dt;
+  a.tol = (float)tol;
+  a.max_contacts = kSurContacts;
+  HIPCHK(c, launch_sur_apply<R>(a, c->nw, c->stream));
+  rc = do_resolve<R>(c, dt, mult, 0, RM_VEL, u.mask);
+  if (rc) return rc;
+  if (nfall_dst) HIPCHK(c, launch_count_u8(u.mask, c->nw, nfall_dst, c->stream));
+  return CBZ_OK;
 }
 
 }  // namespace
@@ -482,6 +798,7 @@ int cbz_destroy(cbz_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_scene(c);
+  sur_free_model(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -574,6 +891,7 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc(c, &c->world_iters, 2 * (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->order, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->d_awake, 1))) return rc;
+  if ((rc = dalloc_bytes(c, &c->pos0, rs * 3 * (size_t)std::max<int64_t>(nb, 1)))) return rc;
   HIPCHK(c, hipMemset(c->world_cnt, 0, n_worlds * sizeof(int32_t)));
   HIPCHK(c, hipMemset(c->world_coff, 0, n_worlds * sizeof(int64_t)));
   HIPCHK(c, hipMemset(c->world_iters, 0, 2 * n_worlds * sizeof(int64_t)));
@@ -1074,6 +1392,363 @@ int cbz_selftest_f64(cbz_ctx* c, int32_t op, int64_t n, const double* a, const d
   (void)hipFree(dout);
   if (e != hipSuccess) return fail(c, CBZ_ERR_HIP, std::string("selftest: ") + hipGetErrorString(e));
   return CBZ_OK;
+}
+
+// --- resolver surrogate -------------------------------------------------
+int64_t cbz_sur_num_params(void) { return layer_offsets().total; }
+
+int cbz_sur_set_params(cbz_ctx* c, const float* p) {
+  if (!c || !p) return fail(c, CBZ_ERR_ARG, "sur_set_params: null");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = sur_alloc_model(c);
+  if (rc) return rc;
+  const LayerOff o = layer_offsets();
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->sur.params, p, o.total * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemset(c->sur.adam_m, 0, o.total * sizeof(float)));
+  HIPCHK(c, hipMemset(c->sur.adam_v, 0, o.total * sizeof(float)));
+  c->sur.adam_t = 0;
+  if ((rc = sur_pack(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->sur.have_model = true;
+  return CBZ_OK;
+}
+
+int cbz_sur_get_params(cbz_ctx* c, float* p) {
+  if (!c || !p) return fail(c, CBZ_ERR_ARG, "sur_get_params: null");
+  if (!c->sur.params) return fail(c, CBZ_ERR_STATE, "sur_get_params: no model");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(p, c->sur.params, layer_offsets().total * sizeof(float), hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+int cbz_sur_set_norm(cbz_ctx* c, const float* n) {
+  if (!c || !n) return fail(c, CBZ_ERR_ARG, "sur_set_norm: null");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = sur_alloc_model(c);
+  if (rc) return rc;
+  std::vector<float> h(kNorm);
+  for (int k = 0; k < kFeat; ++k) {
+    h[k] = n[k];
+    h[kFeat + k] = 1.0f / std::max(n[kFeat + k], 1e-6f);
+  }
+  for (int k = 0; k < kOut; ++k) {
+    h[2 * kFeat + k] = n[2 * kFeat + k];
+    h[2 * kFeat + kOut + k] = std::max(n[2 * kFeat + kOut + k], 1e-6f);
+    h[2 * kFeat + 2 * kOut + k] = 1.0f / h[2 * kFeat + kOut + k];
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->sur.norm, h.data(), kNorm * sizeof(float), hipMemcpyHostToDevice));
+  return CBZ_OK;
+}
+
+int cbz_sur_get_norm(cbz_ctx* c, float* n) {
+  if (!c || !n) return fail(c, CBZ_ERR_ARG, "sur_get_norm: null");
+  if (!c->sur.norm) return fail(c, CBZ_ERR_STATE, "sur_get_norm: no model");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<float> h(kNorm);
+  HIPCHK(c, hipMemcpy(h.data(), c->sur.norm, kNorm * sizeof(float), hipMemcpyDeviceToHost));
+  for (int k = 0; k < kFeat; ++k) {
+    n[k] = h[k];
+    n[kFeat + k] = 1.0f / h[kFeat + k];
+  }
+  for (int k = 0; k < kOut; ++k) {
+    n[2 * kFeat + k] = h[2 * kFeat + k];
+    n[2 * kFeat + kOut + k] = h[2 * kFeat + kOut + k];
+  }
+  return CBZ_OK;
+}
+
+int cbz_sur_reserve(cbz_ctx* c, int64_t rows) {
+  if (!c || rows < 0) return fail(c, CBZ_ERR_ARG, "sur_reserve: bad arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  auto& u = c->sur;
+  if (rows <= u.ds_cap) return CBZ_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float *x = nullptr, *y = nullptr;
+  HIPCHK(c, hipMalloc((void**)&x, (size_t)rows * kFeat * sizeof(float)));
+  HIPCHK(c, hipMalloc((void**)&y, (size_t)rows * kOut * sizeof(float)));
+  if (u.ds_rows > 0) {
+    HIPCHK(c, hipMemcpy(x, u.dsX, u.ds_rows * kFeat * sizeof(float), hipMemcpyDeviceToDevice));
+    HIPCHK(c, hipMemcpy(y, u.dsY, u.ds_rows * kOut * sizeof(float), hipMemcpyDeviceToDevice));
+  }
+  dfree(u.dsX);
+  dfree(u.dsY);
+  u.dsX = x;
+  u.dsY = y;
+  u.ds_cap = rows;
+  return CBZ_OK;
+}
+
+int cbz_sur_collect(cbz_ctx* c, double dt, int64_t iter_per_contact, int64_t* added) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (!c->have_contacts) return fail(c, CBZ_ERR_STATE, "sur_collect: no contacts (run a narrowphase first)");
+  HIPCHK(c, hipSetDevice(c->device));
+  auto& u = c->sur;
+  int64_t tot = 0;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(&tot, c->total, sizeof tot, hipMemcpyDeviceToHost));
+  tot = std::min(tot, c->cap);
+  if (u.ds_rows + tot > u.ds_cap)
+    return fail(c, CBZ_ERR_CAPACITY, "sur_collect: dataset full (cbz_sur_reserve more rows)");
+  if ((rc = sur_ensure_inference(c, tot))) return rc;
+  float* X = u.dsX + u.ds_rows * kFeat;
+  float* Y = u.dsY + u.ds_rows * kOut;
+  const bool f64 = c->precision == CBZ_F64;
+  rc = f64 ? do_resolve<double>(c, dt, iter_per_contact, 0, RM_POS) : do_resolve<float>(c, dt, iter_per_contact, 0, RM_POS);
+  if (rc) return rc;
+  if (f64)
+    HIPCHK(c, launch_features<double>(bodies_view<double>(c), contacts_view<double>(c), tot, u.deg, nullptr, nullptr,
+                                      nullptr, X, c->stream));
+  else
+    HIPCHK(c, launch_features<float>(bodies_view<float>(c), contacts_view<float>(c), tot, u.deg, nullptr, nullptr,
+                                     nullptr, X, c->stream));
+  HIPCHK(c, launch_fill_f32(Y, tot * kOut, 0.f, c->stream));
+  rc = f64 ? do_resolve<double>(c, dt, iter_per_contact, 0, RM_VEL | RM_LABEL, nullptr, Y)
+           : do_resolve<float>(c, dt, iter_per_contact, 0, RM_VEL | RM_LABEL, nullptr, Y);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  u.ds_rows += tot;
+  if (added) *added = tot;
+  return CBZ_OK;
+}
+
+int cbz_sur_dataset(cbz_ctx* c, int64_t* rows, float* X, float* Y) {
+  if (!c) return CBZ_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  auto& u = c->sur;
+  if (rows) *rows = u.ds_rows;
+  if (X && u.ds_rows) HIPCHK(c, hipMemcpy(X, u.dsX, u.ds_rows * kFeat * sizeof(float), hipMemcpyDeviceToHost));
+  if (Y && u.ds_rows) HIPCHK(c, hipMemcpy(Y, u.dsY, u.ds_rows * kOut * sizeof(float), hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+int cbz_sur_load_dataset(cbz_ctx* c, int64_t rows, const float* X, const float* Y) {
+  if (!c || rows < 0 || (rows > 0 && (!X || !Y))) return fail(c, CBZ_ERR_ARG, "sur_load_dataset: bad arguments");
+  int rc = cbz_sur_reserve(c, rows);
+  if (rc) return rc;
+  auto& u = c->sur;
+  if (rows) {
+    HIPCHK(c, hipMemcpy(u.dsX, X, rows * kFeat * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(u.dsY, Y, rows * kOut * sizeof(float), hipMemcpyHostToDevice));
+  }
+  u.ds_rows = rows;
+  return CBZ_OK;
+}
+
+int cbz_sur_init(cbz_ctx* c, uint64_t seed) {
+  if (!c) return CBZ_ERR_ARG;
+  auto& u = c->sur;
+  if (u.ds_rows <= 0) return fail(c, CBZ_ERR_STATE, "sur_init: empty dataset (collect first)");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = sur_alloc_model(c);
+  if (rc) return rc;
+  float* st = nullptr;
+  HIPCHK(c, hipMalloc((void**)&st, 2 * (kFeat + kOut) * sizeof(float)));
+  HIPCHK(c, launch_col_stats(u.dsX, u.ds_rows, kFeat, st, st + kFeat, c->stream));
+  HIPCHK(c, launch_col_stats(u.dsY, u.ds_rows, kOut, st + 2 * kFeat, st + 2 * kFeat + kOut, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<float> h(2 * (kFeat + kOut));
+  HIPCHK(c, hipMemcpy(h.data(), st, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+  (void)hipFree(st);
+  if ((rc = cbz_sur_set_norm(c, h.data()))) return rc;
+  // He-uniform weights (a = sqrt(6 / fan_in)), the output layer scaled by
+  // 0.1; zero biases.  SplitMix64 stream from the seed.
+  const LayerOff o = layer_offsets();
+  std::vector<float> p((size_t)o.total, 0.f);
+  uint64_t x = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  auto next = [&]() {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  for (int l = 0; l < kLayers; ++l) {
+    const double a = std::sqrt(6.0 / layer_in(l)) * (l == kLayers - 1 ? 0.1 : 1.0);
+    for (int64_t i = 0; i < int64_t(layer_out(l)) * layer_in(l); ++i) {
+      const double u01 = double(next() >> 11) * (1.0 / 9007199254740992.0);
+      p[o.w[l] + i] = float((2.0 * u01 - 1.0) * a);
+    }
+  }
+  return cbz_sur_set_params(c, p.data());
+}
+
+int cbz_sur_train(cbz_ctx* c, int32_t steps, int32_t batch, float lr, uint64_t seed, float* losses) {
+  if (!c || steps < 0 || batch <= 0) return fail(c, CBZ_ERR_ARG, "sur_train: bad arguments");
+  auto& u = c->sur;
+  if (!u.have_model || u.ds_rows <= 0) return fail(c, CBZ_ERR_STATE, "sur_train: needs a model and a dataset");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = sur_ensure_batch(c, batch);
+  if (rc) return rc;
+  if (steps > u.losses_cap) {
+    if ((rc = dalloc(c, &u.losses, (size_t)steps))) return rc;
+    u.losses_cap = steps;
+  }
+  const int64_t Mpad = round32(batch);
+  const LayerOff o = layer_offsets();
+  std::vector<int64_t> idx((size_t)batch);
+  uint64_t x = seed * 0xD1B54A32D192ED03ull + 1;
+  auto next = [&]() {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  const float* nm = u.norm;
+  for (int32_t s = 0; s < steps; ++s) {
+    for (int i = 0; i < batch; ++i) idx[i] = int64_t(next() % uint64_t(u.ds_rows));
+    HIPCHK(c, hipMemcpyAsync(u.idx, idx.data(), batch * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_gather_batch(u.dsX, u.dsY, u.idx, (int)Mpad, batch, nm, nm + kFeat, nm + 2 * kFeat,
+                                  nm + 2 * kFeat + 2 * kOut, u.bX, u.At[0], u.bT, c->stream));
+    if ((rc = sur_forward(c, u.bX, batch, Mpad, true))) return rc;
+    if ((rc = sur_backward(c, batch, Mpad))) return rc;
+    HIPCHK(c, launch_sum_f32(u.row_loss, (int)Mpad, u.losses + s, c->stream));
+    u.adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f / (1.0f - std::pow(b1, (float)u.adam_t));
+    const float c2 = 1.0f / (1.0f - std::pow(b2, (float)u.adam_t));
+    HIPCHK(c, launch_adam(u.params, u.grad, u.adam_m, u.adam_v, o.total, lr, b1, b2, 1e-8f, c1, c2, c->stream));
+    if ((rc = sur_pack(c))) return rc;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (losses && steps > 0) HIPCHK(c, hipMemcpy(losses, u.losses, steps * sizeof(float), hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+// rows of raw features (and raw labels) -> the staging batch
+static int sur_stage_rows(cbz_ctx* c, int32_t rows, const float* X, const float* Y) {
+  auto& u = c->sur;
+  int rc = sur_ensure_batch(c, rows);
+  if (rc) return rc;
+  const int64_t Mpad = round32(rows);
+  float *dx = nullptr, *dy = nullptr;
+  HIPCHK(c, hipMalloc((void**)&dx, (size_t)rows * kFeat * sizeof(float)));
+  HIPCHK(c, hipMalloc((void**)&dy, (size_t)rows * kOut * sizeof(float)));
+  std::vector<int64_t> idx((size_t)rows);
+  for (int i = 0; i < rows; ++i) idx[i] = i;
+  hipError_t e = hipMemcpy(dx, X, rows * kFeat * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess && Y) e = hipMemcpy(dy, Y, rows * kOut * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(u.idx, idx.data(), rows * sizeof(int64_t), hipMemcpyHostToDevice);
+  const float* nm = u.norm;
+  if (e == hipSuccess)
+    e = launch_gather_batch(dx, Y ? dy : nullptr, u.idx, (int)Mpad, rows, nm, nm + kFeat, nm + 2 * kFeat,
+                            nm + 2 * kFeat + 2 * kOut, u.bX, u.At[0], u.bT, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  if (e != hipSuccess) return fail(c, CBZ_ERR_HIP, std::string("sur: staging: ") + hipGetErrorString(e));
+  return CBZ_OK;
+}
+
+int cbz_sur_grad(cbz_ctx* c, int32_t rows, const float* X, const float* Y, float* grad, float* loss) {
+  if (!c || rows <= 0 || !X || !Y) return fail(c, CBZ_ERR_ARG, "sur_grad: bad arguments");
+  if (!c->sur.have_model) return fail(c, CBZ_ERR_STATE, "sur_grad: no model");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = sur_stage_rows(c, rows, X, Y);
+  if (rc) return rc;
+  auto& u = c->sur;
+  const int64_t Mpad = round32(rows);
+  if ((rc = sur_forward(c, u.bX, rows, Mpad, true))) return rc;
+  if ((rc = sur_backward(c, rows, Mpad))) return rc;
+  float* dl = nullptr;
+  HIPCHK(c, hipMalloc((void**)&dl, sizeof(float)));
+  HIPCHK(c, launch_sum_f32(u.row_loss, (int)Mpad, dl, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (loss) HIPCHK(c, hipMemcpy(loss, dl, sizeof(float), hipMemcpyDeviceToHost));
+  (void)hipFree(dl);
+  if (grad) HIPCHK(c, hipMemcpy(grad, u.grad, layer_offsets().total * sizeof(float), hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+int cbz_sur_predict(cbz_ctx* c, int32_t rows, const float* X, float* Y) {
+  if (!c || rows <= 0 || !X || !Y) return fail(c, CBZ_ERR_ARG, "sur_predict: bad arguments");
+  if (!c->sur.have_model) return fail(c, CBZ_ERR_STATE, "sur_predict: no model");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = sur_stage_rows(c, rows, X, nullptr);
+  if (rc) return rc;
+  auto& u = c->sur;
+  if ((rc = sur_ensure_inference(c, rows))) return rc;
+  const int64_t Mpad = round32(rows);
+  if ((rc = sur_forward(c, u.bX, rows, Mpad, false))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(Y, u.Yp, (size_t)rows * kOut * sizeof(float), hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+int cbz_sur_resolve(cbz_ctx* c, double dt, int64_t iter_per_contact, double tol, int64_t* n_fallback) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (!c->have_contacts) return fail(c, CBZ_ERR_STATE, "sur_resolve: no contacts (run a narrowphase first)");
+  if (!c->sur.have_model) return fail(c, CBZ_ERR_STATE, "sur_resolve: no model");
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = sur_ensure_inference(c, 32))) return rc;
+  rc = c->precision == CBZ_F64 ? sur_resolve<double>(c, dt, iter_per_contact, tol, c->sur.nfall)
+                               : sur_resolve<float>(c, dt, iter_per_contact, tol, c->sur.nfall);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (n_fallback) {
+    int32_t v = 0;
+    HIPCHK(c, hipMemcpy(&v, c->sur.nfall, sizeof v, hipMemcpyDeviceToHost));
+    *n_fallback = v;
+  }
+  return CBZ_OK;
+}
+
+int cbz_sur_world_mask(cbz_ctx* c, uint8_t* mask) {
+  if (!c || !mask) return CBZ_ERR_ARG;
+  if (!c->sur.mask) return fail(c, CBZ_ERR_STATE, "sur_world_mask: no surrogate resolve yet");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(mask, c->sur.mask, c->nw, hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+int cbz_step_sur(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contact, double tol, int64_t* counts,
+                 int64_t* fallbacks) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (n_steps < 0 || iter_per_contact < 0) return fail(c, CBZ_ERR_ARG, "step_sur: bad arguments");
+  if (!c->sur.have_model) return fail(c, CBZ_ERR_STATE, "step_sur: no model");
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = sur_ensure_inference(c, 32))) return rc;
+  int32_t* hist = nullptr;
+  int32_t* fb = nullptr;
+  if (n_steps > 0) {
+    if (counts) HIPCHK(c, hipMalloc((void**)&hist, (size_t)n_steps * c->nw * sizeof(int32_t)));
+    HIPCHK(c, hipMalloc((void**)&fb, (size_t)n_steps * sizeof(int32_t)));
+  }
+  HIPCHK(c, launch_fill_i32(c->overflow, 2, 0, c->stream));
+  const bool f64 = c->precision == CBZ_F64;
+  for (int32_t s = 0; s < n_steps && rc == CBZ_OK; ++s) {
+    rc = f64 ? do_integrate<double>(c, dt) : do_integrate<float>(c, dt);
+    if (rc) break;
+    rc = f64 ? do_narrowphase<double>(c, -1) : do_narrowphase<float>(c, -1);
+    if (rc) break;
+    if (hist)
+      HIPCHK(c, hipMemcpyAsync(hist + (size_t)s * c->nw, c->world_cnt, c->nw * sizeof(int32_t),
+                               hipMemcpyDeviceToDevice, c->stream));
+    rc = f64 ? sur_resolve<double>(c, dt, iter_per_contact, tol, fb + s)
+             : sur_resolve<float>(c, dt, iter_per_contact, tol, fb + s);
+  }
+  if (rc == CBZ_OK) rc = check_overflow(c);
+  if (rc == CBZ_OK && n_steps > 0) {
+    std::vector<int32_t> t((size_t)n_steps * c->nw), f((size_t)n_steps);
+    hipError_t e = hipSuccess;
+    if (hist) e = hipMemcpy(t.data(), hist, t.size() * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(f.data(), fb, f.size() * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(c, CBZ_ERR_HIP, std::string("step_sur: download: ") + hipGetErrorString(e));
+    if (counts)
+      for (size_t k = 0; k < t.size(); ++k) counts[k] = t[k];
+    if (fallbacks)
+      for (size_t k = 0; k < f.size(); ++k) fallbacks[k] = f[k];
+  }
+  if (hist) (void)hipFree(hist);
+  if (fb) (void)hipFree(fb);
+  return rc;
 }
 
 }  // extern "C"

@@ -7,6 +7,13 @@
 
 namespace cbz {
 
+// Resolver phases: RM_POS = prepareContacts + adjustPositions, RM_VEL =
+// adjustVelocities (which reads the contact velocities prepare stored).
+// Run together they are ResolveContacts; run apart, the velocity phase can
+// be given to a subset of worlds.  RM_LABEL records, per contact, the sum of
+// the velocity and rotation changes applyVelocityChange gave its two bodies.
+enum : int32_t { RM_POS = 1, RM_VEL = 2, RM_LABEL = 4 };
+
 template <typename R>
 struct NarrowLaunch {
   Bodies<R> b;
@@ -38,6 +45,10 @@ struct ResolveLaunch {
   R dt;
   int64_t iter_mult;
   int64_t iter_fixed;
+  int32_t mode = 3;                      // RM_POS | RM_VEL: the whole of ResolveContacts
+  const uint8_t* world_mask = nullptr;   // velocity phase only where set
+  R* pos0 = nullptr;                     // [3][n_bodies] positions at prepare time
+  float* labels = nullptr;               // [cap][12] velocity-phase changes per contact
 };
 
 template <typename R>

@@ -1,0 +1,84 @@
+// cbz_mlp.h — shapes and launch interfaces of the resolver surrogate MLP.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cbz_types.h"
+
+namespace cbz {
+
+constexpr int kFeat = 40;      // input features per contact
+constexpr int kFeatPad = 48;   // padded to the MFMA K step (16)
+constexpr int kHidden = 256;   // hidden width
+constexpr int kOut = 12;       // outputs: dv0, dw0, dv1, dw1
+constexpr int kOutPad = 32;    // padded to the MFMA tile (32)
+constexpr int kLayers = 4;
+constexpr int kSurBodies = 256;     // bodies per world the apply kernel takes
+constexpr int kSurContacts = 1024;  // contacts per world the apply kernel takes
+
+// layer l maps kIn[l] -> kOutL[l] features (unpadded)
+__host__ __device__ constexpr int layer_in(int l) { return l == 0 ? kFeat : kHidden; }
+__host__ __device__ constexpr int layer_out(int l) { return l == kLayers - 1 ? kOut : kHidden; }
+__host__ __device__ constexpr int layer_in_pad(int l) { return l == 0 ? kFeatPad : kHidden; }
+__host__ __device__ constexpr int layer_out_pad(int l) { return l == kLayers - 1 ? kOutPad : kHidden; }
+
+struct GemmArgs {
+  int M, N, K;                  // padded sizes: M, N multiples of 32, K of 16
+  const __bf16* A;              // [M x K] row-major
+  int64_t lda;
+  const __bf16* Bt;             // [N x K] row-major (B transposed)
+  int64_t ldb;
+  const float* bias;            // [N] or null
+  int act;                      // 1: SiLU
+  float* Z;                     // pre-activation FP32 [M x N] (or null)
+  int64_t ldz;
+  float* Y;                     // activation FP32 (or null)
+  int64_t ldy;
+  const float* y_scale;         // Y = act * scale + shift per column (or null)
+  const float* y_shift;
+  __bf16* H;                    // activation BF16 row-major (or null)
+  int64_t ldh;
+  __bf16* Ht;                   // activation BF16 transposed [N x M] (or null)
+  int64_t ldht;
+  int m_valid, n_valid;         // rows / columns that hold data
+};
+
+template <typename R>
+struct SurApply {
+  Bodies<R> b;
+  Contacts<R> c;
+  const int32_t* world_cnt;
+  const int64_t* world_coff;
+  const int32_t* body_off;
+  const int32_t* overflow;
+  const float* Y;               // [contacts][kOut] predicted changes
+  uint8_t* mask;                // out: 1 = the world needs the exact velocity phase
+  R dt;
+  float tol;
+  int max_contacts;
+};
+
+hipError_t launch_gemm(const GemmArgs& g, hipStream_t st);
+template <typename R>
+hipError_t launch_features(const Bodies<R>& b, const Contacts<R>& c, int64_t n, int32_t* deg, const float* mean,
+                           const float* inv_std, __bf16* X, float* raw, hipStream_t st);
+template <typename R>
+hipError_t launch_sur_apply(const SurApply<R>& a, int n_worlds, hipStream_t st);
+hipError_t launch_gather_batch(const float* Xraw, const float* Yraw, const int64_t* idx, int rows, int valid_rows,
+                               const float* in_mean, const float* in_inv_std, const float* out_mean,
+                               const float* out_inv_std, __bf16* X, __bf16* Xt, float* T, hipStream_t st);
+hipError_t launch_silu_bwd(const float* dA, const float* Z, int rows, int valid_rows, int cols, int ld, __bf16* dZ,
+                           __bf16* dZt, int ldt, hipStream_t st);
+hipError_t launch_col_stats(const float* A, int64_t rows, int cols, float* mean, float* stdv, hipStream_t st);
+hipError_t launch_count_u8(const uint8_t* m, int n, int32_t* out, hipStream_t st);
+hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t st);
+hipError_t launch_mse(const float* Y, const float* T, int rows, int valid_rows, float* row_loss, __bf16* dY,
+                      __bf16* dYt, int ldt, hipStream_t st);
+hipError_t launch_colsum(const __bf16* A, int rows, int cols, int ld, float* out, hipStream_t st);
+hipError_t launch_sum_f32(const float* a, int n, float* out, hipStream_t st);
+hipError_t launch_adam(float* p, const float* grad, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, hipStream_t st);
+hipError_t launch_pack_weights(const float* W, int N, int K, __bf16* Wnk, int kpad, __bf16* Wkn, int npad,
+                               hipStream_t st);
+
+}  // namespace cbz

@@ -1,0 +1,449 @@
+// k_mlp.hip — the learned resolver surrogate: a 40 -> 256 -> 256 -> 256 -> 12
+// MLP with SiLU, BF16 weights and activations, FP32 accumulation, run on
+// the matrix cores (v_mfma_f32_32x32x16_bf16) for inference and training.
+//
+// Per contact it predicts the velocity and rotation changes that the exact
+// resolver's velocity phase (adjustVelocities, contact.go:390-445) gives the
+// contact's two bodies, summed over the iterations that chose the contact.
+// Summing the predictions of a body's contacts gives the body's change.
+//
+// Kernels
+//   k_degree      contacts per body (an input feature)
+//   k_features    the 40 input features of every contact, normalised, BF16
+//   k_gemm        C = A * B (+bias, SiLU) on MFMA 32x32x16, FP32 accumulate
+//   k_silu_bwd, k_mse, k_colsum, k_adam   backward pass and optimiser
+//   k_sur_apply   per world: apply the predictions, check the residual
+//                 closing velocities, or mark the world for the exact method
+#include <climits>
+
+#include "cbz_devmath.h"
+#include "cbz_launch.h"
+#include "cbz_mlp.h"
+#include "cbz_types.h"
+#include "cbz_wave.h"
+
+namespace cbz {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <typename R>
+__device__ __forceinline__ V3<R> ld3c(const R* a, int64_t s, int64_t i) {
+  return V3<R>{a[i], a[s + i], a[2 * s + i]};
+}
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// ---------------------------------------------------------------------------
+// features
+// ---------------------------------------------------------------------------
+__global__ void k_degree(const int32_t* body0, const int32_t* body1, int64_t n, int32_t* deg) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  atomicAdd(&deg[body0[g]], 1);
+  if (body1[g] >= 0) atomicAdd(&deg[body1[g]], 1);
+}
+
+// Feature layout (kFeat = 40): normal 3, rel0 3, rel1 3, v0-v1 3, w0-w1 3,
+// invMass0, invMass1, iitWorld0 9, iitWorld1 9, penetration,
+// desiredDeltaVelocity, contactVelocity.x, 1/deg0, 1/deg1.
+template <typename R>
+__global__ void k_features(Bodies<R> b, Contacts<R> c, int64_t n, const int32_t* deg, const float* mean,
+                           const float* inv_std, __bf16* X, float* raw) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int64_t S = b.stride, CS = c.cap;
+  const int b0 = c.body0[g], b1 = c.body1[g];
+  float f[kFeat];
+  int k = 0;
+  for (int d = 0; d < 3; ++d) f[k++] = float(c.normal[d * CS + g]);
+  for (int d = 0; d < 6; ++d) f[k++] = float(c.rel[d * CS + g]);
+  for (int d = 0; d < 3; ++d) f[k++] = float(b.vel[d * S + b0]) - (b1 >= 0 ? float(b.vel[d * S + b1]) : 0.f);
+  for (int d = 0; d < 3; ++d) f[k++] = float(b.rot[d * S + b0]) - (b1 >= 0 ? float(b.rot[d * S + b1]) : 0.f);
+  f[k++] = float(b.inv_mass[b0]);
+  f[k++] = b1 >= 0 ? float(b.inv_mass[b1]) : 0.f;
+  for (int d = 0; d < 9; ++d) f[k++] = float(b.iit_world[d * S + b0]);
+  for (int d = 0; d < 9; ++d) f[k++] = b1 >= 0 ? float(b.iit_world[d * S + b1]) : 0.f;
+  f[k++] = float(c.pen[g]);
+  f[k++] = float(c.ddv[g]);
+  f[k++] = float(c.cvel[g]);
+  f[k++] = 1.0f / float(deg[b0]);
+  f[k++] = b1 >= 0 ? 1.0f / float(deg[b1]) : 0.f;
+  if (raw)
+    for (int i = 0; i < kFeat; ++i) raw[g * kFeat + i] = f[i];
+  if (X) {
+    __bf16* x = X + g * kFeatPad;
+    for (int i = 0; i < kFeat; ++i) x[i] = __bf16((f[i] - mean[i]) * inv_std[i]);
+    for (int i = kFeat; i < kFeatPad; ++i) x[i] = __bf16(0.f);
+  }
+}
+
+// raw features / labels of a sampled batch -> normalised BF16 inputs and
+// FP32 targets (training).
+__global__ void k_gather_batch(const float* Xraw, const float* Yraw, const int64_t* idx, int rows, int valid_rows,
+                               const float* in_mean, const float* in_inv_std, const float* out_mean,
+                               const float* out_inv_std, __bf16* X, __bf16* Xt, float* T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const bool v = i < valid_rows;
+  const int64_t s = v ? idx[i] : 0;
+  for (int k = 0; k < kFeatPad; ++k) {
+    const __bf16 x = __bf16(v && k < kFeat ? (Xraw[s * kFeat + k] - in_mean[k]) * in_inv_std[k] : 0.f);
+    X[int64_t(i) * kFeatPad + k] = x;
+    Xt[int64_t(k) * rows + i] = x;
+  }
+  for (int k = 0; k < kOut; ++k)
+    T[int64_t(i) * kOut + k] = v && Yraw ? (Yraw[s * kOut + k] - out_mean[k]) * out_inv_std[k] : 0.f;
+}
+
+// column mean and standard deviation of a row-major FP32 table (one block
+// per column, FP64 accumulation); std floored at 1e-6.
+__global__ void k_col_stats(const float* A, int64_t rows, int cols, float* mean, float* stdv) {
+  __shared__ double s1[256], s2[256];
+  const int c = blockIdx.x;
+  double a = 0, b = 0;
+  for (int64_t r = threadIdx.x; r < rows; r += 256) {
+    const double v = A[r * cols + c];
+    a += v;
+    b += v * v;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double m = rows > 0 ? s1[0] / double(rows) : 0.0;
+    const double var = rows > 0 ? s2[0] / double(rows) - m * m : 0.0;
+    mean[c] = float(m);
+    stdv[c] = float(var > 1e-12 ? sqrt(var) : 1e-6);
+  }
+}
+
+__global__ void k_count_u8(const uint8_t* m, int n, int32_t* out) {
+  __shared__ int part[256];
+  int s = 0;
+  for (int i = threadIdx.x; i < n; i += 256) s += m[i] ? 1 : 0;
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0];
+}
+
+__global__ void k_fill_f32(float* p, int64_t n, float v) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// GEMM on the matrix cores
+// ---------------------------------------------------------------------------
+// C[M x N] = A[M x K] * B[K x N] with A row-major (K contiguous) and B given
+// as Bt = [N x K] row-major (K contiguous): both operands load their MFMA
+// fragments as 16-byte vectors.  Fragment layout of v_mfma_f32_32x32x16_bf16
+// for lane l (r = l % 32, h = l / 32): A[r][8h .. 8h+7], B[8h .. 8h+7][r],
+// and accumulator j holds D[8 (j/4) + 4h + j%4][r].  Each wave computes one
+// 32x32 tile; a block holds WM x WN waves.  M, N are multiples of 32, K of 16.
+template <int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN) k_gemm(GemmArgs g) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m0 = (blockIdx.x * WM + wave % WM) * 32;
+  const int n0 = (blockIdx.y * WN + wave / WM) * 32;
+  if (m0 >= g.M || n0 >= g.N) return;
+  const int r = lane & 31, h = lane >> 5;
+  const __bf16* ap = g.A + int64_t(m0 + r) * g.lda + 8 * h;
+  const __bf16* bp = g.Bt + int64_t(n0 + r) * g.ldb + 8 * h;
+  f32x16 acc = {};
+  for (int k = 0; k < g.K; k += 16) {
+    const bf16x8 a = *(const bf16x8*)(ap + k);
+    const bf16x8 bb = *(const bf16x8*)(bp + k);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
+  }
+  const int col = n0 + r;
+  const float bias = g.bias ? g.bias[col] : 0.f;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int row = m0 + 8 * (j >> 2) + 4 * h + (j & 3);
+    const float z = acc[j] + bias;
+    const float y = g.act ? silu(z) : z;
+    if (row < g.m_valid && col < g.n_valid) {
+      if (g.Z) g.Z[int64_t(row) * g.ldz + col] = z;
+      if (g.Y) g.Y[int64_t(row) * g.ldy + col] = g.y_scale ? y * g.y_scale[col] + g.y_shift[col] : y;
+    }
+    // BF16 activations keep their zero padding (rows >= m_valid, cols >= n_valid)
+    const float yv = (row < g.m_valid && col < g.n_valid) ? y : 0.f;
+    if (g.H) g.H[int64_t(row) * g.ldh + col] = __bf16(yv);
+    if (g.Ht) g.Ht[int64_t(col) * g.ldht + row] = __bf16(yv);
+  }
+}
+
+// dZ = dA * SiLU'(Z), written as BF16 row-major and transposed.
+__global__ void k_silu_bwd(const float* dA, const float* Z, int rows, int valid_rows, int cols, int ld, __bf16* dZ,
+                           __bf16* dZt, int ldt) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= int64_t(rows) * cols) return;
+  const int r = int(i / cols), c = int(i % cols);
+  const float v = r < valid_rows ? dA[int64_t(r) * ld + c] * silu_grad(Z[int64_t(r) * ld + c]) : 0.f;
+  dZ[int64_t(r) * ld + c] = __bf16(v);
+  dZt[int64_t(c) * ldt + r] = __bf16(v);
+}
+
+// Mean-squared error over rows x kOut; writes dL/dY as BF16 (row-major,
+// width kOutPad, and transposed) and the per-row squared error.
+__global__ void k_mse(const float* Y, const float* T, int rows, int valid_rows, float* row_loss, __bf16* dY,
+                      __bf16* dYt, int ldt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  float acc = 0.f;
+  for (int k = 0; k < kOutPad; ++k) {
+    float d = 0.f;
+    if (k < kOut && i < valid_rows) {
+      const float e = Y[int64_t(i) * kOut + k] - T[int64_t(i) * kOut + k];
+      acc += e * e;
+      d = 2.0f * e / float(valid_rows * kOut);
+    }
+    dY[int64_t(i) * kOutPad + k] = __bf16(d);
+    dYt[int64_t(k) * ldt + i] = __bf16(d);
+  }
+  row_loss[i] = i < valid_rows ? acc / float(valid_rows * kOut) : 0.f;
+}
+
+// column sums of a row-major BF16 matrix: bias gradient, and (FP32 input)
+// the loss.  One thread per column, rows summed in order.
+__global__ void k_colsum(const __bf16* A, int rows, int cols, int ld, float* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; ++r) s += float(A[int64_t(r) * ld + c]);
+  out[c] = s;
+}
+__global__ void k_sum_f32(const float* a, int n, float* out) {
+  __shared__ float part[256];
+  float s = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) s += a[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0];
+}
+
+// Adam on the FP32 master parameters, then refresh the BF16 copies of one
+// layer's weights: Wnk [N x Kpad] (forward B-operand layout) and Wkn
+// [K x Npad] (backward), padding left at zero.
+__global__ void k_adam(float* p, const float* grad, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gr = grad[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gr;
+  const float vi = b2 * v[i] + (1.f - b2) * gr * gr;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+__global__ void k_pack_weights(const float* W, int N, int K, __bf16* Wnk, int kpad, __bf16* Wkn, int npad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N * K) return;
+  const int n = i / K, k = i % K;
+  const __bf16 w = __bf16(W[i]);
+  Wnk[int64_t(n) * kpad + k] = w;
+  Wkn[int64_t(k) * npad + n] = w;
+}
+
+// ---------------------------------------------------------------------------
+// apply + check
+// ---------------------------------------------------------------------------
+// One wave per world.  The predicted per-contact changes are summed per body
+// in contact order (deterministic); every contact's closing velocity is then
+// updated as the resolver's update loop would (contact.go:416-442) and its
+// desired velocity change recomputed.  If the largest remaining desired
+// change is within velocityEpsilon + tol the world keeps the result, else it
+// is flagged for the exact velocity phase and left untouched.
+template <typename R>
+__global__ void __launch_bounds__(64) k_sur_apply(SurApply<R> a) {
+  __shared__ float s_d[kSurBodies][6];
+  __shared__ int32_t s_b[kSurContacts][2];
+  const int w = blockIdx.x, lane = threadIdx.x;
+  const int C = a.world_cnt[w];
+  const int b0w = a.body_off[w], n = a.body_off[w + 1] - b0w;
+  if (a.overflow[0] != 0 || C <= 0 || !(a.dt > R(0))) {
+    if (lane == 0) a.mask[w] = 0;
+    return;
+  }
+  if (n > kSurBodies || C > a.max_contacts || C > kSurContacts) {
+    if (lane == 0) a.mask[w] = 1;
+    return;
+  }
+  const int64_t base = a.world_coff[w];
+  const Bodies<R>& b = a.b;
+  const Contacts<R>& c = a.c;
+  const int64_t S = b.stride, CS = c.cap;
+  for (int j = lane; j < C; j += 64) {
+    s_b[j][0] = c.body0[base + j];
+    s_b[j][1] = c.body1[base + j];
+  }
+  __syncthreads();
+  // body sums, contact order: lane i owns bodies i, i+64, ...
+  for (int i = lane; i < n; i += 64) {
+    float d[6] = {0, 0, 0, 0, 0, 0};
+    const int gb = b0w + i;
+    for (int j = 0; j < C; ++j) {
+      const float* y = a.Y + (base + j) * kOut;
+      if (s_b[j][0] == gb)
+        for (int k = 0; k < 6; ++k) d[k] += y[k];
+      if (s_b[j][1] == gb)
+        for (int k = 0; k < 6; ++k) d[k] += y[6 + k];
+    }
+    for (int k = 0; k < 6; ++k) s_d[i][k] = d[k];
+  }
+  __syncthreads();
+  // residual desired velocity change of every contact
+  float worst = -INFINITY;
+  bool bad = false;
+  for (int j = lane; j < C; j += 64) {
+    const int64_t g = base + j;
+    const int jb[2] = {c.body0[g], c.body1[g]};
+    const V3<R> nrm = ld3c(c.normal, CS, g);
+    R m[9];
+    for (int k = 0; k < 9; ++k) m[k] = c.c2w[k * CS + g];
+    V3<R> cv = ld3c(c.cvel, CS, g);
+    for (int bb = 0; bb < 2; ++bb) {
+      if (jb[bb] < 0) continue;
+      const int l = jb[bb] - b0w;
+      const V3<R> dv{R(s_d[l][0]), R(s_d[l][1]), R(s_d[l][2])};
+      const V3<R> dw{R(s_d[l][3]), R(s_d[l][4]), R(s_d[l][5])};
+      const V3<R> rel = ld3c(c.rel + 3 * bb * CS, CS, g);
+      const V3<R> d = add(cross(dw, rel), dv);
+      cv = add(cv, mul(m3tv(m, d), bb == 1 ? R(-1) : R(1)));
+    }
+    // calculateDesiredDeltaVelocity (contact.go:87-112)
+    R vfa = R(0);
+    if (b.flags[jb[0]] & F_AWAKE) vfa += dot(mul(ld3c(b.last_acc, S, jb[0]), a.dt), nrm);
+    if (jb[1] >= 0 && (b.flags[jb[1]] & F_AWAKE)) vfa -= dot(mul(ld3c(b.last_acc, S, jb[1]), a.dt), nrm);
+    R rest = R(kRestitution);
+    if (dabs(cv.x) < R(0.25)) rest = R(0);
+    const R ddv = -cv.x - rest * (cv.x - vfa);
+    worst = fmaxf(worst, float(ddv));
+    bad |= !(ddv == ddv);
+  }
+  worst = dpp::allreduce(worst, -INFINITY, [](float x, float y) { return y > x ? y : x; });
+  const bool any_bad = __ballot(bad) != 0;
+  const bool accept = !any_bad && worst <= float(kVelocityEpsilon) + a.tol;
+  if (lane == 0) a.mask[w] = accept ? 0 : 1;
+  if (!accept) return;
+  for (int i = lane; i < n; i += 64) {
+    const int gb = b0w + i;
+    for (int k = 0; k < 3; ++k) {
+      b.vel[k * S + gb] += R(s_d[i][k]);
+      b.rot[k * S + gb] += R(s_d[i][3 + k]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+hipError_t launch_gemm(const GemmArgs& g, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  if (g.N >= 128) {
+    dim3 grid((g.M + 63) / 64, (g.N + 63) / 64);
+    k_gemm<2, 2><<<grid, 256, 0, st>>>(g);
+  } else {
+    dim3 grid((g.M + 127) / 128, (g.N + 31) / 32);
+    k_gemm<4, 1><<<grid, 256, 0, st>>>(g);
+  }
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_features(const Bodies<R>& b, const Contacts<R>& c, int64_t n, int32_t* deg, const float* mean,
+                           const float* inv_std, __bf16* X, float* raw, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipError_t e = launch_fill_i32(deg, b.n, 0, st);
+  if (e != hipSuccess) return e;
+  const int blocks = int((n + 255) / 256);
+  k_degree<<<blocks, 256, 0, st>>>(c.body0, c.body1, n, deg);
+  k_features<R><<<blocks, 256, 0, st>>>(b, c, n, deg, mean, inv_std, X, raw);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_sur_apply(const SurApply<R>& a, int n_worlds, hipStream_t st) {
+  if (n_worlds <= 0) return hipSuccess;
+  k_sur_apply<R><<<n_worlds, 64, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_batch(const float* Xraw, const float* Yraw, const int64_t* idx, int rows, int valid_rows,
+                               const float* in_mean, const float* in_inv_std, const float* out_mean,
+                               const float* out_inv_std, __bf16* X, __bf16* Xt, float* T, hipStream_t st) {
+  k_gather_batch<<<(rows + 255) / 256, 256, 0, st>>>(Xraw, Yraw, idx, rows, valid_rows, in_mean, in_inv_std,
+                                                   out_mean, out_inv_std, X, Xt, T);
+  return hipGetLastError();
+}
+hipError_t launch_col_stats(const float* A, int64_t rows, int cols, float* mean, float* stdv, hipStream_t st) {
+  k_col_stats<<<cols, 256, 0, st>>>(A, rows, cols, mean, stdv);
+  return hipGetLastError();
+}
+hipError_t launch_count_u8(const uint8_t* m, int n, int32_t* out, hipStream_t st) {
+  k_count_u8<<<1, 256, 0, st>>>(m, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  k_fill_f32<<<int((n + 255) / 256), 256, 0, st>>>(p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_silu_bwd(const float* dA, const float* Z, int rows, int valid_rows, int cols, int ld, __bf16* dZ,
+                           __bf16* dZt, int ldt, hipStream_t st) {
+  const int64_t n = int64_t(rows) * cols;
+  k_silu_bwd<<<int((n + 255) / 256), 256, 0, st>>>(dA, Z, rows, valid_rows, cols, ld, dZ, dZt, ldt);
+  return hipGetLastError();
+}
+hipError_t launch_mse(const float* Y, const float* T, int rows, int valid_rows, float* row_loss, __bf16* dY,
+                      __bf16* dYt, int ldt, hipStream_t st) {
+  k_mse<<<(rows + 255) / 256, 256, 0, st>>>(Y, T, rows, valid_rows, row_loss, dY, dYt, ldt);
+  return hipGetLastError();
+}
+hipError_t launch_colsum(const __bf16* A, int rows, int cols, int ld, float* out, hipStream_t st) {
+  k_colsum<<<(cols + 63) / 64, 64, 0, st>>>(A, rows, cols, ld, out);
+  return hipGetLastError();
+}
+hipError_t launch_sum_f32(const float* a, int n, float* out, hipStream_t st) {
+  k_sum_f32<<<1, 256, 0, st>>>(a, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_adam(float* p, const float* grad, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, hipStream_t st) {
+  k_adam<<<int((n + 255) / 256), 256, 0, st>>>(p, grad, m, v, n, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+hipError_t launch_pack_weights(const float* W, int N, int K, __bf16* Wnk, int kpad, __bf16* Wkn, int npad,
+                               hipStream_t st) {
+  k_pack_weights<<<(N * K + 255) / 256, 256, 0, st>>>(W, N, K, Wnk, kpad, Wkn, npad);
+  return hipGetLastError();
+}
+
+template hipError_t launch_features<double>(const Bodies<double>&, const Contacts<double>&, int64_t, int32_t*,
+                                            const float*, const float*, __bf16*, float*, hipStream_t);
+template hipError_t launch_features<float>(const Bodies<float>&, const Contacts<float>&, int64_t, int32_t*,
+                                           const float*, const float*, __bf16*, float*, hipStream_t);
+template hipError_t launch_sur_apply<double>(const SurApply<double>&, int, hipStream_t);
+template hipError_t launch_sur_apply<float>(const SurApply<float>&, int, hipStream_t);
+
+}  // namespace cbz

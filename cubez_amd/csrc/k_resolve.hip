@@ -38,7 +38,24 @@ struct RS {
   R dt;
   int64_t iter_mult;
   int64_t iter_fixed;
+  int32_t mode;                // RM_* bits
+  const uint8_t* world_mask;   // velocity phase only where set (null: every world)
+  R* pos0;                     // [3][n_bodies] body positions at prepare time
+  float* labels;               // [cap][12] accumulated velocity-phase changes (null: off)
 };
+
+
+template <typename R>
+__device__ __forceinline__ void add_label(const RS<R>& a, int64_t g, const V3<R> vc[2], const V3<R> rc[2]) {
+  float* L = a.labels + g * 12;
+  const V3<R>* v[4] = {&vc[0], &rc[0], &vc[1], &rc[1]};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    L[3 * k + 0] += float(v[k]->x);
+    L[3 * k + 1] += float(v[k]->y);
+    L[3 * k + 2] += float(v[k]->z);
+  }
+}
 
 template <typename R>
 __device__ __forceinline__ V3<R> ld3(const R* a, int64_t s, int64_t i) {
@@ -356,9 +373,11 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
   const int64_t CS = c.cap;
   const int64_t maxIter = a.iter_fixed > 0 ? a.iter_fixed : a.iter_mult * int64_t(C);
 
+  const bool do_pos = (a.mode & RM_POS) != 0;
+  const bool do_vel = (a.mode & RM_VEL) != 0 && (!a.world_mask || a.world_mask[w]);
   // prepareContacts, contact.go:225-229
   for (int j = lane; j < C; j += 64) {
-    prepare_contact(a, base + j);
+    if (do_pos) prepare_contact(a, base + j);
     if (LDS) {
       key[j] = c.pen[base + j];
       kb0[j] = c.body0[base + j];
@@ -369,7 +388,7 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
 
   // adjustPositions, contact.go:233-283
   int64_t used = 0;
-  while (used < maxIter) {
+  while (do_pos && used < maxIter) {
     R best = R(kPositionEpsilon);
     int bi = INT_MAX;
     for (int j = lane; j < C; j += 64) {
@@ -421,7 +440,7 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
   const int64_t pos_iters = used;
   if (LDS) {
     for (int j = lane; j < C; j += 64) {
-      c.pen[base + j] = key[j];
+      if (do_pos) c.pen[base + j] = key[j];
       key[j] = c.ddv[base + j];
     }
     __syncthreads();
@@ -430,7 +449,7 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
 
   // adjustVelocities, contact.go:390-445
   used = 0;
-  while (used < maxIter) {
+  while (do_vel && used < maxIter) {
     R best = R(kVelocityEpsilon);
     int bi = INT_MAX;
     for (int j = lane; j < C; j += 64) {
@@ -453,6 +472,7 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
     match_awake(a, kb[0], kb[1], aw[0], aw[1], lane);
     V3<R> vc[2], rc[2];
     apply_velocity_change(a, kb, rel, m, best, cv, vc, rc, lane);
+    if ((a.mode & RM_LABEL) && lane == 0) add_label(a, g, vc, rc);
     for (int j = lane; j < C; j += 64) {
       const int jb[2] = {kb0[j], kb1[j]};
       bool any = false;
@@ -487,12 +507,12 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
     __syncthreads();
     ++used;
   }
-  if (LDS) {
+  if (LDS && do_vel) {
     for (int j = lane; j < C; j += 64) c.ddv[base + j] = dkey[j];
   }
   if (lane == 0 && a.world_iters) {
-    a.world_iters[2 * w] = pos_iters;
-    a.world_iters[2 * w + 1] = used;
+    if (do_pos) a.world_iters[2 * w] = pos_iters;
+    if (do_vel) a.world_iters[2 * w + 1] = used;
   }
 }
 
@@ -506,8 +526,8 @@ __device__ __forceinline__ bool fits_registers(int C, int n) { return C <= 64 * 
 template <typename R>
 __device__ void resolve_hbm(const RS<R>& a, int w, int C, char* lds, int lane) {
   if (a.world_iters && lane == 0) {
-    a.world_iters[2 * w] = 0;
-    a.world_iters[2 * w + 1] = 0;
+    if (a.mode & RM_POS) a.world_iters[2 * w] = 0;
+    if (a.mode & RM_VEL) a.world_iters[2 * w + 1] = 0;
   }
   // ResolveContacts' guard, contact.go:210-212
   if (a.overflow[0] != 0 || C <= 0 || a.dt <= R(0)) return;
@@ -598,12 +618,15 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
   R* s_pt = (R*)(lds + L::pt);
   uint8_t* s_flags = (uint8_t*)(lds + L::flags);
   uint32_t& s_bcid = *(uint32_t*)(lds + L::bcid);
+  const bool do_pos = (a.mode & RM_POS) != 0;
+  const bool do_vel = (a.mode & RM_VEL) != 0 && (!a.world_mask || a.world_mask[w]);
   if (a.world_iters && lane == 0) {
-    a.world_iters[2 * w] = 0;
-    a.world_iters[2 * w + 1] = 0;
+    if (do_pos) a.world_iters[2 * w] = 0;
+    if (a.mode & RM_VEL) a.world_iters[2 * w + 1] = 0;
   }
   // ResolveContacts' guard, contact.go:210-212
   if (a.overflow[0] != 0 || C <= 0 || a.dt <= R(0)) return;
+  if (!do_pos && !do_vel) return;
 
   const Bodies<R>& b = a.b;
   const Contacts<R>& c = a.c;
@@ -613,7 +636,10 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
 
   for (int i = lane; i < n; i += 64) {
     const int g = b0w + i;
-    const V3<R> p = ld3(b.pos, S, g);
+    // body positions at prepare time: the current ones when this launch
+    // prepares, else the snapshot the preparing launch stored
+    const V3<R> p = ld3(do_pos ? b.pos : a.pos0, S, g);
+    if (do_pos) st3(a.pos0, S, g, p);
     const V3<R> la = mul(ld3(b.last_acc, S, g), a.dt);
     s_pos0[i] = p.x;
     s_pos0[kRegBodies + i] = p.y;
@@ -646,7 +672,22 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
     ids[s] = 0xFFFFFFFFu;
     nrm[s] = V3<R>{R(0), R(0), R(0)};
     if constexpr (!PT_LDS) ptr[s] = nrm[s];
-    if (j < C) {
+    if (j < C && !do_pos) {  // velocity phase only: reload what prepare stored
+      const int64_t g = base + j;
+      const int c0 = c.body0[g], c1 = c.body1[g];
+      const V3<R> nn = ld3(c.normal, CS, g);
+      const V3<R> p = ld3(c.point, CS, g);
+      s_bs[j] = basis_scale(nn);
+      ids[s] = uint32_t(c0 - b0w) | (uint32_t(c1 >= 0 ? c1 - b0w : int(kNoBody)) << 16);
+      nrm[s] = nn;
+      if constexpr (PT_LDS) {
+        s_pt[j] = p.x;
+        s_pt[CAP + j] = p.y;
+        s_pt[2 * CAP + j] = p.z;
+      } else {
+        ptr[s] = p;
+      }
+    } else if (j < C) {
       const int64_t g = base + j;
       int c0 = c.body0[g], c1 = c.body1[g];
       V3<R> nn = ld3(c.normal, CS, g);
@@ -694,7 +735,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
 
   // adjustPositions, contact.go:233-283
   int64_t used = 0;
-  while (used < maxIter) {
+  while (do_pos && used < maxIter) {
     R best = R(kPositionEpsilon);
     int bi = INT_MAX;
 #pragma unroll
@@ -776,13 +817,13 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
     const int j = 64 * s + lane;
     cvl[s] = V3<R>{R(0), R(0), R(0)};
     if (j < C) {
-      c.pen[base + j] = key[s];
+      if (do_pos) c.pen[base + j] = key[s];
       key[s] = c.ddv[base + j];
       cvl[s] = ld3(c.cvel, CS, base + j);
     }
   }
   used = 0;
-  while (used < maxIter) {
+  while (do_vel && used < maxIter) {
     R best = R(kVelocityEpsilon);
     int bi = INT_MAX;
 #pragma unroll
@@ -829,6 +870,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
     basis_from_scale(n, s_bs[bi], m);
     V3<R> vc[2], rc[2];
     apply_velocity_change(a, kb, rel, m, best, cvk, vc, rc, lane);
+    if ((a.mode & RM_LABEL) && lane == 0) add_label(a, base + bi, vc, rc);
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
       if (64 * s + lane >= C) continue;
@@ -862,18 +904,20 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
     __syncthreads();
     ++used;
   }
+  if (do_vel) {
 #pragma unroll
-  for (int s = 0; s < SLOTS; ++s) {
-    const int j = 64 * s + lane;
-    if (j < C) {
-      c.ddv[base + j] = key[s];
-      st3(c.cvel, CS, base + j, cvl[s]);
+    for (int s = 0; s < SLOTS; ++s) {
+      const int j = 64 * s + lane;
+      if (j < C) {
+        c.ddv[base + j] = key[s];
+        st3(c.cvel, CS, base + j, cvl[s]);
+      }
     }
   }
   for (int i = lane; i < n; i += 64) b.flags[b0w + i] = s_flags[i];
   if (lane == 0 && a.world_iters) {
-    a.world_iters[2 * w] = pos_iters;
-    a.world_iters[2 * w + 1] = used;
+    if (do_pos) a.world_iters[2 * w] = pos_iters;
+    if (do_vel) a.world_iters[2 * w + 1] = used;
   }
 }
 
@@ -926,6 +970,10 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   a.dt = L.dt;
   a.iter_mult = L.iter_mult;
   a.iter_fixed = L.iter_fixed;
+  a.mode = L.mode;
+  a.world_mask = L.world_mask;
+  a.pos0 = L.pos0;
+  a.labels = L.labels;
   if (a.n_worlds <= 0) return hipSuccess;
   k_resolve<R><<<a.n_worlds, 64, 0, st>>>(a);
   return hipGetLastError();

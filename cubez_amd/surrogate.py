@@ -1,0 +1,114 @@
+"""The learned resolver surrogate: data collection, training and use.
+
+The network (40 -> 256 -> 256 -> 256 -> 12, SiLU, BF16 weights on the
+matrix cores) is trained on the GPU by the library's own forward and
+backward kernels (cbz_sur_train) on contacts the exact resolver generates
+(cbz_sur_collect).  The exact resolver remains the judge at run time: every
+world whose residual closing velocity after the surrogate exceeds
+velocityEpsilon + tol is re-solved exactly.
+
+    python -m cubez_amd.surrogate train --out cubez_amd/models/surrogate.npz
+"""
+import argparse
+import os
+import time
+
+import numpy as np
+
+from . import scenes
+from .world import World
+
+LAYERS = ((40, 256), (256, 256), (256, 256), (256, 12))
+DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "surrogate.npz")
+# residual tolerance: 1 mm of extra penetration over one step of 1/60 s
+DEFAULT_TOL = 1e-3 * 60.0
+
+
+def unpack(params):
+    """Flat parameters -> [(W [out x in], b [out]) per layer]."""
+    out, off = [], 0
+    for n_in, n_out in LAYERS:
+        W = params[off:off + n_in * n_out].reshape(n_out, n_in)
+        off += n_in * n_out
+        b = params[off:off + n_out]
+        off += n_out
+        out.append((W, b))
+    assert off == len(params)
+    return out
+
+
+def collect(w, seeds, n_worlds=32, steps=600, at=(200, 400, 600), iter_per_contact=4, reserve=2_600_000,
+            make=None):
+    """Run batched scenes and log the exact resolver's contacts at the given
+    steps; returns the dataset size."""
+    make = make or (lambda s: scenes.batched(s, n_worlds))
+    w.sur_reserve(reserve)
+    for seed in seeds:
+        sc = make(seed)
+        w.load(sc)
+        done = 0
+        for k in sorted(at):
+            if k - 1 > done:
+                w.step(sc.dt, k - 1 - done, iter_per_contact)
+            w.integrate(sc.dt)
+            w.narrowphase()
+            w.sur_collect(sc.dt, iter_per_contact)
+            done = k
+    return w.sur_dataset(download=False)
+
+
+def train(w, steps=3000, batch=4096, lr=1e-3, seed=0, init=True):
+    if init:
+        w.sur_init(seed)
+    losses = []
+    # a short warm-up at a lower rate, then the main rate
+    for s, r in ((min(200, steps), lr * 0.3), (max(steps - 200, 0), lr)):
+        if s:
+            losses.append(w.sur_train(s, batch, r, seed + len(losses)))
+    return np.concatenate(losses) if losses else np.zeros(0)
+
+
+def save(path, params, norm, meta=None):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez(path, params=np.asarray(params, np.float32), norm=np.asarray(norm, np.float32),
+             meta=np.array(repr(meta or {})))
+
+
+def load(path=DEFAULT_MODEL):
+    d = np.load(path, allow_pickle=False)
+    return d["params"], d["norm"]
+
+
+def install(w, path=DEFAULT_MODEL):
+    params, norm = load(path)
+    w.sur_set_norm(norm)
+    w.sur_set_params(params)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train", help="collect exact-resolver contacts and train on the GPU")
+    t.add_argument("--out", default=DEFAULT_MODEL)
+    t.add_argument("--seeds", type=int, default=64)
+    t.add_argument("--worlds", type=int, default=32)
+    t.add_argument("--steps", type=int, default=3000)
+    t.add_argument("--batch", type=int, default=4096)
+    t.add_argument("--lr", type=float, default=1e-3)
+    a = ap.parse_args()
+    if a.cmd == "train":
+        with World(scenes.batched(0, 2)) as w:
+            t0 = time.perf_counter()
+            n = collect(w, range(a.seeds), n_worlds=a.worlds)
+            t1 = time.perf_counter()
+            losses = train(w, a.steps, a.batch, a.lr)
+            t2 = time.perf_counter()
+            meta = {"rows": n, "seeds": a.seeds, "worlds_per_seed": a.worlds, "steps": a.steps,
+                    "batch": a.batch, "lr": a.lr, "loss_first": float(losses[:50].mean()),
+                    "loss_last": float(losses[-50:].mean()), "collect_s": t1 - t0, "train_s": t2 - t1}
+            save(a.out, w.sur_get_params(), w.sur_get_norm(), meta)
+            print(meta)
+
+
+if __name__ == "__main__":
+    main()

@@ -168,3 +168,93 @@ class World:
         out = np.empty_like(a)
         self._chk(self._lib.cbz_selftest_f64(self._h, int(op), len(a), _ptr(a), _ptr(b), _ptr(out)))
         return out
+
+    # -- learned resolver surrogate ---------------------------------------------
+    N_FEAT, N_OUT = 40, 12
+
+    def sur_num_params(self):
+        return int(self._lib.cbz_sur_num_params())
+
+    def sur_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.sur_num_params()
+        self._chk(self._lib.cbz_sur_set_params(self._h, _ptr(p)))
+
+    def sur_get_params(self):
+        p = np.empty(self.sur_num_params(), np.float32)
+        self._chk(self._lib.cbz_sur_get_params(self._h, _ptr(p)))
+        return p
+
+    def sur_set_norm(self, n):
+        n = np.ascontiguousarray(n, np.float32)
+        assert n.size == 104
+        self._chk(self._lib.cbz_sur_set_norm(self._h, _ptr(n)))
+
+    def sur_get_norm(self):
+        n = np.empty(104, np.float32)
+        self._chk(self._lib.cbz_sur_get_norm(self._h, _ptr(n)))
+        return n
+
+    def sur_reserve(self, rows):
+        self._chk(self._lib.cbz_sur_reserve(self._h, int(rows)))
+
+    def sur_collect(self, dt, iter_per_contact=4):
+        n = ctypes.c_int64(0)
+        self._chk(self._lib.cbz_sur_collect(self._h, float(dt), int(iter_per_contact), ctypes.byref(n)))
+        return n.value
+
+    def sur_dataset(self, download=True):
+        n = ctypes.c_int64(0)
+        self._chk(self._lib.cbz_sur_dataset(self._h, ctypes.byref(n), None, None))
+        if not download:
+            return n.value
+        X = np.empty((n.value, self.N_FEAT), np.float32)
+        Y = np.empty((n.value, self.N_OUT), np.float32)
+        self._chk(self._lib.cbz_sur_dataset(self._h, ctypes.byref(n), _ptr(X), _ptr(Y)))
+        return X, Y
+
+    def sur_load_dataset(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        self._chk(self._lib.cbz_sur_load_dataset(self._h, len(X), _ptr(X), _ptr(Y)))
+
+    def sur_init(self, seed=0):
+        self._chk(self._lib.cbz_sur_init(self._h, int(seed)))
+
+    def sur_train(self, steps, batch=4096, lr=1e-3, seed=0):
+        losses = np.empty(max(int(steps), 1), np.float32)
+        self._chk(self._lib.cbz_sur_train(self._h, int(steps), int(batch), float(lr), int(seed), _ptr(losses)))
+        return losses[:steps]
+
+    def sur_grad(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        g = np.empty(self.sur_num_params(), np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(self._lib.cbz_sur_grad(self._h, len(X), _ptr(X), _ptr(Y), _ptr(g), _ptr(loss)))
+        return g, float(loss[0])
+
+    def sur_predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.empty((len(X), self.N_OUT), np.float32)
+        self._chk(self._lib.cbz_sur_predict(self._h, len(X), _ptr(X), _ptr(out)))
+        return out
+
+    def sur_resolve(self, dt, iter_per_contact=4, tol=0.06):
+        n = ctypes.c_int64(0)
+        self._chk(self._lib.cbz_sur_resolve(self._h, float(dt), int(iter_per_contact), float(tol), ctypes.byref(n)))
+        return n.value
+
+    def sur_world_mask(self):
+        m = np.empty(self.n_worlds, np.uint8)
+        self._chk(self._lib.cbz_sur_world_mask(self._h, _ptr(m)))
+        return m
+
+    def step_sur(self, dt, n_steps=1, iter_per_contact=4, tol=0.06, counts=False):
+        """n_steps with the surrogate resolver; returns the per-step fallback
+        counts (and [n_steps, n_worlds] contact counts with counts=True)."""
+        buf = np.empty((n_steps, self.n_worlds), np.int64) if counts else None
+        fb = np.empty(max(n_steps, 1), np.int64)
+        self._chk(self._lib.cbz_step_sur(self._h, float(dt), int(n_steps), int(iter_per_contact), float(tol),
+                                         _ptr(buf), _ptr(fb)))
+        return (fb[:n_steps], buf) if counts else fb[:n_steps]

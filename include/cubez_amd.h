@@ -182,6 +182,58 @@ int cbz_phase_times(cbz_ctx *ctx, double *ms6, int64_t *counts4);
  * op 0: a/b, 1: sqrt(a), 2: a*b, 3: a+b, 4: a-b, elementwise over n. */
 int cbz_selftest_f64(cbz_ctx *ctx, int32_t op, int64_t n, const double *a, const double *b, double *out);
 
+/* --- learned resolver surrogate --------------------------------------------
+ * A 40 -> 256 -> 256 -> 256 -> 12 MLP (SiLU, BF16 weights and activations,
+ * FP32 accumulation, on the matrix cores) predicts, per contact, the
+ * velocity and rotation changes the velocity phase of ResolveContacts
+ * (adjustVelocities, contact.go:390-445) gives the contact's two bodies.
+ * No counterpart in the reference: it is the learned part of this
+ * implementation.  The exact method stays the judge: per world, after the
+ * exact position phase the summed predictions are applied only if every
+ * contact's remaining desired velocity change is at most
+ * velocityEpsilon (0.01, contact.go:12) + tol; otherwise that world runs the
+ * exact velocity phase from its unchanged state, i.e. gets exactly the
+ * plain ResolveContacts result.
+ *
+ * Features (40, per contact): normal 3, relativeContactPosition[0] 3, [1] 3,
+ * body0 - body1 velocity 3 and rotation 3, inverse masses 2, world inverse
+ * inertias 9 + 9, penetration after the position phase, desiredDeltaVelocity,
+ * contactVelocity.x, 1/(contacts of body0), 1/(contacts of body1).
+ * Outputs (12): dv0 3, dw0 3, dv1 3, dw1 3.
+ * Parameters: per layer W [out x in] row-major then b [out], FP32.
+ * Normalisation: in_mean 40, in_std 40, out_mean 12, out_std 12. */
+int64_t cbz_sur_num_params(void);
+int cbz_sur_set_params(cbz_ctx *ctx, const float *params);
+int cbz_sur_get_params(cbz_ctx *ctx, float *params);
+int cbz_sur_set_norm(cbz_ctx *ctx, const float *norm104);
+int cbz_sur_get_norm(cbz_ctx *ctx, float *norm104);
+/* Training data: cbz_sur_collect resolves the current contacts exactly (the
+ * same result as cbz_resolve) and appends every contact's features and
+ * labels (the summed changes the exact velocity phase gave its bodies) to a
+ * device dataset of cbz_sur_reserve rows. */
+int cbz_sur_reserve(cbz_ctx *ctx, int64_t rows);
+int cbz_sur_collect(cbz_ctx *ctx, double dt, int64_t iter_per_contact, int64_t *rows_added);
+int cbz_sur_dataset(cbz_ctx *ctx, int64_t *rows, float *features, float *labels);
+int cbz_sur_load_dataset(cbz_ctx *ctx, int64_t rows, const float *features, const float *labels);
+/* normalisation from the dataset, He-uniform weights from the seed */
+int cbz_sur_init(cbz_ctx *ctx, uint64_t seed);
+/* Adam on the mean-squared error of normalised labels, batches sampled from
+ * the dataset; losses[steps] (may be NULL) */
+int cbz_sur_train(cbz_ctx *ctx, int32_t steps, int32_t batch, float lr, uint64_t seed, float *losses);
+/* loss and parameter gradient on given rows (raw features and labels) */
+int cbz_sur_grad(cbz_ctx *ctx, int32_t rows, const float *features, const float *labels, float *grad, float *loss);
+/* predictions (denormalised) for given rows of raw features */
+int cbz_sur_predict(cbz_ctx *ctx, int32_t rows, const float *features, float *out);
+/* ResolveContacts with the surrogate and per-world fallback (tol in m/s; a
+ * negative tol sends every world to the exact method).  *n_fallback
+ * receives the number of worlds that fell back; cbz_sur_world_mask which. */
+int cbz_sur_resolve(cbz_ctx *ctx, double dt, int64_t iter_per_contact, double tol, int64_t *n_fallback);
+int cbz_sur_world_mask(cbz_ctx *ctx, uint8_t *mask);
+/* cbz_step with the surrogate resolver; fallbacks[n_steps] (may be NULL)
+ * receives the fallback count of every step */
+int cbz_step_sur(cbz_ctx *ctx, double dt, int32_t n_steps, int64_t iter_per_contact, double tol, int64_t *counts,
+                 int64_t *fallbacks);
+
 #ifdef __cplusplus
 }
 #endif

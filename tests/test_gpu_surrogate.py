@@ -1,0 +1,219 @@
+"""The learned resolver surrogate on the GPU: the MFMA forward and backward
+kernels against an FP64 numpy MLP, the data logger against the exact
+resolver, the fallback contract, and what training gains."""
+import numpy as np
+import pytest
+
+from cubez_amd import World, scenes, surrogate
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    """Round float32 to bfloat16 (nearest even), returned as float32."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def ref_forward(params, norm, X):
+    """FP64 forward with the GPU's roundings: BF16 inputs, weights and hidden
+    activations, FP32-exact accumulation modelled in FP64."""
+    mi, si, mo, so = norm[:40], norm[40:80], norm[80:92], norm[92:104]
+    h = bf16((X - mi) / si).astype(np.float64)
+    acts, zs = [h], []
+    layers = surrogate.unpack(params)
+    for l, (W, b) in enumerate(layers):
+        z = h @ bf16(W).astype(np.float64).T + b
+        zs.append(z)
+        if l < len(layers) - 1:
+            h = bf16(silu(z).astype(np.float32)).astype(np.float64)
+            acts.append(h)
+        else:
+            h = z
+    return h, h * so + mo, acts, zs
+
+
+def ref_grad(params, norm, X, Y):
+    """FP64 backprop of mean((y - t)^2) with normalised targets."""
+    mo, so = norm[80:92], norm[92:104]
+    yhat, _, acts, zs = ref_forward(params, norm, X)
+    t = (Y - mo) / so
+    n = yhat.size
+    loss = np.sum((yhat - t) ** 2) / n
+    dz = 2.0 * (yhat - t) / n
+    layers = surrogate.unpack(params)
+    grads = [None] * len(layers)
+    for l in range(len(layers) - 1, -1, -1):
+        W, b = layers[l]
+        grads[l] = (dz.T @ acts[l], dz.sum(axis=0))
+        if l > 0:
+            da = dz @ bf16(W).astype(np.float64)
+            s = 1.0 / (1.0 + np.exp(-zs[l - 1]))
+            dz = da * s * (1.0 + zs[l - 1] * (1.0 - s))
+    flat = np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in grads])
+    return flat, loss
+
+
+@pytest.fixture(scope="module")
+def model():
+    rng = np.random.default_rng(0)
+    w = World(scenes.batched(0, 2))
+    n = w.sur_num_params()
+    params = np.zeros(n, np.float32)
+    off = 0
+    for n_in, n_out in surrogate.LAYERS:
+        params[off:off + n_in * n_out] = rng.uniform(-1, 1, n_in * n_out) * np.sqrt(6.0 / n_in)
+        off += n_in * n_out
+        params[off:off + n_out] = rng.uniform(-0.1, 0.1, n_out)
+        off += n_out
+    norm = np.concatenate([rng.normal(0, 1, 40), rng.uniform(0.5, 2, 40), rng.normal(0, 1, 12),
+                           rng.uniform(0.5, 2, 12)]).astype(np.float32)
+    w.sur_set_norm(norm)
+    w.sur_set_params(params)
+    yield w, params, norm
+    w.close()
+
+
+def test_forward_matches_fp64_reference(model):
+    w, params, norm = model
+    rng = np.random.default_rng(1)
+    X = rng.normal(0, 1.5, (1000, 40)).astype(np.float32)
+    got = w.sur_predict(X)
+    _, want, _, _ = ref_forward(params, norm, X.astype(np.float64))
+    err = np.abs(got - want) / (np.abs(want).max(axis=0) + 1e-6)
+    assert err.max() < 2e-2, err.max()
+    assert np.corrcoef(got.ravel(), want.ravel())[0, 1] > 0.9999
+
+
+def test_gradient_matches_fp64_reference(model):
+    w, params, norm = model
+    rng = np.random.default_rng(2)
+    X = rng.normal(0, 1.0, (700, 40)).astype(np.float32)
+    Y = rng.normal(0, 1.0, (700, 12)).astype(np.float32)
+    g, loss = w.sur_grad(X, Y)
+    gref, lref = ref_grad(params.astype(np.float64), norm.astype(np.float64), X.astype(np.float64),
+                          Y.astype(np.float64))
+    assert abs(loss - lref) / lref < 1e-2
+    rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    assert rel < 3e-2, rel
+    # per layer as well (a transposed operand would fail here)
+    off = 0
+    for n_in, n_out in surrogate.LAYERS:
+        k = n_in * n_out + n_out
+        a, b = g[off:off + k], gref[off:off + k]
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 5e-2
+        assert np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.998
+        off += k
+
+
+def _settled(seed=3, worlds=12, steps=240):
+    sc = scenes.batched(seed, worlds, 8, 120)
+    return sc, steps
+
+
+def test_collect_is_the_exact_resolve_and_labels_sum_to_body_changes():
+    sc, steps = _settled()
+    with World(sc) as a, World(sc) as b:
+        a.step(sc.dt, steps, 4)
+        b.step(sc.dt, steps, 4)
+        a.integrate(sc.dt)
+        b.integrate(sc.dt)
+        a.narrowphase()
+        b.narrowphase()
+        v_before = a.get("vel").copy()
+        w_before = a.get("rot").copy()
+        a.sur_reserve(200_000)
+        n = a.sur_collect(sc.dt, 4)
+        b.resolve(sc.dt, 4)
+        for f in ("pos", "quat", "vel", "rot", "flags"):
+            assert np.array_equal(a.get(f), b.get(f)), f
+        X, Y = a.sur_dataset()
+        assert len(X) == n == int(a.contact_counts().sum())
+        dv = np.zeros_like(v_before)
+        dw = np.zeros_like(w_before)
+        off = 0
+        for wi in range(sc.n_worlds):
+            c = a.contacts(wi, internals=False)
+            m = len(c["pen"])
+            for j in range(m):
+                y = Y[off + j].astype(np.float64)
+                dv[c["body0"][j]] += y[0:3]
+                dw[c["body0"][j]] += y[3:6]
+                if c["body1"][j] >= 0:
+                    dv[c["body1"][j]] += y[6:9]
+                    dw[c["body1"][j]] += y[9:12]
+            off += m
+        assert np.allclose(dv, a.get("vel") - v_before, atol=1e-5, rtol=1e-4)
+        assert np.allclose(dw, a.get("rot") - w_before, atol=1e-5, rtol=1e-4)
+
+
+def _plain_and_sur(tol, params_fn):
+    sc, steps = _settled(5)
+    with World(sc) as p, World(sc) as s:
+        p.step(sc.dt, steps, 4)
+        s.step(sc.dt, steps, 4)
+        s.sur_set_norm(np.concatenate([np.zeros(40), np.ones(40), np.zeros(12), np.ones(12)]).astype(np.float32))
+        s.sur_set_params(params_fn(s.sur_num_params()))
+        for _ in range(5):
+            p.integrate(sc.dt)
+            s.integrate(sc.dt)
+            p.narrowphase()
+            s.narrowphase()
+            p.resolve(sc.dt, 4)
+            nf = s.sur_resolve(sc.dt, 4, tol)
+        return p.state(), s.state(), nf, s.sur_world_mask(), sc
+
+
+def test_negative_tolerance_falls_back_everywhere_and_equals_plain():
+    ps, ss, nf, mask, sc = _plain_and_sur(-1.0, lambda n: np.zeros(n, np.float32))
+    for k in ps:
+        assert np.array_equal(ps[k], ss[k]), k
+
+
+def test_nan_network_falls_back_and_equals_plain():
+    ps, ss, nf, mask, sc = _plain_and_sur(1e30, lambda n: np.full(n, np.nan, np.float32))
+    assert nf > 0
+    for k in ps:
+        assert np.array_equal(ps[k], ss[k]), k
+
+
+def test_infinite_tolerance_accepts_every_world():
+    ps, ss, nf, mask, sc = _plain_and_sur(1e30, lambda n: np.zeros(n, np.float32))
+    assert nf == 0
+
+
+def test_training_gains_over_untrained_network():
+    """Train briefly on contacts of 12 seeds; on fresh seeds the trained
+    network must leave fewer worlds to the exact method than an untrained
+    (randomly initialised) one, and its loss must fall."""
+    with World(scenes.batched(0, 2)) as w:
+        surrogate.collect(w, range(12), n_worlds=24, at=(200, 300, 400), reserve=500_000)
+        w.sur_init(1)
+        untrained = w.sur_get_params()
+        losses = surrogate.train(w, steps=800, batch=2048, lr=1e-3, seed=1, init=False)
+        trained = w.sur_get_params()
+        norm = w.sur_get_norm()
+        assert losses[-50:].mean() < 0.5 * losses[:20].mean()
+
+    def fallback_rate(params):
+        tot = fall = 0
+        for seed in (1000, 1001):
+            sc = scenes.batched(seed, 24)
+            with World(sc) as e:
+                e.step(sc.dt, 300, 4)
+                e.sur_set_norm(norm)
+                e.sur_set_params(params)
+                fb = e.step_sur(sc.dt, 20, 4, surrogate.DEFAULT_TOL)
+                tot += 20 * sc.n_worlds
+                fall += int(fb.sum())
+        return fall / tot
+
+    r_trained = fallback_rate(trained)
+    r_untrained = fallback_rate(untrained)
+    print("fallback rate: trained", r_trained, "untrained", r_untrained)
+    assert r_trained < r_untrained
