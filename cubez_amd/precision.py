@@ -11,9 +11,14 @@ from .world import World
 
 
 def _quat_angle(q1, q2):
-    """Rotation angle between unit quaternions (rad), per body."""
-    d = np.abs(np.sum(q1 * q2, axis=1))
-    return 2.0 * np.arccos(np.clip(d, 0.0, 1.0))
+    """Rotation angle between unit quaternions (rad), per body, as
+    4 atan2(|q1 - q2|, |q1 + q2|) with q2's sign matched to q1 (arccos of the
+    dot product loses half the digits near zero angle)."""
+    q1 = np.asarray(q1, np.float64)
+    q2 = np.asarray(q2, np.float64)
+    s = np.where(np.sum(q1 * q2, axis=1) < 0, -1.0, 1.0)[:, None]
+    q2 = q2 * s
+    return 4.0 * np.arctan2(np.linalg.norm(q1 - q2, axis=1), np.linalg.norm(q1 + q2, axis=1))
 
 
 def compare(scene, checkpoints=(1, 60, 600), iter_per_contact=None, per_step=False):

@@ -15,7 +15,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcbz_oracle.so")
+# CBZ_ORACLE_LIB points the tests at another build of the oracle (the
+# planted-mistake builds of tests/test_oracle_mutants.py)
+LIB_PATH = os.environ.get("CBZ_ORACLE_LIB") or os.path.join(HERE, "libcbz_oracle.so")
 
 CONTACT_DTYPE = np.dtype([
     ("body0", np.int32), ("body1", np.int32), ("friction", np.float64), ("restitution", np.float64),

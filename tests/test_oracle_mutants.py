@@ -1,0 +1,51 @@
+"""Planted mistakes: each mutant is the oracle with one plausible
+transcription error (a term, sign, index, operand order or tie rule changed),
+compiled to a temporary library.  The oracle's own checks (known answers,
+closed forms, brute force) must fail on every one of them; a mutant they
+pass would be a part of the restatement they do not pin."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "oracle", "cbz_oracle.c")
+CFLAGS = ["-O2", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared"]
+
+# name -> (original line fragment, planted mistake)
+MUTANTS = {
+    "cross_product_term": ("r.v[1] = v->v[2] * v2->v[0] - v->v[0] * v2->v[2];",
+                           "r.v[1] = v->v[0] * v2->v[2] - v->v[2] * v2->v[0];"),
+    "position_tie_rule": ("if (contacts[i].Penetration > max) {", "if (contacts[i].Penetration >= max) {"),
+    "velocity_tie_rule": ("if (contacts[i].desiredDeltaVelocity > max) {",
+                          "if (contacts[i].desiredDeltaVelocity >= max) {"),
+    "velocity_update_sign": ("if (b == 1) sign = -1.0;", "if (b == 0) sign = -1.0;"),
+    "position_update_sign": ("if (b == 0) sign = -1.0;", "if (b == 1) sign = -1.0;"),
+    "basis_not_transposed": ("Vector3 contactVelocity = m3_transform_transpose(&c->contactToWorld, &velocity);",
+                             "Vector3 contactVelocity = m3_mul_vec(&c->contactToWorld, &velocity);"),
+    "sat_tie_rule": ("if (penetration < *smallestPenetration) {", "if (penetration <= *smallestPenetration) {"),
+    "restitution_limit": ("const Real velocityLimit = 0.25;", "const Real velocityLimit = 0.0;"),
+    "sleep_average_weights": ("b->motion = bias * b->motion + (1.0 - bias) * currentMotion;",
+                              "b->motion = (1.0 - bias) * b->motion + bias * currentMotion;"),
+    "relative_velocity_operands": (
+        "Vector3 velocity = v3_cross(&body->Rotation, &c->relativeContactPosition[bodyIndex]);",
+        "Vector3 velocity = v3_cross(&c->relativeContactPosition[bodyIndex], &body->Rotation);"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MUTANTS))
+def test_oracle_checks_catch_planted_mistake(name, tmp_path):
+    old, new = MUTANTS[name]
+    src = open(SRC).read()
+    assert src.count(old) == 1, "fragment not found once in the oracle: " + old
+    mut = tmp_path / "mutant.c"
+    mut.write_text(src.replace(old, new))
+    lib = tmp_path / "libmutant.so"
+    subprocess.run(["gcc", *CFLAGS, "-I", os.path.dirname(SRC), "-o", str(lib), str(mut), "-lm"], check=True)
+    env = dict(os.environ, CBZ_ORACLE_LIB=str(lib))
+    r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider", "-m", "not gpu",
+                        os.path.join(ROOT, "tests", "test_oracle_math.py"),
+                        os.path.join(ROOT, "tests", "test_oracle_physics.py")],
+                       cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    assert r.returncode != 0, f"mutant {name} passed every oracle check:\n{r.stdout[-2000:]}"

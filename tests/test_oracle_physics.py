@@ -260,6 +260,64 @@ def test_single_contact_position_then_velocity():
     assert abs(v1[0]) < abs(v0[0])  # friction removes tangential velocity (sticking)
 
 
+def test_position_tie_goes_to_the_lowest_index():
+    """Two independent sphere-plane contacts with bit-equal penetration and
+    one iteration: the reference's strict '>' from positionEpsilon resolves
+    contact 0 (contact.go:238-248), and only its body moves."""
+    o = world([{"shape": "sphere", "r": 0.5, "mass": 1.0, "pos": (-3, 0.45, 0)},
+               {"shape": "sphere", "r": 0.5, "mass": 1.0, "pos": (3, 0.45, 0)}], gravity=0.0)
+    c = o.narrowphase()
+    assert len(c) == 2 and c["pen"][0] == c["pen"][1] and c["body0"][0] == 0
+    y0 = o.get("pos")[:, 1].copy()
+    o.resolve(c, DT, 1)
+    y1 = o.get("pos")[:, 1]
+    assert math.isclose(y1[0], y0[0] + c["pen"][0], rel_tol=1e-12)
+    assert y1[1] == y0[1]
+
+
+def test_velocity_tie_goes_to_the_lowest_index():
+    """Equal desired velocity changes, penetrations below positionEpsilon, one
+    iteration: only contact 0's body gets an impulse (contact.go:394-405)."""
+    o = world([{"shape": "sphere", "r": 0.5, "mass": 1.0, "pos": (-3, 0.495, 0), "vel": (0, -1.0, 0)},
+               {"shape": "sphere", "r": 0.5, "mass": 1.0, "pos": (3, 0.495, 0), "vel": (0, -1.0, 0)}],
+              gravity=0.0)
+    c = o.narrowphase()
+    assert len(c) == 2
+    o.resolve(c, DT, 1)
+    v = o.get("vel")
+    assert v[0, 1] > 0.0
+    assert v[1, 1] == -1.0
+
+
+def test_sat_tie_keeps_the_earlier_axis():
+    """Axis-aligned unit cubes offset by (1.5, 1.5, 0) overlap 0.5 on x and
+    on y: the axes 0, 1, 3 and 4 tie and the strict '<' of tryAxis
+    (colliders.go:472) keeps axis 0, the first cube's x face, so the normal
+    is -x (opposing the centre offset), the penetration 0.5 and the point
+    the second cube's vertex (-1, +1, +1) in its frame: (0.5, 7.5, 1).  The
+    later edge axis z x y = -x ties too and has the same normal; only the
+    point (an edge midpoint) would tell it apart."""
+    o = world([{"half": (1, 1, 1), "mass": 1.0, "pos": (0, 5, 0)},
+               {"half": (1, 1, 1), "mass": 1.0, "pos": (1.5, 6.5, 0)}], planes=(), gravity=0.0)
+    c = o.narrowphase_pairs([0], [1])
+    assert len(c) == 1
+    assert np.array_equal(c["normal"][0], [-1.0, 0.0, 0.0])
+    assert c["pen"][0] == 0.5
+    assert np.array_equal(c["point"][0], [0.5, 7.5, 1.0])
+
+
+@pytest.mark.parametrize("vy, want", [(-0.2, 0.0), (-2.0, 0.2)])
+def test_restitution_only_above_the_velocity_limit(vy, want):
+    """calculateDesiredDeltaVelocity drops restitution when the closing speed
+    is below 0.25 m/s (contact.go:87-112): a slow contact ends at rest along
+    the normal, a fast one bounces at 0.1 of its approach speed."""
+    o = world([{"shape": "sphere", "r": 0.5, "mass": 1.0, "pos": (0, 0.495, 0), "vel": (0, vy, 0)}], gravity=0.0)
+    c = o.narrowphase()
+    assert len(c) == 1
+    o.resolve(c, DT, 4)
+    assert math.isclose(o.get("vel")[0, 1], want, rel_tol=1e-12, abs_tol=1e-12)
+
+
 def test_two_body_impulse_conserves_momentum():
     m0, m1 = 2.0, 5.0
     o = world([{"shape": "sphere", "r": 1.0, "mass": m0, "pos": (0, 10, 0), "vel": (3, 0.2, 0)},
