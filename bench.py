@@ -4,16 +4,23 @@
 Workload: 4096 independent worlds, each a ground plane and n ~ U{8..256}
 cubes (the batched-worlds configuration), advanced with the reference's
 step -- Integrate + CalculateDerivedData, the all-pairs narrowphase,
-ResolveContacts(4 x contacts) -- for K steps at dt = 1/60.  A step is one
-pass of that hot path over all worlds.  The metric is body-steps/s (bodies x
-steps / seconds) with the state resident in HBM.
+ResolveContacts(4 x contacts) -- for K steps at dt = 1/60 from the same
+initial state.  A step is one pass of that hot path over all worlds.  The
+metric is body-steps/s (bodies x steps / seconds) with the state resident in
+HBM, for the learned-surrogate resolver with exact fallback (FP32 state,
+tolerance 0.06 m/s of residual closing speed = 1 mm of penetration per
+step), next to the exact FP64 GPU resolver it is compared with.
 
 Prints one JSON line.  Besides the metric it reports:
-  roofline      K1 (integrate + derived data, HBM-bound) from its average
-                launch time measured with HIP events on the library's stream
-  kernels       per-kernel average ms of the timed steps
-  cpu_baseline  the FP64 oracle (a C restatement of the Go reference) on a
-                bounded sample of the same worlds, on this host's cores
+  exact_f64, exact_f32   the exact resolver (hipGraph-replayed step)
+  zero_prediction        the surrogate's polish started from a zero
+                         prediction: the same method without the network
+  fallback_rate          worlds re-solved by the plain method / world-steps
+  mlp                    the network's forward GEMMs: FLOP, time, MFMA share
+  roofline               K1 (integrate + derived data, HBM-bound) from its
+                         average launch time measured with HIP events
+  cpu_baseline           the FP64 oracle (a C restatement of the Go
+                         reference) on a bounded sample of the same worlds
 """
 import argparse
 import json
@@ -87,8 +94,9 @@ def cpu_baseline(scene, steps, budget_s):
 
 
 def run_path(World, sc, precision, steps, warmup):
-    """Warm up, then time `steps` steps from the initial state (graph
-    replay), then the same steps again with HIP events between kernels."""
+    """Exact resolver: warm up, then time `steps` steps from the initial
+    state (graph replay), then the same steps again with HIP events between
+    kernels."""
     w = World(sc, precision=precision)
     if warmup > 0:
         w.step(sc.dt, warmup, sc.iter_mult)
@@ -118,6 +126,41 @@ def run_path(World, sc, precision, steps, warmup):
     }
 
 
+def run_surrogate(World, sc, model, steps, warmup, tol, polish, zero=False):
+    """Surrogate resolver (FP32 state): timed steps from the initial state,
+    then the same steps again with HIP events around its phases."""
+    params, norm = model
+    w = World(sc, precision="f32")
+    w.sur_set_norm(norm)
+    w.sur_set_params(params)
+    if warmup > 0:
+        w.step_sur(sc.dt, warmup, sc.iter_mult, tol, polish, zero)
+    w.load(sc)
+    w.synchronize()
+    t0 = time.perf_counter()
+    fb = w.step_sur(sc.dt, steps, sc.iter_mult, tol, polish, zero)
+    w.synchronize()
+    wall = time.perf_counter() - t0
+    w.load(sc)
+    w.set_profiling(True)
+    w.step_sur(sc.dt, steps, sc.iter_mult, tol, polish, zero)
+    w.set_profiling(False)
+    ms, cnt = w.sur_phase_times()
+    w.close()
+    n_res, contacts, fallbacks, vel_iters = (int(x) for x in cnt)
+    names = ("position_phase", "features", "network", "per_body_sums", "velocity_phase")
+    return {
+        "value": sc.n_bodies * steps / wall,
+        "ms_per_step": 1000.0 * wall / steps,
+        "fallback_rate": float(fb.sum()) / (steps * sc.n_worlds),
+        "resolver_ms_per_step": {k: round(ms[i] / max(n_res, 1), 5) for i, k in enumerate(names)},
+        "contacts_per_step": contacts / max(n_res, 1),
+        "velocity_iterations_per_step": vel_iters / max(n_res, 1),
+        "network_ms_total": ms[2],
+        "contacts_total": contacts,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,9 +168,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--precision", choices=("f64", "f32"), default="f32",
-                    help="state precision of the timed path (the configuration names FP32)")
-    ap.add_argument("--single", action="store_true", help="skip the other precision's companion run")
+    ap.add_argument("--model", default=None, help="surrogate weights (default: the packaged model)")
+    ap.add_argument("--exact-only", action="store_true", help="time the exact resolver only")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle wall time")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -137,23 +179,43 @@ def main():
               "them with no exchange; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from cubez_amd import World, scenes
+    from cubez_amd import World, scenes, surrogate
 
     sc = scenes.batched(args.seed, args.worlds)
     nb = sc.n_bodies
-    main_path = run_path(World, sc, args.precision, args.steps, args.warmup)
-    other = "f64" if args.precision == "f32" else "f32"
-    side = run_path(World, sc, other, args.steps, args.warmup) if not args.single else None
+    model_path = args.model or surrogate.DEFAULT_MODEL
+    use_sur = not args.exact_only
+    if use_sur and not os.path.exists(model_path):
+        print(f"no surrogate model at {model_path} (python -m cubez_amd.surrogate train); "
+              "use --exact-only to time the exact resolver alone", file=sys.stderr)
+        sys.exit(1)
+    tol, polish = surrogate.DEFAULT_TOL, surrogate.DEFAULT_POLISH
 
-    kern = main_path["kernels"]
-    rs = 8 if args.precision == "f64" else 4
-    # K1 roofline: algorithmic bytes per launch / average launch time
+    exact32 = run_path(World, sc, "f32", args.steps, args.warmup)
+    exact64 = run_path(World, sc, "f64", args.steps, args.warmup)
+    sur = zero = None
+    if use_sur:
+        model = surrogate.load(model_path)
+        sur = run_surrogate(World, sc, model, args.steps, args.warmup, tol, polish)
+        zero = run_surrogate(World, sc, model, args.steps, args.warmup, tol, polish, zero=True)
+
+    kern = exact32["kernels"]
+    rs = 4
+    # K1 roofline (from the exact FP32 run, the same kernel the surrogate
+    # path launches): algorithmic bytes per launch / average launch time
     k1_awake = 29 * rs + 1 + 50 * rs + 1
     k1_sleep = 1 + 12 * rs + 12 * rs
-    steps_p = main_path["steps_profiled"]
-    integrated = main_path["integrated"]
+    steps_p = exact32["steps_profiled"]
+    integrated = exact32["integrated"]
     k1_bytes = (integrated * k1_awake + (nb * steps_p - integrated) * k1_sleep) / steps_p
     achieved = k1_bytes / (kern["integrate"] / 1000.0) / 1e9
+    main_path = sur if use_sur else exact32
+    workload = (f"batched worlds: {args.worlds} worlds, plane + n~U{{8..256}} cubes each ({nb} bodies), "
+                f"first {args.steps} steps at dt=1/60 from the seeded initial state, all-pairs narrowphase, "
+                f"maxIterations=4*C, FP32 state with FP64 SAT and contact basis, ")
+    workload += (f"resolver: exact position phase + BF16 MLP surrogate for the velocity phase, polished by "
+                 f"the exact worst-first loop to velocityEpsilon+{tol} m/s within {polish}*C iterations, "
+                 f"else the world is re-solved by the plain method" if use_sur else "resolver: exact")
     out = {
         "metric": METRIC,
         "value": main_path["value"],
@@ -165,31 +227,41 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.precision,
+        "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"batched worlds: {args.worlds} worlds, plane + n~U{{8..256}} cubes each "
-                        f"({nb} bodies), first {args.steps} steps at dt=1/60, all-pairs narrowphase, "
-                        f"exact resolver with maxIterations=4*C, "
-                        + ("FP32 state with FP64 SAT and contact basis" if args.precision == "f32"
-                           else "FP64 throughout"),
-            "worlds": args.worlds, "bodies": nb, "seed": args.seed, "resolver": "exact",
-            "precision": args.precision, "step": "hipGraph replay of one captured step",
-        },
-        "contacts_per_s": main_path["contacts_per_s"],
-        "contacts_per_step": main_path["contacts_per_step"],
-        "resolver_iterations_per_step": main_path["iters_per_step"],
-        "kernels_ms_per_step": kern,
-        "dominant_kernel": max(("integrate", "np_count", "np_scans", "np_emit", "resolve"), key=lambda k: kern[k]),
-        "roofline": {
-            "kernel": "k_integrate (K1: Integrate + CalculateDerivedData)",
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "bytes_per_launch": k1_bytes,
+            "workload": workload, "worlds": args.worlds, "bodies": nb, "seed": args.seed,
+            "resolver": "surrogate+fallback" if use_sur else "exact", "tolerance_m_per_s": tol,
+            "polish_per_contact": polish, "network": "40-256-256-256-12 SiLU, BF16 MFMA, FP32 accumulate",
         },
     }
-    if side is not None:
-        out["plain_" + other] = {k: side[k] for k in ("value", "ms_per_step", "kernels")}
+    if use_sur:
+        out["speedup_vs_exact_f64"] = sur["value"] / exact64["value"]
+        out["speedup_vs_exact_f32"] = sur["value"] / exact32["value"]
+        out["fallback_rate"] = sur["fallback_rate"]
+        out["velocity_iterations_per_step"] = {"surrogate": sur["velocity_iterations_per_step"],
+                                               "zero_prediction": zero["velocity_iterations_per_step"]}
+        out["surrogate_resolver_ms_per_step"] = sur["resolver_ms_per_step"]
+        out["contacts_per_step"] = sur["contacts_per_step"]
+        out["contacts_per_s"] = sur["contacts_per_step"] * sur["value"] / nb
+        flop = 2 * sum(i * o for i, o in surrogate.LAYERS)  # 288,768 per contact
+        tfl = flop * sur["contacts_total"] / (sur["network_ms_total"] / 1000.0) / 1e12 if sur["network_ms_total"] else 0
+        out["mlp"] = {"flop_per_contact": flop, "ms_per_step": sur["resolver_ms_per_step"]["network"],
+                      "achieved_tflops": tfl, "peak_tflops": 2500.0, "mfma_frac": tfl / 2500.0,
+                      "bound": "mfma", "kernel": "k_gemm (4 layers, forward)"}
+        out["zero_prediction"] = {k: zero[k] for k in ("value", "ms_per_step", "fallback_rate")}
+    else:
+        out["contacts_per_step"] = exact32["contacts_per_step"]
+        out["contacts_per_s"] = exact32["contacts_per_s"]
+    out["exact_f64"] = {k: exact64[k] for k in ("value", "ms_per_step", "kernels", "iters_per_step")}
+    out["exact_f32"] = {k: exact32[k] for k in ("value", "ms_per_step", "kernels", "iters_per_step")}
+    out["kernels_ms_per_step"] = kern
+    out["dominant_kernel"] = max(("integrate", "np_count", "np_scans", "np_emit", "resolve"), key=lambda k: kern[k])
+    out["roofline"] = {
+        "kernel": "k_integrate (K1: Integrate + CalculateDerivedData)",
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": k1_bytes,
+    }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.steps, args.cpu_budget)
     print(json.dumps(out), flush=True)

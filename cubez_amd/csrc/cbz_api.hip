@@ -63,10 +63,13 @@ struct cbz_context {
   int32_t* d_plane_off = nullptr;
 
   // all-pairs pair space
-  int32_t n_tiles = 0;
+  int32_t n_tiles = 0, n_tasks = 0;
   int32_t* d_tile_world = nullptr;
   int32_t* d_world_tile0 = nullptr;
-  int32_t* d_world_q = nullptr;
+  int32_t* d_task_world = nullptr;
+  int32_t* d_world_task0 = nullptr;
+  uint32_t* hits = nullptr;  // [cap] hit pairs of the last count pass
+  int32_t* n_hits = nullptr;
   // explicit-list pair space (grown on demand)
   int32_t l_tiles_cap = 0, l_pairs_cap = 0;
   int32_t* l_tile_world = nullptr;
@@ -117,6 +120,8 @@ struct cbz_context {
     int32_t* deg = nullptr;
     uint8_t* mask = nullptr;
     int32_t* nfall = nullptr;
+    float* delta = nullptr;  // [6][n_bodies] predicted change per body
+    void* vel0 = nullptr;    // [7][n_bodies] velocity, rotation, motion before it
     // training buffers
     int batch_cap = 0;
     int64_t* idx = nullptr;
@@ -141,9 +146,15 @@ struct cbz_context {
 
   // profiling
   bool prof = false;
-  hipEvent_t ev[8] = {};
+  // ev[0..7]: cbz_step phases; ev[8..13]: surrogate resolve phases
+  hipEvent_t ev[14] = {};
   double acc_ms[6] = {0, 0, 0, 0, 0, 0};
   int64_t acc_cnt[4] = {0, 0, 0, 0};  // steps, bodies integrated, contacts, resolver iterations
+  // surrogate resolves: ms [position phase, features, network, per-body
+  // sums, velocity phase], counts [resolves, contacts, fallbacks, velocity
+  // iterations]
+  double sur_ms[5] = {0, 0, 0, 0, 0};
+  int64_t sur_cnt[4] = {0, 0, 0, 0};
   uint32_t* d_awake = nullptr;
   bool count_awake = false;
   void* pos0 = nullptr;  // body positions at the last prepareContacts
@@ -248,6 +259,8 @@ void free_sur_buffers(cbz_ctx* c) {
   dfree(u.deg);
   dfree(u.mask);
   dfree(u.nfall);
+  dfree(u.delta);
+  dfree(u.vel0);
   u.inf_cap = 0;
 }
 
@@ -264,7 +277,10 @@ void free_scene(cbz_ctx* c) {
   dfree(c->d_plane_off);
   dfree(c->d_tile_world);
   dfree(c->d_world_tile0);
-  dfree(c->d_world_q);
+  dfree(c->d_task_world);
+  dfree(c->d_world_task0);
+  dfree(c->hits);
+  dfree(c->n_hits);
   dfree(c->l_tile_world);
   dfree(c->l_world_tile0);
   dfree(c->l_world_q);
@@ -418,7 +434,10 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles) {
     L.ps.n_tiles = c->n_tiles;
     L.ps.tile_world = c->d_tile_world;
     L.ps.world_tile0 = c->d_world_tile0;
-    L.ps.world_q = c->d_world_q;
+    L.ps.world_q = nullptr;
+    L.ps.n_tasks = c->n_tasks;
+    L.ps.task_world = c->d_task_world;
+    L.ps.world_task0 = c->d_world_task0;
     L.ps.list_off = nullptr;
     L.ps.list_one = nullptr;
     L.ps.list_two = nullptr;
@@ -427,6 +446,9 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles) {
     L.ps.tile_world = c->l_tile_world;
     L.ps.world_tile0 = c->l_world_tile0;
     L.ps.world_q = c->l_world_q;
+    L.ps.n_tasks = 0;
+    L.ps.task_world = nullptr;
+    L.ps.world_task0 = nullptr;
     L.ps.list_off = c->l_off;
     L.ps.list_one = c->l_one;
     L.ps.list_two = c->l_two;
@@ -440,6 +462,8 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles) {
   L.world_coff = c->world_coff;
   L.total = c->total;
   L.overflow = c->overflow;
+  L.hits = c->hits;
+  L.n_hits = c->n_hits;
   L.n_worlds = c->nw;
   HIPCHK(c, launch_fill_i32(c->overflow, 1, 0, c->stream));
   HIPCHK(c, launch_narrowphase<R>(L, c->stream, c->prof ? &c->ev[1] : nullptr));
@@ -449,11 +473,17 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles) {
 
 template <typename R>
 int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode = 3,
-               const uint8_t* mask = nullptr, float* labels = nullptr) {
+               uint8_t* mask = nullptr, float* labels = nullptr, double polish_tol = 0, int64_t polish_mult = 0) {
   ResolveLaunch<R> L;
   L.mode = mode;
   L.world_mask = mask;
   L.labels = labels;
+  if (mode & RM_POLISH) {
+    L.delta = c->sur.delta;
+    L.vel0 = (R*)c->sur.vel0;
+    L.polish_tol = (R)polish_tol;
+    L.polish_mult = polish_mult;
+  }
   L.pos0 = (R*)c->pos0;
   L.b = bodies_view<R>(c);
   L.c = contacts_view<R>(c);
@@ -706,30 +736,82 @@ int sur_ensure_inference(cbz_ctx* c, int64_t rows) {
   if ((rc = dalloc(c, &u.deg, (size_t)std::max<int64_t>(c->nb, 1)))) return rc;
   if ((rc = dalloc(c, &u.mask, (size_t)std::max(c->nw, 1)))) return rc;
   if ((rc = dalloc(c, &u.nfall, 1))) return rc;
+  const size_t nb = (size_t)std::max<int64_t>(c->nb, 1);
+  if ((rc = dalloc(c, &u.delta, 6 * nb))) return rc;
+  if ((rc = dalloc_bytes(c, &u.vel0, 7 * nb * real_size(c)))) return rc;
   u.inf_cap = Mpad;
   return CBZ_OK;
 }
 
-// One resolve with the surrogate: exact positions, predicted velocity phase
-// per world with its residual check, exact velocity phase where it fails.
+// Profiling of one surrogate resolve (events ev[8..12] recorded, ev[13]
+// recorded here): accumulates the phase times, the contact count, the
+// fallbacks and the velocity-phase iterations.
+int sur_profile(cbz_ctx* c, int64_t contacts) {
+  hipEvent_t* ev = c->ev + 8;
+  HIPCHK(c, hipEventRecord(ev[5], c->stream));
+  HIPCHK(c, hipEventSynchronize(ev[5]));
+  for (int k = 0; k < 5; ++k) {
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+    c->sur_ms[k] += t;
+  }
+  std::vector<uint8_t> m((size_t)c->nw);
+  std::vector<int64_t> it(2 * (size_t)c->nw);
+  HIPCHK(c, hipMemcpy(m.data(), c->sur.mask, m.size(), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(it.data(), c->world_iters, it.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+  int64_t fb = 0, vi = 0;
+  for (int32_t w = 0; w < c->nw; ++w) {
+    fb += m[w] ? 1 : 0;
+    vi += it[2 * (size_t)w + 1];
+  }
+  c->sur_cnt[0] += 1;
+  c->sur_cnt[1] += contacts;
+  c->sur_cnt[2] += fb;
+  c->sur_cnt[3] += vi;
+  return CBZ_OK;
+}
+
+// One resolve with the surrogate: the exact position phase; the network's
+// per-contact velocity changes summed per body; then per world the velocity
+// loop polishes from there to velocityEpsilon + tol within polish_mult * C
+// iterations, or the world is restored and given the plain velocity phase.
 template <typename R>
-int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int32_t* nfall_dst) {
+int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int64_t polish_mult, int32_t flags,
+                int32_t* nfall_dst) {
   auto& u = c->sur;
+  hipEvent_t* ev = c->ev + 8;  // position phase, features, network, sums, velocity phase
+  if (c->prof) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   int rc = do_resolve<R>(c, dt, mult, 0, RM_POS);
   if (rc) return rc;
-  const int64_t cap_rows = c->cap;  // contacts never exceed the slots
-  // the GEMM grid follows the contact count, read once per resolve
+  if (c->prof) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   int64_t tot = 0;
+  if (tol < 0 || (flags & CBZ_SUR_ZERO_PREDICTION)) {
+    // every world plain (tol < 0), or polished from a zero prediction
+    if (c->prof) {
+      HIPCHK(c, hipEventRecord(ev[2], c->stream));
+      HIPCHK(c, hipEventRecord(ev[3], c->stream));
+    }
+    HIPCHK(c, launch_fill_u8(u.mask, c->nw, tol < 0 ? 1 : 0, c->stream));
+    HIPCHK(c, launch_fill_f32(u.delta, 6 * c->nb, 0.f, c->stream));
+    if (c->prof) HIPCHK(c, hipEventRecord(ev[4], c->stream));
+    rc = do_resolve<R>(c, dt, mult, 0, RM_VEL | RM_POLISH, u.mask, nullptr, tol, polish_mult);
+    if (rc) return rc;
+    if (nfall_dst) HIPCHK(c, launch_count_u8(u.mask, c->nw, nfall_dst, c->stream));
+    return c->prof ? sur_profile(c, tot) : CBZ_OK;
+  }
+  // the GEMM grid follows the contact count, read once per resolve
   HIPCHK(c, hipMemcpyAsync(&tot, c->total, sizeof tot, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  tot = std::min(std::min(tot, c->cap), cap_rows);
+  tot = std::min(tot, c->cap);
   if ((rc = sur_ensure_inference(c, tot))) return rc;
   const int64_t Mpad = round32(std::max<int64_t>(tot, 1));
   // rows tot..Mpad-1 of X are padding: GEMM rows are independent and those
   // outputs are never read
   HIPCHK(c, launch_features<R>(bodies_view<R>(c), contacts_view<R>(c), tot, u.deg, u.norm, u.norm + kFeat, u.X,
                                nullptr, c->stream));
+  if (c->prof) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   if ((rc = sur_forward(c, u.X, tot, Mpad, false))) return rc;
+  if (c->prof) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   SurApply<R> a;
   a.b = bodies_view<R>(c);
   a.c = contacts_view<R>(c);
@@ -738,15 +820,15 @@ int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int32_t* nfall_
   a.body_off = c->d_body_off;
   a.overflow = c->overflow;
   a.Y = u.Yp;
+  a.delta = u.delta;
   a.mask = u.mask;
-  a.dt = (R)dt;
-  a.tol = (float)tol;
   a.max_contacts = kSurContacts;
-  HIPCHK(c, launch_sur_apply<R>(a, c->nw, c->stream));
-  rc = do_resolve<R>(c, dt, mult, 0, RM_VEL, u.mask);
+  HIPCHK(c, launch_sur_scatter<R>(a, c->nw, c->stream));
+  if (c->prof) HIPCHK(c, hipEventRecord(ev[4], c->stream));
+  rc = do_resolve<R>(c, dt, mult, 0, RM_VEL | RM_POLISH, u.mask, nullptr, tol, polish_mult);
   if (rc) return rc;
   if (nfall_dst) HIPCHK(c, launch_count_u8(u.mask, c->nw, nfall_dst, c->stream));
-  return CBZ_OK;
+  return c->prof ? sur_profile(c, tot) : CBZ_OK;
 }
 
 }  // namespace
@@ -779,15 +861,12 @@ int cbz_create(const cbz_config* cfg, cbz_ctx** out) {
     delete c;
     return CBZ_ERR_HIP;
   }
-  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
+  bool ok = hipSetDevice(c->device) == hipSuccess &&
+            hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+  for (auto& e : c->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  if (!ok) {
+    cbz_destroy(c);
     return CBZ_ERR_HIP;
-  }
-  for (auto& e : c->ev) {
-    if (hipEventCreate(&e) != hipSuccess) {
-      delete c;
-      return CBZ_ERR_HIP;
-    }
   }
   *out = c;
   return CBZ_OK;
@@ -856,27 +935,38 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   HIPCHK(c, hipMemcpy(c->d_body_off, body_off, (n_worlds + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_plane_off, plane_off, (n_worlds + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
 
-  // all-pairs tiles
-  std::vector<int32_t> wq(n_worlds), wt0(n_worlds + 1), tw;
-  int64_t nt = 0;
+  // all-pairs tiles: row-aligned, nch = ceil((np + n) / 64) per row; count
+  // tasks: nch column tiles x ceil(n / kRowBlock) row blocks per world
+  std::vector<int32_t> wt0(n_worlds + 1), tw, wk0(n_worlds + 1), kw;
+  int64_t nt = 0, nk = 0;
   for (int32_t w = 0; w < n_worlds; ++w) {
     const int64_t n = body_off[w + 1] - body_off[w], npw = plane_off[w + 1] - plane_off[w];
-    wq[w] = (int32_t)(n * (npw + n));
+    const int64_t nch = (npw + n + 63) / 64;
     wt0[w] = (int32_t)nt;
-    nt += (wq[w] + 63) / 64;
-    if (nt > INT32_MAX / 2) return fail(c, CBZ_ERR_ARG, "set_scene: pair space too large");
+    wk0[w] = (int32_t)nk;
+    nt += n * nch;
+    nk += nch * ((n + kRowBlock - 1) / kRowBlock);
+    if (nt > (int64_t)(INT32_MAX / 64) || nk > INT32_MAX / 2)
+      return fail(c, CBZ_ERR_ARG, "set_scene: pair space too large");
   }
   wt0[n_worlds] = (int32_t)nt;
+  wk0[n_worlds] = (int32_t)nk;
   tw.resize((size_t)nt);
-  for (int32_t w = 0; w < n_worlds; ++w)
+  kw.resize((size_t)nk);
+  for (int32_t w = 0; w < n_worlds; ++w) {
     for (int32_t t = wt0[w]; t < wt0[w + 1]; ++t) tw[t] = w;
+    for (int32_t k = wk0[w]; k < wk0[w + 1]; ++k) kw[k] = w;
+  }
   c->n_tiles = (int32_t)nt;
-  if ((rc = dalloc(c, &c->d_tile_world, (size_t)nt))) return rc;
+  c->n_tasks = (int32_t)nk;
+  if ((rc = dalloc(c, &c->d_tile_world, (size_t)std::max<int64_t>(nt, 1)))) return rc;
   if ((rc = dalloc(c, &c->d_world_tile0, (size_t)n_worlds + 1))) return rc;
-  if ((rc = dalloc(c, &c->d_world_q, (size_t)n_worlds))) return rc;
+  if ((rc = dalloc(c, &c->d_task_world, (size_t)std::max<int64_t>(nk, 1)))) return rc;
+  if ((rc = dalloc(c, &c->d_world_task0, (size_t)n_worlds + 1))) return rc;
   if (nt > 0) HIPCHK(c, hipMemcpy(c->d_tile_world, tw.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_world_tile0, wt0.data(), wt0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(c->d_world_q, wq.data(), wq.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (nk > 0) HIPCHK(c, hipMemcpy(c->d_task_world, kw.data(), nk * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_world_task0, wk0.data(), wk0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   if ((rc = ensure_tiles(c, (int32_t)std::max<int64_t>(nt, 1)))) return rc;
 
   // contacts
@@ -884,6 +974,8 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc_bytes(c, &c->c_body0, sizeof(int32_t) * (size_t)c->cap))) return rc;
   if ((rc = dalloc_bytes(c, &c->c_body1, sizeof(int32_t) * (size_t)c->cap))) return rc;
   if ((rc = dalloc_bytes(c, &c->c_real, rs * 26 * (size_t)c->cap))) return rc;
+  if ((rc = dalloc(c, &c->hits, (size_t)c->cap))) return rc;
+  if ((rc = dalloc(c, &c->n_hits, 1))) return rc;
   if ((rc = dalloc(c, &c->world_cnt, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->world_coff, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->total, 1))) return rc;
@@ -1374,6 +1466,17 @@ int cbz_phase_times(cbz_ctx* c, double* ms6, int64_t* counts4) {
   return CBZ_OK;
 }
 
+int cbz_sur_phase_times(cbz_ctx* c, double* ms5, int64_t* counts4) {
+  if (!c) return CBZ_ERR_ARG;
+  if (ms5)
+    for (int k = 0; k < 5; ++k) ms5[k] = c->sur_ms[k];
+  if (counts4)
+    for (int k = 0; k < 4; ++k) counts4[k] = c->sur_cnt[k];
+  for (double& v : c->sur_ms) v = 0;
+  for (int64_t& v : c->sur_cnt) v = 0;
+  return CBZ_OK;
+}
+
 int cbz_selftest_f64(cbz_ctx* c, int32_t op, int64_t n, const double* a, const double* b, double* out) {
   if (!c || n < 0 || (n > 0 && (!a || !b || !out))) return fail(c, CBZ_ERR_ARG, "selftest: bad arguments");
   if (n == 0) return CBZ_OK;
@@ -1590,18 +1693,9 @@ int cbz_sur_train(cbz_ctx* c, int32_t steps, int32_t batch, float lr, uint64_t s
   }
   const int64_t Mpad = round32(batch);
   const LayerOff o = layer_offsets();
-  std::vector<int64_t> idx((size_t)batch);
-  uint64_t x = seed * 0xD1B54A32D192ED03ull + 1;
-  auto next = [&]() {
-    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-  };
   const float* nm = u.norm;
   for (int32_t s = 0; s < steps; ++s) {
-    for (int i = 0; i < batch; ++i) idx[i] = int64_t(next() % uint64_t(u.ds_rows));
-    HIPCHK(c, hipMemcpyAsync(u.idx, idx.data(), batch * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_sample_idx(u.idx, batch, seed, (uint64_t)s, u.ds_rows, c->stream));
     HIPCHK(c, launch_gather_batch(u.dsX, u.dsY, u.idx, (int)Mpad, batch, nm, nm + kFeat, nm + 2 * kFeat,
                                   nm + 2 * kFeat + 2 * kOut, u.bX, u.At[0], u.bT, c->stream));
     if ((rc = sur_forward(c, u.bX, batch, Mpad, true))) return rc;
@@ -1679,15 +1773,19 @@ int cbz_sur_predict(cbz_ctx* c, int32_t rows, const float* X, float* Y) {
   return CBZ_OK;
 }
 
-int cbz_sur_resolve(cbz_ctx* c, double dt, int64_t iter_per_contact, double tol, int64_t* n_fallback) {
+int cbz_sur_resolve(cbz_ctx* c, double dt, int64_t iter_per_contact, double tol, int64_t polish_per_contact,
+                    int32_t flags, int64_t* n_fallback) {
   int rc = need_scene(c);
   if (rc) return rc;
+  if (iter_per_contact < 0 || polish_per_contact < 0 || !(tol == tol))
+    return fail(c, CBZ_ERR_ARG, "sur_resolve: bad arguments");
   if (!c->have_contacts) return fail(c, CBZ_ERR_STATE, "sur_resolve: no contacts (run a narrowphase first)");
   if (!c->sur.have_model) return fail(c, CBZ_ERR_STATE, "sur_resolve: no model");
   HIPCHK(c, hipSetDevice(c->device));
   if ((rc = sur_ensure_inference(c, 32))) return rc;
-  rc = c->precision == CBZ_F64 ? sur_resolve<double>(c, dt, iter_per_contact, tol, c->sur.nfall)
-                               : sur_resolve<float>(c, dt, iter_per_contact, tol, c->sur.nfall);
+  rc = c->precision == CBZ_F64
+           ? sur_resolve<double>(c, dt, iter_per_contact, tol, polish_per_contact, flags, c->sur.nfall)
+           : sur_resolve<float>(c, dt, iter_per_contact, tol, polish_per_contact, flags, c->sur.nfall);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (n_fallback) {
@@ -1707,11 +1805,12 @@ int cbz_sur_world_mask(cbz_ctx* c, uint8_t* mask) {
   return CBZ_OK;
 }
 
-int cbz_step_sur(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contact, double tol, int64_t* counts,
-                 int64_t* fallbacks) {
+int cbz_step_sur(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contact, double tol,
+                 int64_t polish_per_contact, int32_t flags, int64_t* counts, int64_t* fallbacks) {
   int rc = need_scene(c);
   if (rc) return rc;
-  if (n_steps < 0 || iter_per_contact < 0) return fail(c, CBZ_ERR_ARG, "step_sur: bad arguments");
+  if (n_steps < 0 || iter_per_contact < 0 || polish_per_contact < 0 || !(tol == tol))
+    return fail(c, CBZ_ERR_ARG, "step_sur: bad arguments");
   if (!c->sur.have_model) return fail(c, CBZ_ERR_STATE, "step_sur: no model");
   HIPCHK(c, hipSetDevice(c->device));
   if ((rc = sur_ensure_inference(c, 32))) return rc;
@@ -1731,8 +1830,8 @@ int cbz_step_sur(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contac
     if (hist)
       HIPCHK(c, hipMemcpyAsync(hist + (size_t)s * c->nw, c->world_cnt, c->nw * sizeof(int32_t),
                                hipMemcpyDeviceToDevice, c->stream));
-    rc = f64 ? sur_resolve<double>(c, dt, iter_per_contact, tol, fb + s)
-             : sur_resolve<float>(c, dt, iter_per_contact, tol, fb + s);
+    rc = f64 ? sur_resolve<double>(c, dt, iter_per_contact, tol, polish_per_contact, flags, fb + s)
+             : sur_resolve<float>(c, dt, iter_per_contact, tol, polish_per_contact, flags, fb + s);
   }
   if (rc == CBZ_OK) rc = check_overflow(c);
   if (rc == CBZ_OK && n_steps > 0) {

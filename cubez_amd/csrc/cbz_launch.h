@@ -12,7 +12,14 @@ namespace cbz {
 // Run together they are ResolveContacts; run apart, the velocity phase can
 // be given to a subset of worlds.  RM_LABEL records, per contact, the sum of
 // the velocity and rotation changes applyVelocityChange gave its two bodies.
-enum : int32_t { RM_POS = 1, RM_VEL = 2, RM_LABEL = 4 };
+// RM_POLISH (with RM_VEL): worlds whose mask byte is 0 first take the
+// per-body velocity changes in `delta`, then run the velocity loop until
+// every contact is within velocityEpsilon + polish_tol, at most
+// polish_mult * C iterations; a world that does not get there is restored
+// (velocities, rotations, motion, awake flags) and re-solved by the plain
+// velocity phase, and its mask byte set to 1.  Worlds whose mask byte is 1
+// run the plain velocity phase.
+enum : int32_t { RM_POS = 1, RM_VEL = 2, RM_LABEL = 4, RM_POLISH = 8 };
 
 template <typename R>
 struct NarrowLaunch {
@@ -28,6 +35,8 @@ struct NarrowLaunch {
   int64_t* world_coff;
   int64_t* total;
   int32_t* overflow;
+  uint32_t* hits;     // [cap] hit pairs as tile * 64 + lane (all-pairs)
+  int32_t* n_hits;
   int32_t n_worlds;
 };
 
@@ -46,9 +55,13 @@ struct ResolveLaunch {
   int64_t iter_mult;
   int64_t iter_fixed;
   int32_t mode = 3;                      // RM_POS | RM_VEL: the whole of ResolveContacts
-  const uint8_t* world_mask = nullptr;   // velocity phase only where set
+  uint8_t* world_mask = nullptr;         // velocity phase only where set (RM_POLISH: in/out)
   R* pos0 = nullptr;                     // [3][n_bodies] positions at prepare time
   float* labels = nullptr;               // [cap][12] velocity-phase changes per contact
+  const float* delta = nullptr;          // RM_POLISH: [6][n_bodies] velocity, rotation changes
+  R* vel0 = nullptr;                     // RM_POLISH: [7][n_bodies] snapshot (vel, rot, motion)
+  R polish_tol = R(0);
+  int64_t polish_mult = 0;
 };
 
 template <typename R>
@@ -61,6 +74,7 @@ template <typename R>
 hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st);
 
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);
 
 // Device self-test of the FP64 operations whose rounding the parity claim
 // depends on: op 0 = a/b, 1 = sqrt(a), 2 = a*b, 3 = a+b, 4 = a-b.

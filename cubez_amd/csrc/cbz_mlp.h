@@ -13,8 +13,8 @@ constexpr int kHidden = 256;   // hidden width
 constexpr int kOut = 12;       // outputs: dv0, dw0, dv1, dw1
 constexpr int kOutPad = 32;    // padded to the MFMA tile (32)
 constexpr int kLayers = 4;
-constexpr int kSurBodies = 256;     // bodies per world the apply kernel takes
-constexpr int kSurContacts = 1024;  // contacts per world the apply kernel takes
+constexpr int kSurBodies = 256;     // bodies per world the surrogate takes
+constexpr int kSurContacts = 1024;  // contacts per world the surrogate takes
 
 // layer l maps kIn[l] -> kOutL[l] features (unpadded)
 __host__ __device__ constexpr int layer_in(int l) { return l == 0 ? kFeat : kHidden; }
@@ -52,9 +52,8 @@ struct SurApply {
   const int32_t* body_off;
   const int32_t* overflow;
   const float* Y;               // [contacts][kOut] predicted changes
-  uint8_t* mask;                // out: 1 = the world needs the exact velocity phase
-  R dt;
-  float tol;
+  float* delta;                 // out: [6][n_bodies] summed change per body
+  uint8_t* mask;                // out: 1 = plain velocity phase, 0 = polish
   int max_contacts;
 };
 
@@ -63,12 +62,13 @@ template <typename R>
 hipError_t launch_features(const Bodies<R>& b, const Contacts<R>& c, int64_t n, int32_t* deg, const float* mean,
                            const float* inv_std, __bf16* X, float* raw, hipStream_t st);
 template <typename R>
-hipError_t launch_sur_apply(const SurApply<R>& a, int n_worlds, hipStream_t st);
+hipError_t launch_sur_scatter(const SurApply<R>& a, int n_worlds, hipStream_t st);
 hipError_t launch_gather_batch(const float* Xraw, const float* Yraw, const int64_t* idx, int rows, int valid_rows,
                                const float* in_mean, const float* in_inv_std, const float* out_mean,
                                const float* out_inv_std, __bf16* X, __bf16* Xt, float* T, hipStream_t st);
 hipError_t launch_silu_bwd(const float* dA, const float* Z, int rows, int valid_rows, int cols, int ld, __bf16* dZ,
                            __bf16* dZt, int ldt, hipStream_t st);
+hipError_t launch_sample_idx(int64_t* idx, int n, uint64_t seed, uint64_t step, int64_t rows, hipStream_t st);
 hipError_t launch_col_stats(const float* A, int64_t rows, int cols, float* mean, float* stdv, hipStream_t st);
 hipError_t launch_count_u8(const uint8_t* m, int n, int32_t* out, hipStream_t st);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t st);

@@ -77,11 +77,22 @@ struct Contacts {
 // Pair-space description for the narrowphase: the ordered pairs of world w
 // are split into 64-wide tiles; tile t belongs to world tile_world[t] and
 // covers that world's pair slots [(t - world_tile0[w])*64, +64).
+// All-pairs space: row r of world w (body b0+r) has np + n columns (planes,
+// then bodies j, j == r skipped), cut into 64-column tiles, so tile
+// (r, c) = world_tile0[w] + r * nch + c with nch = ceil((np + n) / 64).
+// The count pass runs one wave per task = (column tile c, block of
+// kRowBlock rows).  An explicit pair list uses 64 consecutive list entries
+// per tile instead.
+constexpr int kRowBlock = 16;
+
 struct PairSpace {
   int32_t n_tiles;
   const int32_t* tile_world;
   const int32_t* world_tile0;
-  const int32_t* world_q;     // pair slots of each world
+  const int32_t* world_q;     // pair slots of each world (explicit list)
+  int32_t n_tasks;            // all-pairs count tasks
+  const int32_t* task_world;
+  const int32_t* world_task0;
   const int32_t* body_off;
   const int32_t* plane_off;
   // explicit pair list (null for all-pairs): pairs of world w are

@@ -12,8 +12,8 @@
 //   k_features    the 40 input features of every contact, normalised, BF16
 //   k_gemm        C = A * B (+bias, SiLU) on MFMA 32x32x16, FP32 accumulate
 //   k_silu_bwd, k_mse, k_colsum, k_adam   backward pass and optimiser
-//   k_sur_apply   per world: apply the predictions, check the residual
-//                 closing velocities, or mark the world for the exact method
+//   k_sur_scatter per world: sum the predicted changes per body (the
+//                 resolver then polishes or re-solves the world)
 #include <climits>
 
 #include "cbz_devmath.h"
@@ -80,6 +80,18 @@ __global__ void k_features(Bodies<R> b, Contacts<R> c, int64_t n, const int32_t*
     for (int i = 0; i < kFeat; ++i) x[i] = __bf16((f[i] - mean[i]) * inv_std[i]);
     for (int i = kFeat; i < kFeatPad; ++i) x[i] = __bf16(0.f);
   }
+}
+
+// Batch sampling on the device: row i of training step `step` is
+// SplitMix64(seed, step * n + i) mod rows (uniform with replacement).
+__global__ void k_sample_idx(int64_t* idx, int n, uint64_t seed, uint64_t step, int64_t rows) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = seed * 0xD1B54A32D192ED03ull + (step * uint64_t(n) + uint64_t(i) + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  idx[i] = int64_t(z % uint64_t(rows));
 }
 
 // raw features / labels of a sampled batch -> normalised BF16 inputs and
@@ -266,93 +278,61 @@ __global__ void k_pack_weights(const float* W, int N, int K, __bf16* Wnk, int kp
 }
 
 // ---------------------------------------------------------------------------
-// apply + check
+// per-body sums of the predictions
 // ---------------------------------------------------------------------------
-// One wave per world.  The predicted per-contact changes are summed per body
-// in contact order (deterministic); every contact's closing velocity is then
-// updated as the resolver's update loop would (contact.go:416-442) and its
-// desired velocity change recomputed.  If the largest remaining desired
-// change is within velocityEpsilon + tol the world keeps the result, else it
-// is flagged for the exact velocity phase and left untouched.
+// One workgroup per world.  Each contact's predicted changes (dv0, dw0 for
+// its first body, dv1, dw1 for its second) are added to its bodies in 32.32
+// fixed point with LDS integer atomics: integer addition is associative, so
+// the sums do not depend on the order the atomics land in and a replay gives
+// the same bits.  Sleeping bodies get no change (the exact loop wakes them
+// when a contact needs it).  A world with a non-finite or implausibly large
+// prediction, or too large for the surrogate, is marked for the plain
+// velocity phase (mask 1); the others are marked for polishing (mask 0).
+constexpr float kMaxPredicted = 1.0e4f;  // |change| in m/s or rad/s
+constexpr double kFix = 4294967296.0;     // 2^32
+
 template <typename R>
-__global__ void __launch_bounds__(64) k_sur_apply(SurApply<R> a) {
-  __shared__ float s_d[kSurBodies][6];
-  __shared__ int32_t s_b[kSurContacts][2];
-  const int w = blockIdx.x, lane = threadIdx.x;
+__global__ void __launch_bounds__(256) k_sur_scatter(SurApply<R> a) {
+  __shared__ unsigned long long acc[6][kSurBodies];
+  __shared__ int bad;
+  const int w = blockIdx.x, t = threadIdx.x;
   const int C = a.world_cnt[w];
   const int b0w = a.body_off[w], n = a.body_off[w + 1] - b0w;
-  if (a.overflow[0] != 0 || C <= 0 || !(a.dt > R(0))) {
-    if (lane == 0) a.mask[w] = 0;
+  if (a.overflow[0] != 0 || C <= 0) {  // nothing to resolve (the resolver's guard)
+    if (t == 0) a.mask[w] = 0;
     return;
   }
-  if (n > kSurBodies || C > a.max_contacts || C > kSurContacts) {
-    if (lane == 0) a.mask[w] = 1;
+  if (n > kSurBodies || C > a.max_contacts) {
+    if (t == 0) a.mask[w] = 1;
     return;
   }
+  for (int i = t; i < 6 * kSurBodies; i += 256) acc[i / kSurBodies][i % kSurBodies] = 0ull;
+  if (t == 0) bad = 0;
+  __syncthreads();
   const int64_t base = a.world_coff[w];
-  const Bodies<R>& b = a.b;
-  const Contacts<R>& c = a.c;
-  const int64_t S = b.stride, CS = c.cap;
-  for (int j = lane; j < C; j += 64) {
-    s_b[j][0] = c.body0[base + j];
-    s_b[j][1] = c.body1[base + j];
-  }
-  __syncthreads();
-  // body sums, contact order: lane i owns bodies i, i+64, ...
-  for (int i = lane; i < n; i += 64) {
-    float d[6] = {0, 0, 0, 0, 0, 0};
-    const int gb = b0w + i;
-    for (int j = 0; j < C; ++j) {
-      const float* y = a.Y + (base + j) * kOut;
-      if (s_b[j][0] == gb)
-        for (int k = 0; k < 6; ++k) d[k] += y[k];
-      if (s_b[j][1] == gb)
-        for (int k = 0; k < 6; ++k) d[k] += y[6 + k];
-    }
-    for (int k = 0; k < 6; ++k) s_d[i][k] = d[k];
-  }
-  __syncthreads();
-  // residual desired velocity change of every contact
-  float worst = -INFINITY;
-  bool bad = false;
-  for (int j = lane; j < C; j += 64) {
+  const uint8_t* fl = a.b.flags;
+  for (int j = t; j < C; j += 256) {
     const int64_t g = base + j;
-    const int jb[2] = {c.body0[g], c.body1[g]};
-    const V3<R> nrm = ld3c(c.normal, CS, g);
-    R m[9];
-    for (int k = 0; k < 9; ++k) m[k] = c.c2w[k * CS + g];
-    V3<R> cv = ld3c(c.cvel, CS, g);
+    const int gb[2] = {a.c.body0[g], a.c.body1[g]};
+    const float* y = a.Y + g * kOut;
+    bool ok = true;
+    for (int k = 0; k < kOut; ++k) ok &= fabsf(y[k]) <= kMaxPredicted;  // false for NaN
+    if (!ok) {
+      bad = 1;
+      continue;
+    }
     for (int bb = 0; bb < 2; ++bb) {
-      if (jb[bb] < 0) continue;
-      const int l = jb[bb] - b0w;
-      const V3<R> dv{R(s_d[l][0]), R(s_d[l][1]), R(s_d[l][2])};
-      const V3<R> dw{R(s_d[l][3]), R(s_d[l][4]), R(s_d[l][5])};
-      const V3<R> rel = ld3c(c.rel + 3 * bb * CS, CS, g);
-      const V3<R> d = add(cross(dw, rel), dv);
-      cv = add(cv, mul(m3tv(m, d), bb == 1 ? R(-1) : R(1)));
-    }
-    // calculateDesiredDeltaVelocity (contact.go:87-112)
-    R vfa = R(0);
-    if (b.flags[jb[0]] & F_AWAKE) vfa += dot(mul(ld3c(b.last_acc, S, jb[0]), a.dt), nrm);
-    if (jb[1] >= 0 && (b.flags[jb[1]] & F_AWAKE)) vfa -= dot(mul(ld3c(b.last_acc, S, jb[1]), a.dt), nrm);
-    R rest = R(kRestitution);
-    if (dabs(cv.x) < R(0.25)) rest = R(0);
-    const R ddv = -cv.x - rest * (cv.x - vfa);
-    worst = fmaxf(worst, float(ddv));
-    bad |= !(ddv == ddv);
-  }
-  worst = dpp::allreduce(worst, -INFINITY, [](float x, float y) { return y > x ? y : x; });
-  const bool any_bad = __ballot(bad) != 0;
-  const bool accept = !any_bad && worst <= float(kVelocityEpsilon) + a.tol;
-  if (lane == 0) a.mask[w] = accept ? 0 : 1;
-  if (!accept) return;
-  for (int i = lane; i < n; i += 64) {
-    const int gb = b0w + i;
-    for (int k = 0; k < 3; ++k) {
-      b.vel[k * S + gb] += R(s_d[i][k]);
-      b.rot[k * S + gb] += R(s_d[i][3 + k]);
+      if (gb[bb] < 0 || !(fl[gb[bb]] & F_AWAKE)) continue;
+      const int l = gb[bb] - b0w;
+      for (int k = 0; k < 6; ++k)
+        atomicAdd(&acc[k][l], (unsigned long long)__double2ll_rn(double(y[6 * bb + k]) * kFix));
     }
   }
+  __syncthreads();
+  const int64_t S = a.b.stride;
+  for (int i = t; i < n; i += 256)
+    for (int k = 0; k < 6; ++k) a.delta[k * S + b0w + i] = float(double((long long)acc[k][i]) / kFix);
+  if (t == 0) a.mask[w] = bad ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -383,9 +363,9 @@ hipError_t launch_features(const Bodies<R>& b, const Contacts<R>& c, int64_t n, 
 }
 
 template <typename R>
-hipError_t launch_sur_apply(const SurApply<R>& a, int n_worlds, hipStream_t st) {
+hipError_t launch_sur_scatter(const SurApply<R>& a, int n_worlds, hipStream_t st) {
   if (n_worlds <= 0) return hipSuccess;
-  k_sur_apply<R><<<n_worlds, 64, 0, st>>>(a);
+  k_sur_scatter<R><<<n_worlds, 256, 0, st>>>(a);
   return hipGetLastError();
 }
 
@@ -394,6 +374,11 @@ hipError_t launch_gather_batch(const float* Xraw, const float* Yraw, const int64
                                const float* out_inv_std, __bf16* X, __bf16* Xt, float* T, hipStream_t st) {
   k_gather_batch<<<(rows + 255) / 256, 256, 0, st>>>(Xraw, Yraw, idx, rows, valid_rows, in_mean, in_inv_std,
                                                    out_mean, out_inv_std, X, Xt, T);
+  return hipGetLastError();
+}
+hipError_t launch_sample_idx(int64_t* idx, int n, uint64_t seed, uint64_t step, int64_t rows, hipStream_t st) {
+  if (n <= 0 || rows <= 0) return hipSuccess;
+  k_sample_idx<<<(n + 255) / 256, 256, 0, st>>>(idx, n, seed, step, rows);
   return hipGetLastError();
 }
 hipError_t launch_col_stats(const float* A, int64_t rows, int cols, float* mean, float* stdv, hipStream_t st) {
@@ -443,7 +428,7 @@ template hipError_t launch_features<double>(const Bodies<double>&, const Contact
                                             const float*, const float*, __bf16*, float*, hipStream_t);
 template hipError_t launch_features<float>(const Bodies<float>&, const Contacts<float>&, int64_t, int32_t*,
                                            const float*, const float*, __bf16*, float*, hipStream_t);
-template hipError_t launch_sur_apply<double>(const SurApply<double>&, int, hipStream_t);
-template hipError_t launch_sur_apply<float>(const SurApply<float>&, int, hipStream_t);
+template hipError_t launch_sur_scatter<double>(const SurApply<double>&, int, hipStream_t);
+template hipError_t launch_sur_scatter<float>(const SurApply<float>&, int, hipStream_t);
 
 }  // namespace cbz

@@ -4,20 +4,25 @@
 // Emission order (examples/cubedrop.go:47-64): for each body i of a world,
 // first its half-space checks (planes in order, up to 8 cube vertices each
 // in the `mults` order of colliders.go:319-327), then CheckForCollisions(i, j)
-// for every j != i in ascending order.  Pair slot q of world w is
-// (row r, column col) = (q / (np+n), q % (np+n)); columns < np are planes.
-// An explicit pair list (any caller loop) uses its own order instead.
+// for every j != i in ascending order.  Row r of a world has np + n columns
+// (planes, then bodies; column j == r is never a pair), cut into 64-column
+// tiles; tiles are ordered row by row, so scanning their contact totals in
+// tile order gives every contact its slot in the reference's order.  An
+// explicit pair list (any caller loop) uses its own order instead, 64
+// consecutive entries per tile.
 //
-// Four launches:
-//   count  — one lane per pair slot; 64-slot tiles record a hit bitmask and
-//            the tile's contact total.  Cube-cube and cube-sphere pairs whose
-//            bounding spheres are far apart are rejected before the SAT: the
-//            bound guarantees that the reference's own test reports no
-//            contact (see far_apart below), so the result is unchanged.
+// Launches (all-pairs):
+//   count_rows  — one wave per (column tile, 16 rows): per tile a hit
+//                 bitmask, the contact total, and the hit pairs appended to
+//                 a list.  Cube-cube and cube-sphere pairs whose bounding
+//                 spheres are far apart are rejected before the SAT: the
+//                 bound guarantees that the reference's own test reports no
+//                 contact (see far_apart below), so the result is unchanged.
 //   scan_tiles  — one wave per world: exclusive scan of tile totals.
 //   scan_worlds — one block: exclusive scan of world totals.
-//   emit   — one wave per tile with hits: recompute the hit pairs and write
-//            each contact at world_offset + tile_offset + lane prefix.
+//   emit_hits   — one thread per hit pair: recompute it and write its
+//                 contacts at world offset + tile offset + in-tile prefix.
+// (The explicit-list form counts and emits one wave per tile.)
 // All geometry is evaluated in FP64 whatever the storage type R.
 #include "cbz_devmath.h"
 #include "cbz_launch.h"
@@ -59,6 +64,8 @@ struct NP {
   int64_t* world_coff;
   int64_t* total;
   int32_t* overflow;
+  uint32_t* hits;
+  int32_t* n_hits;
   int32_t n_worlds;
 };
 
@@ -350,27 +357,121 @@ __device__ __forceinline__ int pair_eval(const NP<R>& a, int one, int two, int64
   return cube_cube<R, EMIT>(a, c1, one, c2, two, slot);
 }
 
-// Pair slot q of world w -> (one, two).  Returns false for the j == i
-// diagonal of the all-pairs space (no check is made there).
-__device__ __forceinline__ bool pair_of(const PairSpace& ps, int w, int q, int& one, int& two) {
-  if (ps.list_one) {
-    const int p = ps.list_off[w] + q;
-    one = ps.list_one[p];
-    two = ps.list_two[p];
-    return true;
-  }
-  const int b0 = ps.body_off[w], n = ps.body_off[w + 1] - b0;
-  const int p0 = ps.plane_off[w], np = ps.plane_off[w + 1] - p0;
-  const int width = np + n;
-  const int r = q / width, col = q - r * width;
-  one = b0 + r;
-  if (col < np) {
-    two = -(p0 + col) - 1;
-    return true;
-  }
+// Entry q of world w's explicit pair list -> (one, two).
+__device__ __forceinline__ void pair_of(const PairSpace& ps, int w, int q, int& one, int& two) {
+  const int p = ps.list_off[w] + q;
+  one = ps.list_one[p];
+  two = ps.list_two[p];
+}
+
+// ---------------------------------------------------------------------------
+// all-pairs space
+// ---------------------------------------------------------------------------
+// Count: one wave per task (world w, column tile c, rows r0 .. r0+15).  Lane
+// l owns column 64c + l for the whole task: a plane, or body j whose centre,
+// bounding radius and shape stay in registers; the row body is the same for
+// every lane (uniform loads).  Per row the wave records the tile's hit mask
+// and contact total and appends its hit pairs to the hit list (one atomic
+// per wave and row; the list's order is irrelevant, emission slots come from
+// the scans).
+template <typename R>
+__global__ void __launch_bounds__(256) k_np_count_rows(NP<R> a) {
+  const int lane = threadIdx.x & 63;
+  const int task = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (task >= a.ps.n_tasks) return;
+  const int w = a.ps.task_world[task];
+  const int b0 = a.ps.body_off[w], n = a.ps.body_off[w + 1] - b0;
+  const int p0 = a.ps.plane_off[w], np = a.ps.plane_off[w + 1] - p0;
+  const int nch = (np + n + 63) >> 6;
+  const int k = task - a.ps.world_task0[w];
+  const int c = k % nch;
+  const int r0 = (k / nch) * kRowBlock, r1 = min(n, r0 + kRowBlock);
+  const int col = 64 * c + lane;
+  const bool is_plane = col < np;
   const int j = col - np;
-  two = b0 + j;
-  return j != r;
+  const bool is_body = !is_plane && j < n;
+  const int64_t S = a.b.stride;
+  double tj[3] = {0, 0, 0}, rj = 0;
+  int sj = 0;
+  if (is_body) {
+    const int g = b0 + j;
+    tj[0] = double(a.b.ctransform[9 * S + g]);
+    tj[1] = double(a.b.ctransform[10 * S + g]);
+    tj[2] = double(a.b.ctransform[11 * S + g]);
+    rj = a.brad[g];
+    sj = a.b.shape[g];
+  }
+  const int t0 = a.ps.world_tile0[w];
+  for (int r = r0; r < r1; ++r) {
+    const int i = b0 + r;
+    int cnt = 0;
+    if (is_plane) {
+      cnt = pair_eval<R, false>(a, i, -(p0 + col) - 1, 0);
+    } else if (is_body && j != r) {
+      // the same conservative test as far_apart, from registers
+      const double dx = tj[0] - double(a.b.ctransform[9 * S + i]);
+      const double dy = tj[1] - double(a.b.ctransform[10 * S + i]);
+      const double dz = tj[2] - double(a.b.ctransform[11 * S + i]);
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      const double rr = a.brad[i] + rj;
+      const int si = a.b.shape[i];
+      const bool spheres = si == SHAPE_SPHERE && sj == SHAPE_SPHERE;
+      const double kf = (si == SHAPE_CUBE && sj == SHAPE_CUBE) ? 3.003 : 1.001;
+      if (spheres || !(d2 > kf * rr * rr)) cnt = pair_eval<R, false>(a, i, b0 + j, 0);
+    }
+    const uint64_t m = __ballot(cnt > 0);
+    const int tot = wave_sum_i32(cnt);
+    const int t = t0 + r * nch + c;
+    if (lane == 0) {
+      a.tile_cnt[t] = tot;
+      a.tile_mask[t] = m;
+    }
+    if (m) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(a.n_hits, __popcll(m));
+      base = __shfl(base, 0, 64);
+      if (cnt > 0) {
+        const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (idx < a.c.cap) a.hits[idx] = uint32_t(t) * 64u + uint32_t(lane);
+      }
+    }
+  }
+}
+
+// Emit: one thread per hit pair.  Its first contact goes to world offset +
+// tile offset + the contacts of the tile's earlier hit pairs: one for a body
+// pair (a body pair that hit holds exactly one contact), a recount for a
+// plane column (up to 8 cube vertices).
+template <typename R>
+__global__ void __launch_bounds__(256) k_np_emit_hits(NP<R> a) {
+  if (a.overflow[0] != 0) return;
+  const int H = min(a.n_hits[0], int(min(a.c.cap, int64_t(INT32_MAX))));
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < H; e += gridDim.x * blockDim.x) {
+    const uint32_t code = a.hits[e];
+    const int t = int(code >> 6), lane = int(code & 63u);
+    const int w = a.ps.tile_world[t];
+    const int b0 = a.ps.body_off[w], n = a.ps.body_off[w + 1] - b0;
+    const int p0 = a.ps.plane_off[w], np = a.ps.plane_off[w + 1] - p0;
+    const int nch = (np + n + 63) >> 6;
+    const int k = t - a.ps.world_tile0[w];
+    const int r = k / nch, c = k - r * nch;
+    const int i = b0 + r;
+    const uint64_t below = a.tile_mask[t] & ((1ull << lane) - 1ull);
+    // lanes of this tile that are plane columns
+    const int np_here = min(max(np - 64 * c, 0), 64);
+    const uint64_t plane_lanes = np_here >= 64 ? ~0ull : ((1ull << np_here) - 1ull);
+    int64_t prefix = __popcll(below & ~plane_lanes);
+    for (uint64_t pb = below & plane_lanes; pb; pb &= pb - 1ull) {
+      const int l = __ffsll((unsigned long long)pb) - 1;
+      prefix += pair_eval<R, false>(a, i, -(p0 + 64 * c + l) - 1, 0);
+    }
+    const int64_t slot = a.world_coff[w] + a.tile_off[t] + prefix;
+    const int col = 64 * c + lane;
+    if (col < np)
+      pair_eval<R, true>(a, i, -(p0 + col) - 1, slot);
+    else
+      pair_eval<R, true>(a, i, b0 + (col - np), slot);
+  }
 }
 
 template <typename R>
@@ -383,7 +484,8 @@ __global__ void __launch_bounds__(256) k_np_count(NP<R> a) {
   int cnt = 0;
   if (q < a.ps.world_q[w]) {
     int one, two;
-    if (pair_of(a.ps, w, q, one, two)) cnt = pair_eval<R, false>(a, one, two, 0);
+    pair_of(a.ps, w, q, one, two);
+    cnt = pair_eval<R, false>(a, one, two, 0);
   }
   const uint64_t m = __ballot(cnt > 0);
   const int tot = wave_sum_i32(cnt);
@@ -476,18 +578,35 @@ hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st, hipEvent
   a.world_coff = L.world_coff;
   a.total = L.total;
   a.overflow = L.overflow;
+  a.hits = L.hits;
+  a.n_hits = L.n_hits;
   a.n_worlds = L.n_worlds;
   if (a.n_worlds <= 0) return hipSuccess;
+  const bool list = a.ps.list_one != nullptr;
+  if (!list && (!a.hits || !a.n_hits)) return hipErrorInvalidValue;
   // ev (optional): recorded before the count, after it, after the scans
   // and after the emit, for per-kernel timing on this stream.
   const int nb = (a.ps.n_tiles + 3) / 4;
   if (ev) (void)hipEventRecord(ev[0], st);
-  if (nb > 0) k_np_count<R><<<nb, 256, 0, st>>>(a);
+  if (list) {
+    if (nb > 0) k_np_count<R><<<nb, 256, 0, st>>>(a);
+  } else {
+    hipError_t e = launch_fill_i32(a.n_hits, 1, 0, st);
+    if (e != hipSuccess) return e;
+    const int nbt = (a.ps.n_tasks + 3) / 4;
+    if (nbt > 0) k_np_count_rows<R><<<nbt, 256, 0, st>>>(a);
+  }
   if (ev) (void)hipEventRecord(ev[1], st);
   k_np_scan_tiles<R><<<a.n_worlds, 64, 0, st>>>(a);
   k_np_scan_worlds<R><<<1, 1024, 0, st>>>(a);
   if (ev) (void)hipEventRecord(ev[2], st);
-  if (nb > 0) k_np_emit<R><<<nb, 256, 0, st>>>(a);
+  if (list) {
+    if (nb > 0) k_np_emit<R><<<nb, 256, 0, st>>>(a);
+  } else {
+    const int64_t want = (a.c.cap + 255) / 256;
+    const int nbe = int(want < 8192 ? (want > 0 ? want : 1) : 8192);
+    k_np_emit_hits<R><<<nbe, 256, 0, st>>>(a);
+  }
   if (ev) (void)hipEventRecord(ev[3], st);
   return hipGetLastError();
 }

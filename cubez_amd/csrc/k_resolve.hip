@@ -39,9 +39,13 @@ struct RS {
   int64_t iter_mult;
   int64_t iter_fixed;
   int32_t mode;                // RM_* bits
-  const uint8_t* world_mask;   // velocity phase only where set (null: every world)
+  uint8_t* world_mask;         // velocity phase only where set (null: every world); RM_POLISH: in/out
   R* pos0;                     // [3][n_bodies] body positions at prepare time
   float* labels;               // [cap][12] accumulated velocity-phase changes (null: off)
+  const float* delta;          // RM_POLISH: [6][n_bodies] velocity and rotation changes
+  R* vel0;                     // RM_POLISH: [7][n_bodies] vel, rot, motion before them
+  R polish_tol;
+  int64_t polish_mult;
 };
 
 
@@ -374,7 +378,9 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
   const int64_t maxIter = a.iter_fixed > 0 ? a.iter_fixed : a.iter_mult * int64_t(C);
 
   const bool do_pos = (a.mode & RM_POS) != 0;
-  const bool do_vel = (a.mode & RM_VEL) != 0 && (!a.world_mask || a.world_mask[w]);
+  const bool do_vel = (a.mode & RM_VEL) != 0 && ((a.mode & RM_POLISH) || !a.world_mask || a.world_mask[w]);
+  // worlds this path takes are never polished: plain velocity phase
+  if ((a.mode & RM_POLISH) && lane == 0) a.world_mask[w] = 1;
   // prepareContacts, contact.go:225-229
   for (int j = lane; j < C; j += 64) {
     if (do_pos) prepare_contact(a, base + j);
@@ -619,7 +625,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
   uint8_t* s_flags = (uint8_t*)(lds + L::flags);
   uint32_t& s_bcid = *(uint32_t*)(lds + L::bcid);
   const bool do_pos = (a.mode & RM_POS) != 0;
-  const bool do_vel = (a.mode & RM_VEL) != 0 && (!a.world_mask || a.world_mask[w]);
+  const bool do_vel = (a.mode & RM_VEL) != 0 && ((a.mode & RM_POLISH) || !a.world_mask || a.world_mask[w]);
   if (a.world_iters && lane == 0) {
     if (do_pos) a.world_iters[2 * w] = 0;
     if (a.mode & RM_VEL) a.world_iters[2 * w + 1] = 0;
@@ -815,96 +821,180 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
 #pragma unroll
   for (int s = 0; s < SLOTS; ++s) {
     const int j = 64 * s + lane;
-    cvl[s] = V3<R>{R(0), R(0), R(0)};
-    if (j < C) {
-      if (do_pos) c.pen[base + j] = key[s];
-      key[s] = c.ddv[base + j];
-      cvl[s] = ld3(c.cvel, CS, base + j);
-    }
+    if (do_pos && j < C) c.pen[base + j] = key[s];
   }
-  used = 0;
-  while (do_vel && used < maxIter) {
-    R best = R(kVelocityEpsilon);
-    int bi = INT_MAX;
+  // selection keys and contact velocities as prepare stored them
+  auto load_vel_keys = [&]() {
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
-      if (64 * s + lane < C && key[s] > best) {
-        best = key[s];
-        bi = 64 * s + lane;
+      const int j = 64 * s + lane;
+      cvl[s] = V3<R>{R(0), R(0), R(0)};
+      if (j < C) {
+        key[s] = c.ddv[base + j];
+        cvl[s] = ld3(c.cvel, CS, base + j);
       }
     }
-    wave_argmax_dpp(best, bi);
-    if (bi == INT_MAX) break;
-    if (lane == (bi & 63)) {
+  };
+  // The reference's loop with threshold eps and `budget` iterations; true
+  // when it stopped because every contact was within eps.
+  auto vel_loop = [&](R eps, int64_t budget, int64_t& it) -> bool {
+    while (true) {
+      R best = eps;
+      int bi = INT_MAX;
 #pragma unroll
       for (int s = 0; s < SLOTS; ++s) {
-        if (s == (bi >> 6)) {
-          const V3<R> q = get_pt(s);
-          s_bc[0] = nrm[s].x; s_bc[1] = nrm[s].y; s_bc[2] = nrm[s].z;
-          s_bc[3] = q.x;      s_bc[4] = q.y;      s_bc[5] = q.z;
-          s_bc[6] = cvl[s].x; s_bc[7] = cvl[s].y; s_bc[8] = cvl[s].z;
-          s_bcid = ids[s];
+        if (64 * s + lane < C && key[s] > best) {
+          best = key[s];
+          bi = 64 * s + lane;
         }
       }
-    }
-    __syncthreads();
-    const V3<R> n{s_bc[0], s_bc[1], s_bc[2]};
-    const V3<R> p{s_bc[3], s_bc[4], s_bc[5]};
-    const V3<R> cvk{s_bc[6], s_bc[7], s_bc[8]};
-    const uint32_t kid = s_bcid;
-    const int kl[2] = {id_lo(kid), id_hi(kid)};
-    const int kb[2] = {b0w + kl[0], kl[1] >= 0 ? b0w + kl[1] : -1};
-    V3<R> rel[2] = {sub(p, lds3(s_pos0, kl[0])), V3<R>{R(0), R(0), R(0)}};
-    if (kl[1] >= 0) rel[1] = sub(p, lds3(s_pos0, kl[1]));
-    bool aw0 = (s_flags[kl[0]] & F_AWAKE) != 0;
-    bool aw1 = kl[1] >= 0 && (s_flags[kl[1]] & F_AWAKE) != 0;
-    if (kl[1] >= 0 && (aw0 || aw1) && !(aw0 && aw1)) {
-      const int t = aw0 ? kl[1] : kl[0];
-      if (lane == 0) {
-        s_flags[t] = uint8_t(s_flags[t] | F_AWAKE);
-        b.motion[b0w + t] = R(kSleepEpsilon * 2.0);
-      }
-    }
-    __syncthreads();
-    R m[9];
-    basis_from_scale(n, s_bs[bi], m);
-    V3<R> vc[2], rc[2];
-    apply_velocity_change(a, kb, rel, m, best, cvk, vc, rc, lane);
-    if ((a.mode & RM_LABEL) && lane == 0) add_label(a, base + bi, vc, rc);
+      wave_argmax_dpp(best, bi);
+      if (bi == INT_MAX) return true;
+      if (it >= budget) return false;
+      if (lane == (bi & 63)) {
 #pragma unroll
-    for (int s = 0; s < SLOTS; ++s) {
-      if (64 * s + lane >= C) continue;
-      const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
-      bool any = false;
-#pragma unroll
-      for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-        for (int d = 0; d < 2; ++d) any |= (jb[bb] >= 0 && jb[bb] == kl[d]);
-      if (!any) continue;
-      R mj[9];
-      basis_from_scale(nrm[s], s_bs[64 * s + lane], mj);
-      V3<R> cvj = cvl[s];
-#pragma unroll
-      for (int bb = 0; bb < 2; ++bb) {
-        if (jb[bb] < 0) continue;
-        const V3<R> rj = sub(get_pt(s), lds3(s_pos0, jb[bb]));
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-          if (jb[bb] == kl[d]) {
-            V3<R> dv = add(cross(rc[d], rj), vc[d]);
-            R sign = R(1);
-            if (bb == 1) sign = R(-1);
-            cvj = add(cvj, mul(m3tv(mj, dv), sign));
+        for (int s = 0; s < SLOTS; ++s) {
+          if (s == (bi >> 6)) {
+            const V3<R> q = get_pt(s);
+            s_bc[0] = nrm[s].x; s_bc[1] = nrm[s].y; s_bc[2] = nrm[s].z;
+            s_bc[3] = q.x;      s_bc[4] = q.y;      s_bc[5] = q.z;
+            s_bc[6] = cvl[s].x; s_bc[7] = cvl[s].y; s_bc[8] = cvl[s].z;
+            s_bcid = ids[s];
           }
         }
       }
-      cvl[s] = cvj;
-      key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x);
+      __syncthreads();
+      const V3<R> n{s_bc[0], s_bc[1], s_bc[2]};
+      const V3<R> p{s_bc[3], s_bc[4], s_bc[5]};
+      const V3<R> cvk{s_bc[6], s_bc[7], s_bc[8]};
+      const uint32_t kid = s_bcid;
+      const int kl[2] = {id_lo(kid), id_hi(kid)};
+      const int kb[2] = {b0w + kl[0], kl[1] >= 0 ? b0w + kl[1] : -1};
+      V3<R> rel[2] = {sub(p, lds3(s_pos0, kl[0])), V3<R>{R(0), R(0), R(0)}};
+      if (kl[1] >= 0) rel[1] = sub(p, lds3(s_pos0, kl[1]));
+      bool aw0 = (s_flags[kl[0]] & F_AWAKE) != 0;
+      bool aw1 = kl[1] >= 0 && (s_flags[kl[1]] & F_AWAKE) != 0;
+      if (kl[1] >= 0 && (aw0 || aw1) && !(aw0 && aw1)) {
+        const int t = aw0 ? kl[1] : kl[0];
+        if (lane == 0) {
+          s_flags[t] = uint8_t(s_flags[t] | F_AWAKE);
+          b.motion[b0w + t] = R(kSleepEpsilon * 2.0);
+        }
+      }
+      __syncthreads();
+      R m[9];
+      basis_from_scale(n, s_bs[bi], m);
+      V3<R> vc[2], rc[2];
+      apply_velocity_change(a, kb, rel, m, best, cvk, vc, rc, lane);
+      if ((a.mode & RM_LABEL) && lane == 0) add_label(a, base + bi, vc, rc);
+#pragma unroll
+      for (int s = 0; s < SLOTS; ++s) {
+        if (64 * s + lane >= C) continue;
+        const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+        bool any = false;
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+          for (int d = 0; d < 2; ++d) any |= (jb[bb] >= 0 && jb[bb] == kl[d]);
+        if (!any) continue;
+        R mj[9];
+        basis_from_scale(nrm[s], s_bs[64 * s + lane], mj);
+        V3<R> cvj = cvl[s];
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+          if (jb[bb] < 0) continue;
+          const V3<R> rj = sub(get_pt(s), lds3(s_pos0, jb[bb]));
+#pragma unroll
+          for (int d = 0; d < 2; ++d) {
+            if (jb[bb] == kl[d]) {
+              V3<R> dv = add(cross(rc[d], rj), vc[d]);
+              R sign = R(1);
+              if (bb == 1) sign = R(-1);
+              cvj = add(cvj, mul(m3tv(mj, dv), sign));
+            }
+          }
+        }
+        cvl[s] = cvj;
+        key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x);
+      }
+      __syncthreads();
+      ++it;
     }
-    __syncthreads();
-    ++used;
-  }
+  };
+
+  used = 0;
   if (do_vel) {
+    load_vel_keys();
+    const bool polish = (a.mode & RM_POLISH) != 0 && a.world_mask[w] == 0;
+    bool done = false;
+    if (polish) {
+      // take the predicted changes: snapshot, then add them to the bodies
+      const float* dl = a.delta;
+      for (int i = lane; i < n; i += 64) {
+        const int g = b0w + i;
+        const V3<R> v = ld3(b.vel, S, g), r = ld3(b.rot, S, g);
+        st3(a.vel0, S, g, v);
+        st3(a.vel0 + 3 * S, S, g, r);
+        a.vel0[6 * S + g] = b.motion[g];
+        const V3<R> dv{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
+        const V3<R> dw{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
+        if (dv.x != R(0) || dv.y != R(0) || dv.z != R(0) || dw.x != R(0) || dw.y != R(0) || dw.z != R(0)) {
+          st3(b.vel, S, g, add(v, dv));
+          st3(b.rot, S, g, add(r, dw));
+        }
+      }
+      // contact velocities after those changes: the update of
+      // contact.go:424-441 applied to each body's summed change
+#pragma unroll
+      for (int s = 0; s < SLOTS; ++s) {
+        if (64 * s + lane >= C) continue;
+        const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+        R mj[9];
+        basis_from_scale(nrm[s], s_bs[64 * s + lane], mj);
+        V3<R> cvj = cvl[s];
+        bool moved = false;
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+          if (jb[bb] < 0) continue;
+          const int g = b0w + jb[bb];
+          const V3<R> dv{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
+          const V3<R> dw{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
+          if (dv.x == R(0) && dv.y == R(0) && dv.z == R(0) && dw.x == R(0) && dw.y == R(0) && dw.z == R(0)) continue;
+          moved = true;
+          const V3<R> rj = sub(get_pt(s), lds3(s_pos0, jb[bb]));
+          R sign = R(1);
+          if (bb == 1) sign = R(-1);
+          cvj = add(cvj, mul(m3tv(mj, add(cross(dw, rj), dv)), sign));
+        }
+        // a contact neither of whose bodies changed keeps the key prepare
+        // gave it, as in the plain loop
+        if (moved) {
+          cvl[s] = cvj;
+          key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x);
+        }
+      }
+      __syncthreads();
+      done = vel_loop(R(kVelocityEpsilon) + a.polish_tol, a.polish_mult * int64_t(C), used);
+      if (!done) {
+        // not within tolerance in the budget: undo everything the
+        // prediction and the loop changed, then the plain velocity phase
+        for (int i = lane; i < n; i += 64) {
+          const int g = b0w + i;
+          st3(b.vel, S, g, ld3(a.vel0, S, g));
+          st3(b.rot, S, g, ld3(a.vel0 + 3 * S, S, g));
+          b.motion[g] = a.vel0[6 * S + g];
+          s_flags[i] = b.flags[g];
+        }
+        __syncthreads();
+        load_vel_keys();
+        if (lane == 0) a.world_mask[w] = 1;
+      }
+    }
+    if (!done) {
+      int64_t plain = 0;
+      vel_loop(R(kVelocityEpsilon), maxIter, plain);
+      used += plain;
+    }
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
       const int j = 64 * s + lane;
@@ -974,6 +1064,11 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   a.world_mask = L.world_mask;
   a.pos0 = L.pos0;
   a.labels = L.labels;
+  a.delta = L.delta;
+  a.vel0 = L.vel0;
+  a.polish_tol = L.polish_tol;
+  a.polish_mult = L.polish_mult;
+  if ((a.mode & RM_POLISH) && (!a.world_mask || !a.delta || !a.vel0)) return hipErrorInvalidValue;
   if (a.n_worlds <= 0) return hipSuccess;
   k_resolve<R><<<a.n_worlds, 64, 0, st>>>(a);
   return hipGetLastError();

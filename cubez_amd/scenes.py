@@ -157,6 +157,22 @@ class Scene:
                      name=self.name, dt=self.dt, steps=self.steps, iter_mult=self.iter_mult)
 
 
+def concat(parts):
+    """One Scene holding the worlds of several Scenes, in order (the first
+    one's dt, steps and iteration multiplier)."""
+    parts = list(parts)
+    bo, po = [0], [0]
+    for sc in parts:
+        bo.extend(bo[-1] + np.asarray(sc.body_off[1:], np.int64))
+        po.extend(po[-1] + np.asarray(sc.plane_off[1:], np.int64))
+    f0 = parts[0]
+    return Scene(np.asarray(bo, np.int32), np.asarray(po, np.int32),
+                 np.concatenate([sc.plane_normal for sc in parts]), np.concatenate([sc.plane_offset for sc in parts]),
+                 {k: np.concatenate([sc.fields[k] for sc in parts]) for k in f0.fields},
+                 name="+".join(sc.name for sc in parts[:3]) + ("..." if len(parts) > 3 else ""),
+                 dt=f0.dt, steps=f0.steps, iter_mult=f0.iter_mult)
+
+
 def _new_bodies(n):
     """NewRigidBody (rigidbody.go:104-114) defaults for n bodies."""
     f = {

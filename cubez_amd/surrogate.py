@@ -3,9 +3,11 @@
 The network (40 -> 256 -> 256 -> 256 -> 12, SiLU, BF16 weights on the
 matrix cores) is trained on the GPU by the library's own forward and
 backward kernels (cbz_sur_train) on contacts the exact resolver generates
-(cbz_sur_collect).  The exact resolver remains the judge at run time: every
-world whose residual closing velocity after the surrogate exceeds
-velocityEpsilon + tol is re-solved exactly.
+(cbz_sur_collect).  The exact resolver remains the judge at run time: from the
+predicted velocities its worst-first loop runs until every contact of the
+world is within velocityEpsilon + tol, for at most DEFAULT_POLISH * C
+iterations; a world that does not get there is restored and re-solved by the
+plain velocity phase.
 
     python -m cubez_amd.surrogate train --out cubez_amd/models/surrogate.npz
 """
@@ -22,6 +24,8 @@ LAYERS = ((40, 256), (256, 256), (256, 256), (256, 12))
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "surrogate.npz")
 # residual tolerance: 1 mm of extra penetration over one step of 1/60 s
 DEFAULT_TOL = 1e-3 * 60.0
+# polish budget per contact (iterations of the exact loop after the network)
+DEFAULT_POLISH = 1
 
 
 def unpack(params):
@@ -37,34 +41,47 @@ def unpack(params):
     return out
 
 
-def collect(w, seeds, n_worlds=32, steps=600, at=(200, 400, 600), iter_per_contact=4, reserve=2_600_000,
-            make=None):
-    """Run batched scenes and log the exact resolver's contacts at the given
-    steps; returns the dataset size."""
+def collect(w, seeds, n_worlds=32, at=(300, 600), iter_per_contact=4, reserve=3_200_000, make=None, log=None):
+    """Run the worlds of every seed together as one batched scene (64 seeds
+    x 32 worlds at steps 300 and 600 give about 2 M rows) and log the exact
+    resolver's contacts at the given steps.  A snapshot that would overflow
+    the reserved rows is skipped.  Returns the dataset size."""
     make = make or (lambda s: scenes.batched(s, n_worlds))
+    sc = scenes.concat([make(s) for s in seeds])
     w.sur_reserve(reserve)
-    for seed in seeds:
-        sc = make(seed)
-        w.load(sc)
-        done = 0
-        for k in sorted(at):
-            if k - 1 > done:
-                w.step(sc.dt, k - 1 - done, iter_per_contact)
-            w.integrate(sc.dt)
-            w.narrowphase()
+    w.load(sc)
+    done = 0
+    for k in sorted(at):
+        if k - 1 > done:
+            w.step(sc.dt, k - 1 - done, iter_per_contact)
+        w.integrate(sc.dt)
+        w.narrowphase()
+        rows = w.sur_dataset(download=False)
+        if int(w.contact_counts().sum()) + rows <= reserve:
             w.sur_collect(sc.dt, iter_per_contact)
-            done = k
+        done = k
+        if log:
+            log(f"collect: step {k} of {sc.n_worlds} worlds, {w.sur_dataset(download=False)} rows")
     return w.sur_dataset(download=False)
 
 
-def train(w, steps=3000, batch=4096, lr=1e-3, seed=0, init=True):
+def train(w, steps=3000, batch=4096, lr=1e-3, seed=0, init=True, log=None):
+    """Adam on the dataset: a short warm-up at 0.3 lr, the main rate, then
+    the last quarter at 0.3 and 0.1 lr (run in chunks of at most 1000 steps)."""
     if init:
         w.sur_init(seed)
-    losses = []
-    # a short warm-up at a lower rate, then the main rate
-    for s, r in ((min(200, steps), lr * 0.3), (max(steps - 200, 0), lr)):
-        if s:
-            losses.append(w.sur_train(s, batch, r, seed + len(losses)))
+    warm = min(200, steps // 10)
+    tail = (steps - warm) // 4
+    plan = ((warm, 0.3), (steps - warm - 2 * tail, 1.0), (tail, 0.3), (tail, 0.1))
+    losses, chunk = [], 0
+    for s, f in plan:
+        while s > 0:
+            n = min(s, 1000)
+            losses.append(w.sur_train(n, batch, lr * f, seed + 7919 * chunk))
+            chunk += 1
+            s -= n
+            if log:
+                log(f"train: {sum(len(x) for x in losses)} steps, lr {lr * f:.2e}, loss {losses[-1][-50:].mean():.4f}")
     return np.concatenate(losses) if losses else np.zeros(0)
 
 
@@ -98,16 +115,19 @@ def main():
     a = ap.parse_args()
     if a.cmd == "train":
         with World(scenes.batched(0, 2)) as w:
+            def log(m):
+                print(m, flush=True)
+
             t0 = time.perf_counter()
-            n = collect(w, range(a.seeds), n_worlds=a.worlds)
+            n = collect(w, range(a.seeds), n_worlds=a.worlds, log=log)
             t1 = time.perf_counter()
-            losses = train(w, a.steps, a.batch, a.lr)
+            losses = train(w, a.steps, a.batch, a.lr, log=log)
             t2 = time.perf_counter()
             meta = {"rows": n, "seeds": a.seeds, "worlds_per_seed": a.worlds, "steps": a.steps,
                     "batch": a.batch, "lr": a.lr, "loss_first": float(losses[:50].mean()),
                     "loss_last": float(losses[-50:].mean()), "collect_s": t1 - t0, "train_s": t2 - t1}
             save(a.out, w.sur_get_params(), w.sur_get_norm(), meta)
-            print(meta)
+            print(meta, flush=True)
 
 
 if __name__ == "__main__":

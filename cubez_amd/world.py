@@ -240,21 +240,40 @@ class World:
         self._chk(self._lib.cbz_sur_predict(self._h, len(X), _ptr(X), _ptr(out)))
         return out
 
-    def sur_resolve(self, dt, iter_per_contact=4, tol=0.06):
+    SUR_ZERO_PREDICTION = 1
+
+    def sur_resolve(self, dt, iter_per_contact=4, tol=0.06, polish=1, zero_prediction=False):
+        """ResolveContacts with the surrogate: exact positions, predicted
+        velocity changes polished by the worst-first loop to
+        velocityEpsilon + tol within polish * C iterations, else the plain
+        velocity phase.  Returns the number of worlds that fell back."""
         n = ctypes.c_int64(0)
-        self._chk(self._lib.cbz_sur_resolve(self._h, float(dt), int(iter_per_contact), float(tol), ctypes.byref(n)))
+        flags = self.SUR_ZERO_PREDICTION if zero_prediction else 0
+        self._chk(self._lib.cbz_sur_resolve(self._h, float(dt), int(iter_per_contact), float(tol), int(polish), flags,
+                                            ctypes.byref(n)))
         return n.value
+
+    def sur_phase_times(self):
+        """Accumulated ms [position phase, features, network, per-body sums,
+        velocity phase] and counts [resolves, contacts, fallbacks, velocity
+        iterations] of profiled surrogate resolves since the last call."""
+        ms = np.zeros(5)
+        cnt = np.zeros(4, np.int64)
+        self._chk(self._lib.cbz_sur_phase_times(self._h, _ptr(ms), _ptr(cnt)))
+        return ms, cnt
 
     def sur_world_mask(self):
         m = np.empty(self.n_worlds, np.uint8)
         self._chk(self._lib.cbz_sur_world_mask(self._h, _ptr(m)))
         return m
 
-    def step_sur(self, dt, n_steps=1, iter_per_contact=4, tol=0.06, counts=False):
+    def step_sur(self, dt, n_steps=1, iter_per_contact=4, tol=0.06, polish=1, zero_prediction=False,
+                 counts=False):
         """n_steps with the surrogate resolver; returns the per-step fallback
         counts (and [n_steps, n_worlds] contact counts with counts=True)."""
         buf = np.empty((n_steps, self.n_worlds), np.int64) if counts else None
         fb = np.empty(max(n_steps, 1), np.int64)
+        flags = self.SUR_ZERO_PREDICTION if zero_prediction else 0
         self._chk(self._lib.cbz_step_sur(self._h, float(dt), int(n_steps), int(iter_per_contact), float(tol),
-                                         _ptr(buf), _ptr(fb)))
+                                         int(polish), flags, _ptr(buf), _ptr(fb)))
         return (fb[:n_steps], buf) if counts else fb[:n_steps]

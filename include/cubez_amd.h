@@ -224,15 +224,32 @@ int cbz_sur_train(cbz_ctx *ctx, int32_t steps, int32_t batch, float lr, uint64_t
 int cbz_sur_grad(cbz_ctx *ctx, int32_t rows, const float *features, const float *labels, float *grad, float *loss);
 /* predictions (denormalised) for given rows of raw features */
 int cbz_sur_predict(cbz_ctx *ctx, int32_t rows, const float *features, float *out);
-/* ResolveContacts with the surrogate and per-world fallback (tol in m/s; a
- * negative tol sends every world to the exact method).  *n_fallback
- * receives the number of worlds that fell back; cbz_sur_world_mask which. */
-int cbz_sur_resolve(cbz_ctx *ctx, double dt, int64_t iter_per_contact, double tol, int64_t *n_fallback);
+/* ResolveContacts with the surrogate.  The position phase is the exact
+ * one.  The network's per-contact velocity changes are summed per body and
+ * applied; then, per world, the reference's worst-first velocity loop runs
+ * from that state until every contact's desired velocity change is within
+ * velocityEpsilon + tol (tol in m/s), for at most polish_per_contact * C
+ * iterations.  A world that does not get there is restored and re-solved by
+ * the plain velocity phase (a fallback).  tol < 0 sends every world to the
+ * plain phase; flags CBZ_SUR_ZERO_PREDICTION polishes from a zero prediction
+ * (the same method without the network).  *n_fallback receives the number
+ * of worlds that fell back, cbz_sur_world_mask which.
+ * Replaces ResolveContacts (contact.go:208-222) for callers that accept the
+ * tolerance. */
+#define CBZ_SUR_ZERO_PREDICTION 1
+int cbz_sur_resolve(cbz_ctx *ctx, double dt, int64_t iter_per_contact, double tol, int64_t polish_per_contact,
+                    int32_t flags, int64_t *n_fallback);
 int cbz_sur_world_mask(cbz_ctx *ctx, uint8_t *mask);
+/* With profiling on (cbz_set_profiling), every surrogate resolve records HIP
+ * events on the context's stream; this returns the accumulated ms
+ * [exact position phase, features, network forward, per-body sums, velocity
+ * phase (polish and fallbacks)] and counts [resolves, contacts, fallbacks,
+ * velocity-phase iterations], and resets them. */
+int cbz_sur_phase_times(cbz_ctx *ctx, double *ms5, int64_t *counts4);
 /* cbz_step with the surrogate resolver; fallbacks[n_steps] (may be NULL)
  * receives the fallback count of every step */
-int cbz_step_sur(cbz_ctx *ctx, double dt, int32_t n_steps, int64_t iter_per_contact, double tol, int64_t *counts,
-                 int64_t *fallbacks);
+int cbz_step_sur(cbz_ctx *ctx, double dt, int32_t n_steps, int64_t iter_per_contact, double tol,
+                 int64_t polish_per_contact, int32_t flags, int64_t *counts, int64_t *fallbacks);
 
 #ifdef __cplusplus
 }
