@@ -127,7 +127,7 @@ def load():
         "cbz_sur_init": ([P, ctypes.c_uint64], i32),
         "cbz_sur_train": ([P, i32, i32, ctypes.c_float, ctypes.c_uint64, P], i32),
         "cbz_sur_grad": ([P, i32, P, P, P, P], i32),
-        "cbz_sur_predict": ([P, i32, P, P], i32),
+        "cbz_sur_predict": ([P, i32, P, P, i32], i32),
         "cbz_sur_resolve": ([P, f64, i64, f64, i64, i32, P], i32),
         "cbz_sur_world_mask": ([P, P], i32),
         "cbz_step_sur": ([P, f64, i32, i64, f64, i64, i32, P, P], i32),

@@ -68,8 +68,11 @@ struct cbz_context {
   int32_t* d_world_tile0 = nullptr;
   int32_t* d_task_world = nullptr;
   int32_t* d_world_task0 = nullptr;
-  uint32_t* hits = nullptr;  // [cap] hit pairs of the last count pass
+  uint32_t* hits = nullptr;  // [cap] hit pairs of the last narrowphase
   int32_t* n_hits = nullptr;
+  uint32_t* cand = nullptr;  // [cand_cap] candidate pairs of the last cull
+  int32_t* n_cand = nullptr;
+  int64_t cand_cap = 0;
   // explicit-list pair space (grown on demand)
   int32_t l_tiles_cap = 0, l_pairs_cap = 0;
   int32_t* l_tile_world = nullptr;
@@ -281,6 +284,8 @@ void free_scene(cbz_ctx* c) {
   dfree(c->d_world_task0);
   dfree(c->hits);
   dfree(c->n_hits);
+  dfree(c->cand);
+  dfree(c->n_cand);
   dfree(c->l_tile_world);
   dfree(c->l_world_tile0);
   dfree(c->l_world_q);
@@ -464,6 +469,9 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles) {
   L.overflow = c->overflow;
   L.hits = c->hits;
   L.n_hits = c->n_hits;
+  L.cand = c->cand;
+  L.n_cand = c->n_cand;
+  L.cand_cap = c->cand_cap;
   L.n_worlds = c->nw;
   HIPCHK(c, launch_fill_i32(c->overflow, 1, 0, c->stream));
   HIPCHK(c, launch_narrowphase<R>(L, c->stream, c->prof ? &c->ev[1] : nullptr));
@@ -652,6 +660,25 @@ int sur_forward(cbz_ctx* c, const __bf16* X, int64_t M, int64_t Mpad, bool train
   return CBZ_OK;
 }
 
+// Inference forward: all four layers in one fused launch (k_mlp_forward),
+// rows of X [M][kFeatPad] -> denormalised predictions in u.Yp.
+int sur_infer(cbz_ctx* c, const __bf16* X, int64_t M) {
+  auto& u = c->sur;
+  const LayerOff o = layer_offsets();
+  MlpArgs f{};
+  f.M = (int)M;
+  f.X = X;
+  for (int l = 0; l < kLayers; ++l) {
+    f.W[l] = u.Wnk[l];
+    f.bias[l] = u.params + o.b[l];
+  }
+  f.y_scale = u.norm + 2 * kFeat + kOut;  // out_std
+  f.y_shift = u.norm + 2 * kFeat;         // out_mean
+  f.Y = u.Yp;
+  HIPCHK(c, launch_mlp_forward(f, c->stream));
+  return CBZ_OK;
+}
+
 // Backward pass of the mean-squared error against bT; gradients into grad.
 int sur_backward(cbz_ctx* c, int64_t M, int64_t Mpad) {
   auto& u = c->sur;
@@ -804,13 +831,10 @@ int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int64_t polish_
   HIPCHK(c, hipStreamSynchronize(c->stream));
   tot = std::min(tot, c->cap);
   if ((rc = sur_ensure_inference(c, tot))) return rc;
-  const int64_t Mpad = round32(std::max<int64_t>(tot, 1));
-  // rows tot..Mpad-1 of X are padding: GEMM rows are independent and those
-  // outputs are never read
   HIPCHK(c, launch_features<R>(bodies_view<R>(c), contacts_view<R>(c), tot, u.deg, u.norm, u.norm + kFeat, u.X,
                                nullptr, c->stream));
   if (c->prof) HIPCHK(c, hipEventRecord(ev[2], c->stream));
-  if ((rc = sur_forward(c, u.X, tot, Mpad, false))) return rc;
+  if ((rc = sur_infer(c, u.X, tot))) return rc;
   if (c->prof) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   SurApply<R> a;
   a.b = bodies_view<R>(c);
@@ -976,6 +1000,10 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc_bytes(c, &c->c_real, rs * 26 * (size_t)c->cap))) return rc;
   if ((rc = dalloc(c, &c->hits, (size_t)c->cap))) return rc;
   if ((rc = dalloc(c, &c->n_hits, 1))) return rc;
+  // candidate pairs of the cull: 4 per contact slot (64 per body by default)
+  c->cand_cap = std::min<int64_t>(4 * c->cap, INT32_MAX);
+  if ((rc = dalloc(c, &c->cand, (size_t)c->cand_cap))) return rc;
+  if ((rc = dalloc(c, &c->n_cand, 1))) return rc;
   if ((rc = dalloc(c, &c->world_cnt, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->world_coff, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->total, 1))) return rc;
@@ -1758,7 +1786,7 @@ int cbz_sur_grad(cbz_ctx* c, int32_t rows, const float* X, const float* Y, float
   return CBZ_OK;
 }
 
-int cbz_sur_predict(cbz_ctx* c, int32_t rows, const float* X, float* Y) {
+int cbz_sur_predict(cbz_ctx* c, int32_t rows, const float* X, float* Y, int32_t flags) {
   if (!c || rows <= 0 || !X || !Y) return fail(c, CBZ_ERR_ARG, "sur_predict: bad arguments");
   if (!c->sur.have_model) return fail(c, CBZ_ERR_STATE, "sur_predict: no model");
   HIPCHK(c, hipSetDevice(c->device));
@@ -1767,7 +1795,11 @@ int cbz_sur_predict(cbz_ctx* c, int32_t rows, const float* X, float* Y) {
   auto& u = c->sur;
   if ((rc = sur_ensure_inference(c, rows))) return rc;
   const int64_t Mpad = round32(rows);
-  if ((rc = sur_forward(c, u.bX, rows, Mpad, false))) return rc;
+  if (flags & CBZ_SUR_PREDICT_LAYERED)
+    rc = sur_forward(c, u.bX, rows, Mpad, false);
+  else
+    rc = sur_infer(c, u.bX, rows);
+  if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(Y, u.Yp, (size_t)rows * kOut * sizeof(float), hipMemcpyDeviceToHost));
   return CBZ_OK;

@@ -37,6 +37,9 @@ struct NarrowLaunch {
   int32_t* overflow;
   uint32_t* hits;     // [cap] hit pairs as tile * 64 + lane (all-pairs)
   int32_t* n_hits;
+  uint32_t* cand;     // [cand_cap] candidate pairs, same coding
+  int32_t* n_cand;
+  int64_t cand_cap;
   int32_t n_worlds;
 };
 

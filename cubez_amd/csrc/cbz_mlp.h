@@ -43,6 +43,17 @@ struct GemmArgs {
   int m_valid, n_valid;         // rows / columns that hold data
 };
 
+// fused inference forward: all four layers per block of rows
+struct MlpArgs {
+  int M;                          // rows
+  const __bf16* X;                // [M][kFeatPad] normalised features
+  const __bf16* W[kLayers];       // per layer [Npad][Kpad] row-major (the k_gemm Bt operand)
+  const float* bias[kLayers];
+  const float* y_scale;           // output = y * scale + shift (denormalisation)
+  const float* y_shift;
+  float* Y;                       // [M][kOut]
+};
+
 template <typename R>
 struct SurApply {
   Bodies<R> b;
@@ -58,6 +69,7 @@ struct SurApply {
 };
 
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t st);
+hipError_t launch_mlp_forward(const MlpArgs& f, hipStream_t st);
 template <typename R>
 hipError_t launch_features(const Bodies<R>& b, const Contacts<R>& c, int64_t n, int32_t* deg, const float* mean,
                            const float* inv_std, __bf16* X, float* raw, hipStream_t st);

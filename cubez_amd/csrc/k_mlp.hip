@@ -278,6 +278,117 @@ __global__ void k_pack_weights(const float* W, int N, int K, __bf16* Wnk, int kp
 }
 
 // ---------------------------------------------------------------------------
+// fused forward (inference)
+// ---------------------------------------------------------------------------
+// One workgroup (4 waves) per block of kFuseRows rows runs all four layers:
+// the block's activations stay in LDS (one 64 x 256 BF16 tile, updated in
+// place), so HBM sees only the BF16 features in and the 12 outputs out; the
+// weights stream from the L2.  Each wave computes a 64-row x 64-column slab
+// of a hidden layer as 2 x 2 MFMA 32x32x16 tiles (four independent
+// accumulators).  The k order of every dot product is that of k_gemm, so the
+// results are the same bits.  LDS element (row, k) sits at
+// row * 256 + (((k / 8) ^ (row % 32)) * 8) + k % 8: the 16-byte fragments
+// that the 32 rows of a tile read at one k land in different banks.
+constexpr int kFuseRows = 64;
+
+__device__ __forceinline__ int fswz(int row, int k) { return row * kHidden + ((((k >> 3) ^ (row & 31))) << 3) + (k & 7); }
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_mlp_forward(MlpArgs f) {
+  __shared__ __align__(16) __bf16 H[kFuseRows * kHidden];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 31, h = lane >> 5;
+  const int64_t m0 = int64_t(blockIdx.x) * kFuseRows;
+  f32x16 acc[2][2];
+  // hidden-layer epilogue: bias, SiLU, BF16 into the LDS tile
+  auto store_hidden = [&](const float* bias) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int col = wave * 64 + 32 * u + r;
+      const float bv = bias[col];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int row = 32 * t + 8 * (j >> 2) + 4 * h + (j & 3);
+          H[fswz(row, col)] = __bf16(silu(acc[t][u][j] + bv));
+        }
+    }
+  };
+  // layer 0: A from the features in HBM
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) acc[t][u] = f32x16{};
+#pragma unroll
+  for (int k = 0; k < kFeatPad; k += 16) {
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int64_t row = m0 + 32 * t + r;
+      a[t] = row < f.M ? *(const bf16x8*)(f.X + row * kFeatPad + k + 8 * h) : bf16x8{};
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) b[u] = *(const bf16x8*)(f.W[0] + (wave * 64 + 32 * u + r) * kFeatPad + k + 8 * h);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+  }
+  store_hidden(f.bias[0]);
+  __syncthreads();
+  // layers 1 and 2: A from the LDS tile, results back into it
+  for (int l = 1; l < kLayers - 1; ++l) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) acc[t][u] = f32x16{};
+    const __bf16* W = f.W[l];
+#pragma unroll 4
+    for (int k = 0; k < kHidden; k += 16) {
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) a[t] = *(const bf16x8*)(H + fswz(32 * t + r, k + 8 * h));
+#pragma unroll
+      for (int u = 0; u < 2; ++u) b[u] = *(const bf16x8*)(W + (wave * 64 + 32 * u + r) * kHidden + k + 8 * h);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+    }
+    __syncthreads();  // every wave has read the tile
+    store_hidden(f.bias[l]);
+    __syncthreads();
+  }
+  // layer 3: 12 outputs (one 32-column tile); waves 0 and 1 take a row tile each
+  if (wave < 2) {
+    f32x16 y = {};
+    const __bf16* W = f.W[kLayers - 1];
+#pragma unroll 4
+    for (int k = 0; k < kHidden; k += 16) {
+      const bf16x8 a = *(const bf16x8*)(H + fswz(32 * wave + r, k + 8 * h));
+      const bf16x8 b = *(const bf16x8*)(W + r * kHidden + k + 8 * h);
+      y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, y, 0, 0, 0);
+    }
+    const int col = r;
+    if (col < kOut) {
+      const float bv = f.bias[kLayers - 1][col], sc = f.y_scale[col], sh = f.y_shift[col];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int64_t row = m0 + 32 * wave + 8 * (j >> 2) + 4 * h + (j & 3);
+        if (row < f.M) f.Y[row * kOut + col] = (y[j] + bv) * sc + sh;
+      }
+    }
+  }
+}
+
+hipError_t launch_mlp_forward(const MlpArgs& f, hipStream_t st) {
+  if (f.M <= 0) return hipSuccess;
+  const int64_t blocks = (int64_t(f.M) + kFuseRows - 1) / kFuseRows;
+  k_mlp_forward<<<int(blocks), 256, 0, st>>>(f);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // per-body sums of the predictions
 // ---------------------------------------------------------------------------
 // One workgroup per world.  Each contact's predicted changes (dv0, dw0 for

@@ -12,12 +12,14 @@
 // consecutive entries per tile.
 //
 // Launches (all-pairs):
-//   count_rows  — one wave per (column tile, 16 rows): per tile a hit
-//                 bitmask, the contact total, and the hit pairs appended to
-//                 a list.  Cube-cube and cube-sphere pairs whose bounding
-//                 spheres are far apart are rejected before the SAT: the
-//                 bound guarantees that the reference's own test reports no
-//                 contact (see far_apart below), so the result is unchanged.
+//   cull        — one wave per (column tile, 16 rows): clears the tiles and
+//                 lists the candidate pairs.  Cube-cube and cube-sphere
+//                 pairs whose bounding spheres are far apart are rejected:
+//                 the bound guarantees that the reference's own test reports
+//                 no contact (see far_apart below), so the result is
+//                 unchanged.
+//   eval        — one thread per candidate: the reference's test; per tile
+//                 a hit bitmask and contact total, and the hit list.
 //   scan_tiles  — one wave per world: exclusive scan of tile totals.
 //   scan_worlds — one block: exclusive scan of world totals.
 //   emit_hits   — one thread per hit pair: recompute it and write its
@@ -66,6 +68,9 @@ struct NP {
   int32_t* overflow;
   uint32_t* hits;
   int32_t* n_hits;
+  uint32_t* cand;
+  int32_t* n_cand;
+  int64_t cand_cap;
   int32_t n_worlds;
 };
 
@@ -367,74 +372,145 @@ __device__ __forceinline__ void pair_of(const PairSpace& ps, int w, int q, int& 
 // ---------------------------------------------------------------------------
 // all-pairs space
 // ---------------------------------------------------------------------------
-// Count: one wave per task (world w, column tile c, rows r0 .. r0+15).  Lane
+// Block-wide append to a global list: every thread of the block calls it
+// with its entry count; returns the list index of the thread's first entry.
+// One atomic per block (a single shared counter hit by every wave of the
+// grid would serialise in the L2).
+__device__ __forceinline__ int64_t block_reserve(int mine, int32_t* counter, int* s_wtot, int* s_base) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int incl = wave_incl_scan_i32(mine);
+  if (lane == 63) s_wtot[wv] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int u = 0; u < nw; ++u) tot += s_wtot[u];
+    *s_base = tot > 0 ? atomicAdd(counter, tot) : 0;
+  }
+  __syncthreads();
+  int64_t base = *s_base;
+  for (int u = 0; u < wv; ++u) base += s_wtot[u];
+  __syncthreads();  // s_wtot / s_base are reused by the next call
+  return base + incl - mine;
+}
+
+// Cull: one wave per task (world w, column tile c, rows r0 .. r0+15).  Lane
 // l owns column 64c + l for the whole task: a plane, or body j whose centre,
-// bounding radius and shape stay in registers; the row body is the same for
-// every lane (uniform loads).  Per row the wave records the tile's hit mask
-// and contact total and appends its hit pairs to the hit list (one atomic
-// per wave and row; the list's order is irrelevant, emission slots come from
-// the scans).
+// bounding radius and shape stay in registers; the 16 row bodies are loaded
+// once (lane k holds row r0 + k) and broadcast.  Each tile is cleared here;
+// its plane pairs and the body pairs that pass the conservative bound (see
+// far_apart) go to the candidate list.
 template <typename R>
-__global__ void __launch_bounds__(256) k_np_count_rows(NP<R> a) {
+__global__ void __launch_bounds__(256) k_np_cull(NP<R> a) {
+  __shared__ int s_wtot[4];
+  __shared__ int s_base;
   const int lane = threadIdx.x & 63;
   const int task = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (task >= a.ps.n_tasks) return;
-  const int w = a.ps.task_world[task];
-  const int b0 = a.ps.body_off[w], n = a.ps.body_off[w + 1] - b0;
-  const int p0 = a.ps.plane_off[w], np = a.ps.plane_off[w + 1] - p0;
-  const int nch = (np + n + 63) >> 6;
-  const int k = task - a.ps.world_task0[w];
-  const int c = k % nch;
-  const int r0 = (k / nch) * kRowBlock, r1 = min(n, r0 + kRowBlock);
-  const int col = 64 * c + lane;
-  const bool is_plane = col < np;
-  const int j = col - np;
-  const bool is_body = !is_plane && j < n;
-  const int64_t S = a.b.stride;
-  double tj[3] = {0, 0, 0}, rj = 0;
-  int sj = 0;
-  if (is_body) {
-    const int g = b0 + j;
-    tj[0] = double(a.b.ctransform[9 * S + g]);
-    tj[1] = double(a.b.ctransform[10 * S + g]);
-    tj[2] = double(a.b.ctransform[11 * S + g]);
-    rj = a.brad[g];
-    sj = a.b.shape[g];
-  }
-  const int t0 = a.ps.world_tile0[w];
-  for (int r = r0; r < r1; ++r) {
-    const int i = b0 + r;
-    int cnt = 0;
-    if (is_plane) {
-      cnt = pair_eval<R, false>(a, i, -(p0 + col) - 1, 0);
-    } else if (is_body && j != r) {
-      // the same conservative test as far_apart, from registers
-      const double dx = tj[0] - double(a.b.ctransform[9 * S + i]);
-      const double dy = tj[1] - double(a.b.ctransform[10 * S + i]);
-      const double dz = tj[2] - double(a.b.ctransform[11 * S + i]);
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      const double rr = a.brad[i] + rj;
-      const int si = a.b.shape[i];
-      const bool spheres = si == SHAPE_SPHERE && sj == SHAPE_SPHERE;
-      const double kf = (si == SHAPE_CUBE && sj == SHAPE_CUBE) ? 3.003 : 1.001;
-      if (spheres || !(d2 > kf * rr * rr)) cnt = pair_eval<R, false>(a, i, b0 + j, 0);
+  const bool valid = task < a.ps.n_tasks;  // the other waves still join the block's reservation
+  uint32_t cm = 0;                         // bit k: row r0 + k is a candidate for this lane
+  int t0 = 0, r0 = 0, nch = 1, c = 0;
+  if (valid) {
+    const int w = a.ps.task_world[task];
+    const int b0 = a.ps.body_off[w], n = a.ps.body_off[w + 1] - b0;
+    const int p0 = a.ps.plane_off[w], np = a.ps.plane_off[w + 1] - p0;
+    nch = (np + n + 63) >> 6;
+    const int k = task - a.ps.world_task0[w];
+    c = k % nch;
+    r0 = (k / nch) * kRowBlock;
+    const int r1 = min(n, r0 + kRowBlock);
+    const int col = 64 * c + lane;
+    const bool is_plane = col < np;
+    const int j = col - np;
+    const bool is_body = !is_plane && j < n;
+    const int64_t S = a.b.stride;
+    double tj[3] = {0, 0, 0}, rj = 0;
+    int sj = 0;
+    if (is_body) {
+      const int g = b0 + j;
+      tj[0] = double(a.b.ctransform[9 * S + g]);
+      tj[1] = double(a.b.ctransform[10 * S + g]);
+      tj[2] = double(a.b.ctransform[11 * S + g]);
+      rj = a.brad[g];
+      sj = a.b.shape[g];
     }
-    const uint64_t m = __ballot(cnt > 0);
-    const int tot = wave_sum_i32(cnt);
-    const int t = t0 + r * nch + c;
-    if (lane == 0) {
-      a.tile_cnt[t] = tot;
-      a.tile_mask[t] = m;
+    // row bodies r0 .. r1-1: lane k < 16 loads row r0 + k
+    double ti[3] = {0, 0, 0}, ri = 0;
+    int si = 0;
+    if (lane < r1 - r0) {
+      const int g = b0 + r0 + lane;
+      ti[0] = double(a.b.ctransform[9 * S + g]);
+      ti[1] = double(a.b.ctransform[10 * S + g]);
+      ti[2] = double(a.b.ctransform[11 * S + g]);
+      ri = a.brad[g];
+      si = a.b.shape[g];
     }
-    if (m) {
-      int base = 0;
-      if (lane == 0) base = atomicAdd(a.n_hits, __popcll(m));
-      base = __shfl(base, 0, 64);
-      if (cnt > 0) {
-        const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
-        if (idx < a.c.cap) a.hits[idx] = uint32_t(t) * 64u + uint32_t(lane);
+    t0 = a.ps.world_tile0[w];
+    for (int r = r0; r < r1; ++r) {
+      const int src = r - r0;
+      const double x = __shfl(ti[0], src, 64), y = __shfl(ti[1], src, 64), z = __shfl(ti[2], src, 64);
+      const double rr = __shfl(ri, src, 64) + rj;
+      const int s_i = __shfl(si, src, 64);
+      bool cand = is_plane;
+      if (is_body && j != r) {
+        const double dx = tj[0] - x, dy = tj[1] - y, dz = tj[2] - z;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        const bool spheres = s_i == SHAPE_SPHERE && sj == SHAPE_SPHERE;
+        const double kf = (s_i == SHAPE_CUBE && sj == SHAPE_CUBE) ? 3.003 : 1.001;
+        cand = spheres || !(d2 > kf * rr * rr);
+      }
+      if (cand) cm |= 1u << src;
+      if (lane == 0) {
+        const int t = t0 + r * nch + c;
+        a.tile_cnt[t] = 0;
+        a.tile_mask[t] = 0ull;
       }
     }
+  }
+  int64_t pos = block_reserve(__popc(cm), a.n_cand, s_wtot, &s_base);
+  for (uint32_t m = cm; m; m &= m - 1u, ++pos) {
+    const int r = r0 + __ffs(m) - 1;
+    if (pos < a.cand_cap) a.cand[pos] = uint32_t(t0 + r * nch + c) * 64u + uint32_t(lane);
+  }
+}
+
+// Eval: one thread per candidate pair: the reference's test (contact count
+// only); hits add to their tile's total and mask (integer atomics: the sums
+// do not depend on the order) and join the hit list.  Too many candidates
+// for the list raise the overflow flags (the step reports a capacity error).
+template <typename R>
+__global__ void __launch_bounds__(256) k_np_eval(NP<R> a) {
+  __shared__ int s_wtot[4];
+  __shared__ int s_base;
+  const int64_t nc = a.n_cand[0];
+  if (nc > a.cand_cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.overflow[0] = 1;
+      a.overflow[1] = 1;
+    }
+    return;  // uniform over the grid
+  }
+  for (int64_t e0 = int64_t(blockIdx.x) * blockDim.x; e0 < nc; e0 += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t e = e0 + threadIdx.x;
+    int cnt = 0;
+    uint32_t code = 0;
+    if (e < nc) {
+      code = a.cand[e];
+      const int t = int(code >> 6), l = int(code & 63u);
+      const int w = a.ps.tile_world[t];
+      const int b0 = a.ps.body_off[w], n = a.ps.body_off[w + 1] - b0;
+      const int p0 = a.ps.plane_off[w], np = a.ps.plane_off[w + 1] - p0;
+      const int nch = (np + n + 63) >> 6;
+      const int k = t - a.ps.world_tile0[w];
+      const int r = k / nch, c = k - r * nch;
+      const int col = 64 * c + l;
+      const int i = b0 + r;
+      cnt = col < np ? pair_eval<R, false>(a, i, -(p0 + col) - 1, 0) : pair_eval<R, false>(a, i, b0 + (col - np), 0);
+      if (cnt > 0) {
+        atomicAdd(&a.tile_cnt[t], cnt);
+        atomicOr((unsigned long long*)&a.tile_mask[t], 1ull << l);
+      }
+    }
+    const int64_t idx = block_reserve(cnt > 0 ? 1 : 0, a.n_hits, s_wtot, &s_base);
+    if (cnt > 0 && idx < a.c.cap) a.hits[idx] = code;
   }
 }
 
@@ -580,10 +656,13 @@ hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st, hipEvent
   a.overflow = L.overflow;
   a.hits = L.hits;
   a.n_hits = L.n_hits;
+  a.cand = L.cand;
+  a.n_cand = L.n_cand;
+  a.cand_cap = L.cand_cap;
   a.n_worlds = L.n_worlds;
   if (a.n_worlds <= 0) return hipSuccess;
   const bool list = a.ps.list_one != nullptr;
-  if (!list && (!a.hits || !a.n_hits)) return hipErrorInvalidValue;
+  if (!list && (!a.hits || !a.n_hits || !a.cand || !a.n_cand)) return hipErrorInvalidValue;
   // ev (optional): recorded before the count, after it, after the scans
   // and after the emit, for per-kernel timing on this stream.
   const int nb = (a.ps.n_tiles + 3) / 4;
@@ -592,9 +671,12 @@ hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st, hipEvent
     if (nb > 0) k_np_count<R><<<nb, 256, 0, st>>>(a);
   } else {
     hipError_t e = launch_fill_i32(a.n_hits, 1, 0, st);
+    if (e == hipSuccess) e = launch_fill_i32(a.n_cand, 1, 0, st);
     if (e != hipSuccess) return e;
     const int nbt = (a.ps.n_tasks + 3) / 4;
-    if (nbt > 0) k_np_count_rows<R><<<nbt, 256, 0, st>>>(a);
+    if (nbt > 0) k_np_cull<R><<<nbt, 256, 0, st>>>(a);
+    const int64_t want = (a.cand_cap + 255) / 256;
+    k_np_eval<R><<<int(want < 8192 ? (want > 0 ? want : 1) : 8192), 256, 0, st>>>(a);
   }
   if (ev) (void)hipEventRecord(ev[1], st);
   k_np_scan_tiles<R><<<a.n_worlds, 64, 0, st>>>(a);

@@ -24,6 +24,13 @@ namespace cbz {
 
 constexpr int kKeyCap = 2048;
 
+// Compile-time variants of the resolver kernel, so that the plain
+// resolver's hot loops carry no code (and no register pressure) of the
+// other two: KIND_PLAIN (ResolveContacts, or one of its phases),
+// KIND_LABEL (also records every impulse's body changes per contact),
+// KIND_POLISH (the surrogate's polish with the plain fallback).
+enum : int { KIND_PLAIN = 0, KIND_LABEL = 1, KIND_POLISH = 2 };
+
 template <typename R>
 struct RS {
   Bodies<R> b;
@@ -369,7 +376,7 @@ __device__ void apply_velocity_change(const RS<R>& a, const int kb[2], const V3<
   }
 }
 
-template <typename R, bool LDS>
+template <typename R, bool LDS, int KIND>
 __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key, int32_t* kb0, int32_t* kb1,
                               int lane) {
   const Bodies<R>& b = a.b;
@@ -378,9 +385,9 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
   const int64_t maxIter = a.iter_fixed > 0 ? a.iter_fixed : a.iter_mult * int64_t(C);
 
   const bool do_pos = (a.mode & RM_POS) != 0;
-  const bool do_vel = (a.mode & RM_VEL) != 0 && ((a.mode & RM_POLISH) || !a.world_mask || a.world_mask[w]);
+  const bool do_vel = (a.mode & RM_VEL) != 0 && (KIND == KIND_POLISH || !a.world_mask || a.world_mask[w]);
   // worlds this path takes are never polished: plain velocity phase
-  if ((a.mode & RM_POLISH) && lane == 0) a.world_mask[w] = 1;
+  if (KIND == KIND_POLISH && lane == 0) a.world_mask[w] = 1;
   // prepareContacts, contact.go:225-229
   for (int j = lane; j < C; j += 64) {
     if (do_pos) prepare_contact(a, base + j);
@@ -478,7 +485,8 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
     match_awake(a, kb[0], kb[1], aw[0], aw[1], lane);
     V3<R> vc[2], rc[2];
     apply_velocity_change(a, kb, rel, m, best, cv, vc, rc, lane);
-    if ((a.mode & RM_LABEL) && lane == 0) add_label(a, g, vc, rc);
+    if constexpr (KIND == KIND_LABEL)
+      if (lane == 0) add_label(a, g, vc, rc);
     for (int j = lane; j < C; j += 64) {
       const int jb[2] = {kb0[j], kb1[j]};
       bool any = false;
@@ -529,7 +537,7 @@ __device__ __forceinline__ bool fits_registers(int C, int n) { return C <= 64 * 
 
 // HBM-resident path (worlds too large for the register kernel): selection
 // keys and body ids in LDS when the world has at most kKeyCap contacts.
-template <typename R>
+template <typename R, int KIND>
 __device__ void resolve_hbm(const RS<R>& a, int w, int C, char* lds, int lane) {
   if (a.world_iters && lane == 0) {
     if (a.mode & RM_POS) a.world_iters[2 * w] = 0;
@@ -542,9 +550,9 @@ __device__ void resolve_hbm(const RS<R>& a, int w, int C, char* lds, int lane) {
     R* s_key = (R*)lds;
     int32_t* s_b0 = (int32_t*)(lds + kKeyCap * sizeof(R));
     int32_t* s_b1 = s_b0 + kKeyCap;
-    resolve_world<R, true>(a, w, C, base, s_key, s_b0, s_b1, lane);
+    resolve_world<R, true, KIND>(a, w, C, base, s_key, s_b0, s_b1, lane);
   } else {
-    resolve_world<R, false>(a, w, C, base, a.c.pen + base, a.c.body0 + base, a.c.body1 + base, lane);
+    resolve_world<R, false, KIND>(a, w, C, base, a.c.pen + base, a.c.body0 + base, a.c.body1 + base, lane);
   }
 }
 
@@ -600,7 +608,7 @@ __device__ __forceinline__ int id_hi(uint32_t id) { return (id >> 16) == kNoBody
 template <typename R, int SLOTS>
 struct RegLds {
   // the widest variants keep contact points in LDS (in VGPRs they would spill)
-  static constexpr bool PT_LDS = SLOTS > 12;
+  static constexpr bool PT_LDS = SLOTS >= 12;
   static constexpr int CAP = 64 * SLOTS;
   static constexpr size_t pos0 = 0;                                   // R[3][kRegBodies]
   static constexpr size_t lad = pos0 + 3 * kRegBodies * sizeof(R);    // R[3][kRegBodies]
@@ -612,7 +620,7 @@ struct RegLds {
   static constexpr size_t bytes = bcid + 16;
 };
 
-template <typename R, int SLOTS>
+template <typename R, int SLOTS, int KIND>
 __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* lds, int lane) {
   using L = RegLds<R, SLOTS>;
   constexpr bool PT_LDS = L::PT_LDS;
@@ -625,7 +633,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
   uint8_t* s_flags = (uint8_t*)(lds + L::flags);
   uint32_t& s_bcid = *(uint32_t*)(lds + L::bcid);
   const bool do_pos = (a.mode & RM_POS) != 0;
-  const bool do_vel = (a.mode & RM_VEL) != 0 && ((a.mode & RM_POLISH) || !a.world_mask || a.world_mask[w]);
+  const bool do_vel = (a.mode & RM_VEL) != 0 && (KIND == KIND_POLISH || !a.world_mask || a.world_mask[w]);
   if (a.world_iters && lane == 0) {
     if (do_pos) a.world_iters[2 * w] = 0;
     if (a.mode & RM_VEL) a.world_iters[2 * w + 1] = 0;
@@ -822,112 +830,17 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
   for (int s = 0; s < SLOTS; ++s) {
     const int j = 64 * s + lane;
     if (do_pos && j < C) c.pen[base + j] = key[s];
+    cvl[s] = V3<R>{R(0), R(0), R(0)};
+    if (do_vel && j < C) {  // selection keys and contact velocities as prepare stored them
+      key[s] = c.ddv[base + j];
+      cvl[s] = ld3(c.cvel, CS, base + j);
+    }
   }
-  // selection keys and contact velocities as prepare stored them
-  auto load_vel_keys = [&]() {
-#pragma unroll
-    for (int s = 0; s < SLOTS; ++s) {
-      const int j = 64 * s + lane;
-      cvl[s] = V3<R>{R(0), R(0), R(0)};
-      if (j < C) {
-        key[s] = c.ddv[base + j];
-        cvl[s] = ld3(c.cvel, CS, base + j);
-      }
-    }
-  };
-  // The reference's loop with threshold eps and `budget` iterations; true
-  // when it stopped because every contact was within eps.
-  auto vel_loop = [&](R eps, int64_t budget, int64_t& it) -> bool {
-    while (true) {
-      R best = eps;
-      int bi = INT_MAX;
-#pragma unroll
-      for (int s = 0; s < SLOTS; ++s) {
-        if (64 * s + lane < C && key[s] > best) {
-          best = key[s];
-          bi = 64 * s + lane;
-        }
-      }
-      wave_argmax_dpp(best, bi);
-      if (bi == INT_MAX) return true;
-      if (it >= budget) return false;
-      if (lane == (bi & 63)) {
-#pragma unroll
-        for (int s = 0; s < SLOTS; ++s) {
-          if (s == (bi >> 6)) {
-            const V3<R> q = get_pt(s);
-            s_bc[0] = nrm[s].x; s_bc[1] = nrm[s].y; s_bc[2] = nrm[s].z;
-            s_bc[3] = q.x;      s_bc[4] = q.y;      s_bc[5] = q.z;
-            s_bc[6] = cvl[s].x; s_bc[7] = cvl[s].y; s_bc[8] = cvl[s].z;
-            s_bcid = ids[s];
-          }
-        }
-      }
-      __syncthreads();
-      const V3<R> n{s_bc[0], s_bc[1], s_bc[2]};
-      const V3<R> p{s_bc[3], s_bc[4], s_bc[5]};
-      const V3<R> cvk{s_bc[6], s_bc[7], s_bc[8]};
-      const uint32_t kid = s_bcid;
-      const int kl[2] = {id_lo(kid), id_hi(kid)};
-      const int kb[2] = {b0w + kl[0], kl[1] >= 0 ? b0w + kl[1] : -1};
-      V3<R> rel[2] = {sub(p, lds3(s_pos0, kl[0])), V3<R>{R(0), R(0), R(0)}};
-      if (kl[1] >= 0) rel[1] = sub(p, lds3(s_pos0, kl[1]));
-      bool aw0 = (s_flags[kl[0]] & F_AWAKE) != 0;
-      bool aw1 = kl[1] >= 0 && (s_flags[kl[1]] & F_AWAKE) != 0;
-      if (kl[1] >= 0 && (aw0 || aw1) && !(aw0 && aw1)) {
-        const int t = aw0 ? kl[1] : kl[0];
-        if (lane == 0) {
-          s_flags[t] = uint8_t(s_flags[t] | F_AWAKE);
-          b.motion[b0w + t] = R(kSleepEpsilon * 2.0);
-        }
-      }
-      __syncthreads();
-      R m[9];
-      basis_from_scale(n, s_bs[bi], m);
-      V3<R> vc[2], rc[2];
-      apply_velocity_change(a, kb, rel, m, best, cvk, vc, rc, lane);
-      if ((a.mode & RM_LABEL) && lane == 0) add_label(a, base + bi, vc, rc);
-#pragma unroll
-      for (int s = 0; s < SLOTS; ++s) {
-        if (64 * s + lane >= C) continue;
-        const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
-        bool any = false;
-#pragma unroll
-        for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-          for (int d = 0; d < 2; ++d) any |= (jb[bb] >= 0 && jb[bb] == kl[d]);
-        if (!any) continue;
-        R mj[9];
-        basis_from_scale(nrm[s], s_bs[64 * s + lane], mj);
-        V3<R> cvj = cvl[s];
-#pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {
-          if (jb[bb] < 0) continue;
-          const V3<R> rj = sub(get_pt(s), lds3(s_pos0, jb[bb]));
-#pragma unroll
-          for (int d = 0; d < 2; ++d) {
-            if (jb[bb] == kl[d]) {
-              V3<R> dv = add(cross(rc[d], rj), vc[d]);
-              R sign = R(1);
-              if (bb == 1) sign = R(-1);
-              cvj = add(cvj, mul(m3tv(mj, dv), sign));
-            }
-          }
-        }
-        cvl[s] = cvj;
-        key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x);
-      }
-      __syncthreads();
-      ++it;
-    }
-  };
-
   used = 0;
   if (do_vel) {
-    load_vel_keys();
-    const bool polish = (a.mode & RM_POLISH) != 0 && a.world_mask[w] == 0;
-    bool done = false;
-    if (polish) {
+    bool polishing = KIND == KIND_POLISH && a.world_mask[w] == 0;
+    if constexpr (KIND == KIND_POLISH) {
+      if (polishing) {
       // take the predicted changes: snapshot, then add them to the bodies
       const float* dl = a.delta;
       for (int i = lane; i < n; i += 64) {
@@ -974,10 +887,29 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
         }
       }
       __syncthreads();
-      done = vel_loop(R(kVelocityEpsilon) + a.polish_tol, a.polish_mult * int64_t(C), used);
-      if (!done) {
-        // not within tolerance in the budget: undo everything the
-        // prediction and the loop changed, then the plain velocity phase
+      }
+    }
+    // The reference's loop.  A polished world first runs it to
+    // velocityEpsilon + polish_tol within polish_mult * C iterations; if that
+    // does not get there, everything the prediction and the loop changed is
+    // undone and the same loop runs as the plain phase.
+    R eps = polishing ? R(kVelocityEpsilon) + a.polish_tol : R(kVelocityEpsilon);
+    int64_t budget = polishing ? a.polish_mult * int64_t(C) : maxIter;
+    int64_t it = 0;
+    while (true) {
+      R best = eps;
+      int bi = INT_MAX;
+#pragma unroll
+      for (int s = 0; s < SLOTS; ++s) {
+        if (64 * s + lane < C && key[s] > best) {
+          best = key[s];
+          bi = 64 * s + lane;
+        }
+      }
+      wave_argmax_dpp(best, bi);
+      if (bi == INT_MAX) break;
+      if (it >= budget) {
+        if (KIND != KIND_POLISH || !polishing) break;
         for (int i = lane; i < n; i += 64) {
           const int g = b0w + i;
           st3(b.vel, S, g, ld3(a.vel0, S, g));
@@ -985,15 +917,92 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
           b.motion[g] = a.vel0[6 * S + g];
           s_flags[i] = b.flags[g];
         }
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) {
+          const int j = 64 * s + lane;
+          if (j < C) {
+            key[s] = c.ddv[base + j];
+            cvl[s] = ld3(c.cvel, CS, base + j);
+          }
+        }
         __syncthreads();
-        load_vel_keys();
         if (lane == 0) a.world_mask[w] = 1;
+        polishing = false;
+        eps = R(kVelocityEpsilon);
+        budget = maxIter;
+        it = 0;
+        continue;
       }
-    }
-    if (!done) {
-      int64_t plain = 0;
-      vel_loop(R(kVelocityEpsilon), maxIter, plain);
-      used += plain;
+      if (lane == (bi & 63)) {
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) {
+          if (s == (bi >> 6)) {
+            const V3<R> q = get_pt(s);
+            s_bc[0] = nrm[s].x; s_bc[1] = nrm[s].y; s_bc[2] = nrm[s].z;
+            s_bc[3] = q.x;      s_bc[4] = q.y;      s_bc[5] = q.z;
+            s_bc[6] = cvl[s].x; s_bc[7] = cvl[s].y; s_bc[8] = cvl[s].z;
+            s_bcid = ids[s];
+          }
+        }
+      }
+      __syncthreads();
+      const V3<R> n{s_bc[0], s_bc[1], s_bc[2]};
+      const V3<R> p{s_bc[3], s_bc[4], s_bc[5]};
+      const V3<R> cvk{s_bc[6], s_bc[7], s_bc[8]};
+      const uint32_t kid = s_bcid;
+      const int kl[2] = {id_lo(kid), id_hi(kid)};
+      const int kb[2] = {b0w + kl[0], kl[1] >= 0 ? b0w + kl[1] : -1};
+      V3<R> rel[2] = {sub(p, lds3(s_pos0, kl[0])), V3<R>{R(0), R(0), R(0)}};
+      if (kl[1] >= 0) rel[1] = sub(p, lds3(s_pos0, kl[1]));
+      bool aw0 = (s_flags[kl[0]] & F_AWAKE) != 0;
+      bool aw1 = kl[1] >= 0 && (s_flags[kl[1]] & F_AWAKE) != 0;
+      if (kl[1] >= 0 && (aw0 || aw1) && !(aw0 && aw1)) {
+        const int t = aw0 ? kl[1] : kl[0];
+        if (lane == 0) {
+          s_flags[t] = uint8_t(s_flags[t] | F_AWAKE);
+          b.motion[b0w + t] = R(kSleepEpsilon * 2.0);
+        }
+      }
+      __syncthreads();
+      R m[9];
+      basis_from_scale(n, s_bs[bi], m);
+      V3<R> vc[2], rc[2];
+      apply_velocity_change(a, kb, rel, m, best, cvk, vc, rc, lane);
+      if constexpr (KIND == KIND_LABEL)
+        if (lane == 0) add_label(a, base + bi, vc, rc);
+#pragma unroll
+      for (int s = 0; s < SLOTS; ++s) {
+        if (64 * s + lane >= C) continue;
+        const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+        bool any = false;
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+          for (int d = 0; d < 2; ++d) any |= (jb[bb] >= 0 && jb[bb] == kl[d]);
+        if (!any) continue;
+        R mj[9];
+        basis_from_scale(nrm[s], s_bs[64 * s + lane], mj);
+        V3<R> cvj = cvl[s];
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+          if (jb[bb] < 0) continue;
+          const V3<R> rj = sub(get_pt(s), lds3(s_pos0, jb[bb]));
+#pragma unroll
+          for (int d = 0; d < 2; ++d) {
+            if (jb[bb] == kl[d]) {
+              V3<R> dv = add(cross(rc[d], rj), vc[d]);
+              R sign = R(1);
+              if (bb == 1) sign = R(-1);
+              cvj = add(cvj, mul(m3tv(mj, dv), sign));
+            }
+          }
+        }
+        cvl[s] = cvj;
+        key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x);
+      }
+      __syncthreads();
+      ++it;
+      ++used;
     }
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
@@ -1020,7 +1029,7 @@ constexpr size_t resolve_lds_bytes() {
 
 // One launch for every world: each workgroup takes the world order[blockIdx]
 // (larger worlds first) and runs the variant for its contact count.
-template <typename R>
+template <typename R, int KIND>
 __global__ void __launch_bounds__(64) k_resolve(RS<R> a) {
   __shared__ __align__(16) char lds[resolve_lds_bytes<R>()];
   const int lane = threadIdx.x;
@@ -1029,19 +1038,19 @@ __global__ void __launch_bounds__(64) k_resolve(RS<R> a) {
   const int C = a.world_cnt[w];
   const int b0w = a.body_off[w], n = a.body_off[w + 1] - b0w;
   if (!fits_registers(C, n)) {
-    resolve_hbm(a, w, C, lds, lane);
+    resolve_hbm<R, KIND>(a, w, C, lds, lane);
   } else if (C <= 64) {
-    resolve_reg<R, 1>(a, w, C, b0w, n, lds, lane);
+    resolve_reg<R, 1, KIND>(a, w, C, b0w, n, lds, lane);
   } else if (C <= 128) {
-    resolve_reg<R, 2>(a, w, C, b0w, n, lds, lane);
+    resolve_reg<R, 2, KIND>(a, w, C, b0w, n, lds, lane);
   } else if (C <= 256) {
-    resolve_reg<R, 4>(a, w, C, b0w, n, lds, lane);
+    resolve_reg<R, 4, KIND>(a, w, C, b0w, n, lds, lane);
   } else if (C <= 512) {
-    resolve_reg<R, 8>(a, w, C, b0w, n, lds, lane);
+    resolve_reg<R, 8, KIND>(a, w, C, b0w, n, lds, lane);
   } else if (C <= 768) {
-    resolve_reg<R, 12>(a, w, C, b0w, n, lds, lane);
+    resolve_reg<R, 12, KIND>(a, w, C, b0w, n, lds, lane);
   } else {
-    resolve_reg<R, 16>(a, w, C, b0w, n, lds, lane);
+    resolve_reg<R, 16, KIND>(a, w, C, b0w, n, lds, lane);
   }
 }
 
@@ -1069,8 +1078,14 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   a.polish_tol = L.polish_tol;
   a.polish_mult = L.polish_mult;
   if ((a.mode & RM_POLISH) && (!a.world_mask || !a.delta || !a.vel0)) return hipErrorInvalidValue;
+  if ((a.mode & RM_LABEL) && !a.labels) return hipErrorInvalidValue;
   if (a.n_worlds <= 0) return hipSuccess;
-  k_resolve<R><<<a.n_worlds, 64, 0, st>>>(a);
+  if (a.mode & RM_POLISH)
+    k_resolve<R, KIND_POLISH><<<a.n_worlds, 64, 0, st>>>(a);
+  else if (a.mode & RM_LABEL)
+    k_resolve<R, KIND_LABEL><<<a.n_worlds, 64, 0, st>>>(a);
+  else
+    k_resolve<R, KIND_PLAIN><<<a.n_worlds, 64, 0, st>>>(a);
   return hipGetLastError();
 }
 

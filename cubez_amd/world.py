@@ -234,10 +234,12 @@ class World:
         self._chk(self._lib.cbz_sur_grad(self._h, len(X), _ptr(X), _ptr(Y), _ptr(g), _ptr(loss)))
         return g, float(loss[0])
 
-    def sur_predict(self, X):
+    def sur_predict(self, X, layered=False):
+        """Predictions for raw feature rows: the fused inference kernel, or
+        (layered=True) the per-layer kernels training uses."""
         X = np.ascontiguousarray(X, np.float32)
         out = np.empty((len(X), self.N_OUT), np.float32)
-        self._chk(self._lib.cbz_sur_predict(self._h, len(X), _ptr(X), _ptr(out)))
+        self._chk(self._lib.cbz_sur_predict(self._h, len(X), _ptr(X), _ptr(out), 1 if layered else 0))
         return out
 
     SUR_ZERO_PREDICTION = 1

@@ -222,8 +222,12 @@ int cbz_sur_init(cbz_ctx *ctx, uint64_t seed);
 int cbz_sur_train(cbz_ctx *ctx, int32_t steps, int32_t batch, float lr, uint64_t seed, float *losses);
 /* loss and parameter gradient on given rows (raw features and labels) */
 int cbz_sur_grad(cbz_ctx *ctx, int32_t rows, const float *features, const float *labels, float *grad, float *loss);
-/* predictions (denormalised) for given rows of raw features */
-int cbz_sur_predict(cbz_ctx *ctx, int32_t rows, const float *features, float *out);
+/* predictions (denormalised) for given rows of raw features, by the fused
+ * inference kernel (all layers in one launch), or with
+ * CBZ_SUR_PREDICT_LAYERED by the per-layer kernels training uses (the two
+ * give the same bits) */
+#define CBZ_SUR_PREDICT_LAYERED 1
+int cbz_sur_predict(cbz_ctx *ctx, int32_t rows, const float *features, float *out, int32_t flags);
 /* ResolveContacts with the surrogate.  The position phase is the exact
  * one.  The network's per-contact velocity changes are summed per body and
  * applied; then, per world, the reference's worst-first velocity loop runs
