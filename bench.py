@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "body-steps/s, 4096 worlds: surrogate vs exact GPU resolver at ≤1e-3 m error"
 HBM_PEAK_GBS = 8000.0
+K1_LAUNCHES = 50
 
 # Algorithmic bytes of K1 per body-step: an integrated body reads 29 reals
 # (pos 3, quat 4, vel 3, rot 3, acc 3, inv_mass 1, iit_body 9, motion 1, the
@@ -44,6 +45,19 @@ HBM_PEAK_GBS = 8000.0
 
 
 _SCENE = None
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_integrate_hbm_traffic.md")
+
+
+def traffic_ratio(precision):
+    """HBM traffic / algorithmic bytes of K1, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE measurement (copy-calibrated), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith(f"traffic_over_algorithmic_{precision}:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def _cpu_world(args):
@@ -111,6 +125,9 @@ def run_path(World, sc, precision, steps, warmup):
     w.step(sc.dt, steps, sc.iter_mult)
     w.set_profiling(False)
     ms, cnt = w.phase_times()
+    # K1 alone: back-to-back launches on the initial state between two events
+    w.load(sc)
+    k1_ms, k1_bodies = w.integrate_timed(sc.dt, K1_LAUNCHES)
     w.close()
     sp, integrated, contacts, iters = (int(x) for x in cnt)
     names = ("integrate", "np_count", "np_scans", "np_emit", "resolve", "step_profiled")
@@ -123,6 +140,8 @@ def run_path(World, sc, precision, steps, warmup):
         "contacts_per_step": contacts / sp,
         "contacts_per_s": contacts / (ms[5] / 1000.0) if ms[5] > 0 else None,
         "iters_per_step": iters / sp,
+        "k1_ms_per_launch": k1_ms,
+        "k1_bodies_per_launch": k1_bodies / K1_LAUNCHES,
     }
 
 
@@ -161,6 +180,27 @@ def run_surrogate(World, sc, model, steps, warmup, tol, polish, zero=False):
     }
 
 
+def probe_k1(World, sc, launches=20):
+    """K1 launches and a calibration copy for hardware-counter runs (rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE): the copy moves a known number of bytes,
+    which scales the counters of K1."""
+    import torch
+
+    for prec in ("f32", "f64"):
+        with World(sc, precision=prec) as w:
+            for _ in range(launches):
+                w.integrate(sc.dt)
+            w.synchronize()
+        print(f"probe: {launches} x k_integrate<{prec}> on {sc.n_bodies} bodies", flush=True)
+    n = 64 << 20
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(launches):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"probe: {launches} copies of {4 * n} bytes (read {4 * n}, write {4 * n})", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +212,9 @@ def main():
     ap.add_argument("--exact-only", action="store_true", help="time the exact resolver only")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle wall time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-k1", action="store_true",
+                    help="for hardware-counter runs: 20 launches of K1 (FP32 and FP64) on the workload, then 20 "
+                         "copies of a 256 MiB float32 buffer as the calibration kernel; prints no JSON")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -182,6 +225,9 @@ def main():
     from cubez_amd import World, scenes, surrogate
 
     sc = scenes.batched(args.seed, args.worlds)
+    if args.probe_k1:
+        probe_k1(World, sc)
+        return
     nb = sc.n_bodies
     model_path = args.model or surrogate.DEFAULT_MODEL
     use_sur = not args.exact_only
@@ -201,14 +247,15 @@ def main():
 
     kern = exact32["kernels"]
     rs = 4
-    # K1 roofline (from the exact FP32 run, the same kernel the surrogate
-    # path launches): algorithmic bytes per launch / average launch time
+    # K1 roofline (FP32, the kernel the surrogate path launches): algorithmic
+    # bytes per launch / average launch time of K1_LAUNCHES back-to-back
+    # launches on the workload's initial state, timed with HIP events on the
+    # library's stream
     k1_awake = 29 * rs + 1 + 50 * rs + 1
     k1_sleep = 1 + 12 * rs + 12 * rs
-    steps_p = exact32["steps_profiled"]
-    integrated = exact32["integrated"]
-    k1_bytes = (integrated * k1_awake + (nb * steps_p - integrated) * k1_sleep) / steps_p
-    achieved = k1_bytes / (kern["integrate"] / 1000.0) / 1e9
+    aw = exact32["k1_bodies_per_launch"]
+    k1_bytes = aw * k1_awake + (nb - aw) * k1_sleep
+    achieved = k1_bytes / (exact32["k1_ms_per_launch"] / 1000.0) / 1e9
     main_path = sur if use_sur else exact32
     workload = (f"batched worlds: {args.worlds} worlds, plane + n~U{{8..256}} cubes each ({nb} bodies), "
                 f"first {args.steps} steps at dt=1/60 from the seeded initial state, all-pairs narrowphase, "
@@ -257,10 +304,16 @@ def main():
     out["exact_f32"] = {k: exact32[k] for k in ("value", "ms_per_step", "kernels", "iters_per_step")}
     out["kernels_ms_per_step"] = kern
     out["dominant_kernel"] = max(("integrate", "np_count", "np_scans", "np_emit", "resolve"), key=lambda k: kern[k])
+    ratio = traffic_ratio("f32")
     out["roofline"] = {
         "kernel": "k_integrate (K1: Integrate + CalculateDerivedData)",
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": k1_bytes,
+        "frac": achieved / HBM_PEAK_GBS, "traffic": ratio * k1_bytes if ratio else None,
+        "bytes_per_launch": k1_bytes, "ms_per_launch": exact32["k1_ms_per_launch"],
+        "measured_on": f"{K1_LAUNCHES} back-to-back launches on the initial state (every body awake)",
+        "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE of k_integrate<float>, copy-calibrated, "
+                           f"traffic/algorithmic = {ratio} ({os.path.relpath(TRAFFIC_PROFILE, ROOT)})")
+        if ratio else None,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.steps, args.cpu_budget)

@@ -56,7 +56,7 @@ EXPORTS = (
     "cbz_phase_times", "cbz_selftest_f64",
     "cbz_sur_num_params", "cbz_sur_set_params", "cbz_sur_get_params", "cbz_sur_set_norm", "cbz_sur_get_norm",
     "cbz_sur_reserve", "cbz_sur_collect", "cbz_sur_dataset", "cbz_sur_load_dataset", "cbz_sur_init",
-    "cbz_sur_train", "cbz_sur_grad", "cbz_sur_predict", "cbz_sur_resolve", "cbz_sur_world_mask", "cbz_sur_phase_times", "cbz_step_sur",
+    "cbz_sur_train", "cbz_sur_grad", "cbz_sur_predict", "cbz_sur_resolve", "cbz_sur_world_mask", "cbz_sur_phase_times", "cbz_step_sur", "cbz_integrate_timed",
 )
 
 
@@ -132,6 +132,7 @@ def load():
         "cbz_sur_world_mask": ([P, P], i32),
         "cbz_step_sur": ([P, f64, i32, i64, f64, i64, i32, P, P], i32),
         "cbz_sur_phase_times": ([P, P, P], i32),
+        "cbz_integrate_timed": ([P, f64, i32, P, P], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

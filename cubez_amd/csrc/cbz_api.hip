@@ -160,6 +160,8 @@ struct cbz_context {
   int64_t sur_cnt[4] = {0, 0, 0, 0};
   uint32_t* d_awake = nullptr;
   bool count_awake = false;
+  int64_t* d_prof = nullptr;           // [4] device-side step counters (k_prof_accum)
+  std::vector<hipEvent_t> pev;         // per-step event pool of profiled cbz_step calls
   void* pos0 = nullptr;  // body positions at the last prepareContacts
 };
 
@@ -305,6 +307,7 @@ void free_scene(cbz_ctx* c) {
   dfree(c->world_iters);
   dfree(c->order);
   dfree(c->d_awake);
+  dfree(c->d_prof);
   dfree(c->pos0);
   c->l_tiles_cap = c->l_pairs_cap = c->tiles_cap = 0;
   c->have_contacts = false;
@@ -425,7 +428,7 @@ int do_integrate(cbz_ctx* c, double dt) {
 
 // list_tiles < 0 selects the all-pairs space.
 template <typename R>
-int do_narrowphase(cbz_ctx* c, int32_t list_tiles) {
+int do_narrowphase(cbz_ctx* c, int32_t list_tiles, hipEvent_t* ev = nullptr) {
   int rc = refresh_brad(c);
   if (rc) return rc;
   NarrowLaunch<R> L;
@@ -474,7 +477,7 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles) {
   L.cand_cap = c->cand_cap;
   L.n_worlds = c->nw;
   HIPCHK(c, launch_fill_i32(c->overflow, 1, 0, c->stream));
-  HIPCHK(c, launch_narrowphase<R>(L, c->stream, c->prof ? &c->ev[1] : nullptr));
+  HIPCHK(c, launch_narrowphase<R>(L, c->stream, ev));
   c->have_contacts = true;
   return CBZ_OK;
 }
@@ -904,6 +907,8 @@ int cbz_destroy(cbz_ctx* c) {
   sur_free_model(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->pev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CBZ_OK;
@@ -1010,7 +1015,8 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc(c, &c->overflow, 2))) return rc;
   if ((rc = dalloc(c, &c->world_iters, 2 * (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->order, (size_t)n_worlds))) return rc;
-  if ((rc = dalloc(c, &c->d_awake, 1))) return rc;
+  if ((rc = dalloc(c, &c->d_awake, kAwakeSlots))) return rc;
+  if ((rc = dalloc(c, &c->d_prof, 4))) return rc;
   if ((rc = dalloc_bytes(c, &c->pos0, rs * 3 * (size_t)std::max<int64_t>(nb, 1)))) return rc;
   HIPCHK(c, hipMemset(c->world_cnt, 0, n_worlds * sizeof(int32_t)));
   HIPCHK(c, hipMemset(c->world_coff, 0, n_worlds * sizeof(int64_t)));
@@ -1411,52 +1417,62 @@ int cbz_step(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contact, i
   }
   int32_t* hist = nullptr;
   if (counts && n_steps > 0) HIPCHK(c, hipMalloc((void**)&hist, (size_t)n_steps * c->nw * sizeof(int32_t)));
-  // profiling events: ev[0] step start, ev[1..4] narrowphase (before count,
-  // after count, after scans, after emit), ev[5] after resolve
-  std::vector<int64_t> it_host;
-  if (c->prof) it_host.resize(2 * (size_t)c->nw);
+  // Profiling: per step s, events E[8s + k]: k = 0 step start, 7 after
+  // integrate, 1..4 narrowphase (before count, after count, after scans,
+  // after emit), 6 before resolve, 5 after resolve.  Nothing waits between
+  // steps (the counters accumulate on the device), so the stream stays ahead
+  // of the GPU and each event pair brackets its kernels alone.
+  hipEvent_t* E = nullptr;
+  if (c->prof && n_steps > 0) {
+    const size_t need = 8 * (size_t)n_steps;
+    while (c->pev.size() < need) {
+      hipEvent_t e = nullptr;
+      HIPCHK(c, hipEventCreate(&e));
+      c->pev.push_back(e);
+    }
+    E = c->pev.data();
+    HIPCHK(c, launch_fill_i32((int32_t*)c->d_prof, 8, 0, c->stream));
+    HIPCHK(c, launch_fill_i32((int32_t*)c->d_awake, kAwakeSlots, 0, c->stream));
+  }
   c->count_awake = c->prof;
   for (int32_t s = 0; s < n_steps && rc == CBZ_OK; ++s) {
-    if (c->prof) {
-      HIPCHK(c, launch_fill_i32((int32_t*)c->d_awake, 1, 0, c->stream));
-      HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    }
+    hipEvent_t* e = E ? E + 8 * (size_t)s : nullptr;
+    if (e) HIPCHK(c, hipEventRecord(e[0], c->stream));
     rc = c->precision == CBZ_F64 ? do_integrate<double>(c, dt) : do_integrate<float>(c, dt);
     if (rc) break;
-    if (c->prof) HIPCHK(c, hipEventRecord(c->ev[7], c->stream));
-    rc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, -1) : do_narrowphase<float>(c, -1);
+    if (e) HIPCHK(c, hipEventRecord(e[7], c->stream));
+    rc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, -1, e ? e + 1 : nullptr)
+                                 : do_narrowphase<float>(c, -1, e ? e + 1 : nullptr);
     if (rc) break;
     if (hist)
       HIPCHK(c, hipMemcpyAsync(hist + (size_t)s * c->nw, c->world_cnt, c->nw * sizeof(int32_t),
                                hipMemcpyDeviceToDevice, c->stream));
-    if (c->prof) HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
+    if (e) HIPCHK(c, hipEventRecord(e[6], c->stream));
     rc = c->precision == CBZ_F64 ? do_resolve<double>(c, dt, iter_per_contact, 0)
                                  : do_resolve<float>(c, dt, iter_per_contact, 0);
     if (rc) break;
-    if (c->prof) {
-      HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
-      HIPCHK(c, hipEventSynchronize(c->ev[5]));
+    if (e) {
+      HIPCHK(c, hipEventRecord(e[5], c->stream));
+      HIPCHK(c, launch_prof_accum(c->d_awake, c->total, c->world_iters, 2 * c->nw, c->d_prof, c->stream));
+    }
+  }
+  if (E && rc == CBZ_OK) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int32_t s = 0; s < n_steps; ++s) {
+      hipEvent_t* e = E + 8 * (size_t)s;
       float t[5] = {0, 0, 0, 0, 0}, whole = 0;
-      HIPCHK(c, hipEventElapsedTime(&t[0], c->ev[0], c->ev[7]));  // integrate
-      HIPCHK(c, hipEventElapsedTime(&t[1], c->ev[1], c->ev[2]));  // count
-      HIPCHK(c, hipEventElapsedTime(&t[2], c->ev[2], c->ev[3]));  // scans
-      HIPCHK(c, hipEventElapsedTime(&t[3], c->ev[3], c->ev[4]));  // emit
-      HIPCHK(c, hipEventElapsedTime(&t[4], c->ev[6], c->ev[5]));  // resolve
-      HIPCHK(c, hipEventElapsedTime(&whole, c->ev[0], c->ev[5]));
+      HIPCHK(c, hipEventElapsedTime(&t[0], e[0], e[7]));  // integrate
+      HIPCHK(c, hipEventElapsedTime(&t[1], e[1], e[2]));  // count (cull + eval)
+      HIPCHK(c, hipEventElapsedTime(&t[2], e[2], e[3]));  // scans
+      HIPCHK(c, hipEventElapsedTime(&t[3], e[3], e[4]));  // emit
+      HIPCHK(c, hipEventElapsedTime(&t[4], e[6], e[5]));  // resolve
+      HIPCHK(c, hipEventElapsedTime(&whole, e[0], e[5]));
       for (int k = 0; k < 5; ++k) c->acc_ms[k] += t[k];
       c->acc_ms[5] += whole;
-      uint32_t awake = 0;
-      int64_t tot = 0;
-      HIPCHK(c, hipMemcpy(&awake, c->d_awake, sizeof awake, hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(&tot, c->total, sizeof tot, hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(it_host.data(), c->world_iters, it_host.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-      int64_t its = 0;
-      for (int64_t v : it_host) its += v;
-      c->acc_cnt[0] += 1;
-      c->acc_cnt[1] += awake;
-      c->acc_cnt[2] += tot;
-      c->acc_cnt[3] += its;
     }
+    int64_t acc[4] = {0, 0, 0, 0};
+    HIPCHK(c, hipMemcpy(acc, c->d_prof, sizeof acc, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 4; ++k) c->acc_cnt[k] += acc[k];
   }
   c->count_awake = false;
   if (rc == CBZ_OK) rc = check_overflow(c);
@@ -1491,6 +1507,32 @@ int cbz_phase_times(cbz_ctx* c, double* ms6, int64_t* counts4) {
     for (int k = 0; k < 4; ++k) counts4[k] = c->acc_cnt[k];
   for (double& v : c->acc_ms) v = 0;
   for (int64_t& v : c->acc_cnt) v = 0;
+  return CBZ_OK;
+}
+
+int cbz_integrate_timed(cbz_ctx* c, double dt, int32_t n, double* ms_per_launch, int64_t* integrated) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (n <= 0) return fail(c, CBZ_ERR_ARG, "integrate_timed: n must be positive");
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = refresh_pow(c, dt))) return rc;
+  HIPCHK(c, launch_fill_i32((int32_t*)c->d_awake, kAwakeSlots, 0, c->stream));
+  c->count_awake = true;
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  for (int32_t k = 0; k < n && rc == CBZ_OK; ++k)
+    rc = c->precision == CBZ_F64 ? do_integrate<double>(c, dt) : do_integrate<float>(c, dt);
+  c->count_awake = false;
+  if (rc) return rc;
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev[1]));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  uint32_t aw[kAwakeSlots];
+  HIPCHK(c, hipMemcpy(aw, c->d_awake, sizeof aw, hipMemcpyDeviceToHost));
+  int64_t tot = 0;
+  for (uint32_t v : aw) tot += v;
+  if (ms_per_launch) *ms_per_launch = ms / n;
+  if (integrated) *integrated = tot;
   return CBZ_OK;
 }
 

@@ -78,6 +78,8 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st);
 
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);
+hipError_t launch_prof_accum(uint32_t* awake, const int64_t* total, const int64_t* iters, int n, int64_t* acc,
+                             hipStream_t st);
 
 // Device self-test of the FP64 operations whose rounding the parity claim
 // depends on: op 0 = a/b, 1 = sqrt(a), 2 = a*b, 3 = a+b, 4 = a-b.

@@ -85,6 +85,10 @@ struct Contacts {
 // per tile instead.
 constexpr int kRowBlock = 16;
 
+// counters of awake bodies the integrate kernel fills when asked (summed by
+// the reader)
+constexpr int kAwakeSlots = 256;
+
 struct PairSpace {
   int32_t n_tiles;
   const int32_t* tile_world;

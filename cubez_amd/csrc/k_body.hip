@@ -81,9 +81,10 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_integrate(Bodies<R> b, R dt, R bias) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   uint8_t f = i < b.n ? b.flags[i] : uint8_t(F_STATIC);
-  if (b.n_awake) {
+  if (b.n_awake) {  // kAwakeSlots counters, so the waves' atomics do not queue on one address
     const uint64_t m = __ballot(!(f & F_STATIC) && (f & F_AWAKE));
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(b.n_awake, uint32_t(__popcll(m)));
+    const int wv = int(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(b.n_awake + (wv & (kAwakeSlots - 1)), uint32_t(__popcll(m)));
   }
   if (i >= b.n) return;
   const int64_t S = b.stride;

@@ -147,6 +147,14 @@ class World:
         self._chk(self._lib.cbz_step(self._h, float(dt), int(n_steps), int(iter_per_contact), _ptr(buf)))
         return buf
 
+    def integrate_timed(self, dt, n=50):
+        """n back-to-back integrate launches timed with HIP events on the
+        library's stream: (ms per launch, bodies integrated over the n)."""
+        ms = np.zeros(1)
+        cnt = np.zeros(1, np.int64)
+        self._chk(self._lib.cbz_integrate_timed(self._h, float(dt), int(n), _ptr(ms), _ptr(cnt)))
+        return float(ms[0]), int(cnt[0])
+
     def synchronize(self):
         self._chk(self._lib.cbz_synchronize(self._h))
 

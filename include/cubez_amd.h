@@ -178,6 +178,12 @@ int cbz_synchronize(cbz_ctx *ctx);
  * resets the accumulators. */
 int cbz_set_profiling(cbz_ctx *ctx, int32_t on);
 int cbz_phase_times(cbz_ctx *ctx, double *ms6, int64_t *counts4);
+/* n back-to-back cbz_integrate(dt) launches timed by two HIP events on the
+ * context's stream: the average ms per launch and the bodies integrated
+ * (awake at Integrate) over the n launches.  Per-kernel event pairs inside a
+ * step add tens of microseconds of their own to a kernel this short; this
+ * is how the bench times K1.  Advances the state by n integrations. */
+int cbz_integrate_timed(cbz_ctx *ctx, double dt, int32_t n, double *ms_per_launch, int64_t *integrated);
 /* Device FP64 arithmetic self-test (the rounding the exact path relies on):
  * op 0: a/b, 1: sqrt(a), 2: a*b, 3: a+b, 4: a-b, elementwise over n. */
 int cbz_selftest_f64(cbz_ctx *ctx, int32_t op, int64_t n, const double *a, const double *b, double *out);
