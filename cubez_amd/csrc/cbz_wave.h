@@ -15,6 +15,12 @@ __device__ __forceinline__ int wave_sum_i32(int v) {
   return v;
 }
 
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // Inclusive prefix sum across the 64 lanes.
 __device__ __forceinline__ int wave_incl_scan_i32(int v) {
   const int l = lane_id();

@@ -24,6 +24,11 @@ namespace cbz {
 
 constexpr int kKeyCap = 2048;
 
+// Relative energy slack of the surrogate's acceptance test (rounding); the
+// absolute slack is 1/2 tol^2 per kilogram: the kinetic energy of the
+// residual closing speed the test already accepts.
+constexpr double kEnergyRel = 1e-5;
+
 // Compile-time variants of the resolver kernel, so that the plain
 // resolver's hot loops carry no code (and no register pressure) of the
 // other two: KIND_PLAIN (ResolveContacts, or one of its phases),
@@ -571,6 +576,25 @@ __device__ void resolve_hbm(const RS<R>& a, int w, int C, char* lds, int lane) {
 // ---------------------------------------------------------------------------
 constexpr uint32_t kNoBody = 0xFFFFu;
 
+// Kinetic energy of a body, 1/2 (m |v|^2 + w . I w) with I the inverse of
+// the world inverse inertia tensor (a zero matrix for an infinite tensor,
+// math/matrix.go:133-137), in FP64; 0 for an infinite mass.  Also returns m.
+template <typename R>
+__device__ __forceinline__ double kinetic_energy(const V3<R>& v, const V3<R>& w, R inv_mass, const R* iit,
+                                                 double& mass) {
+  mass = 0.0;
+  if (!(inv_mass > R(0))) return 0.0;
+  mass = 1.0 / double(inv_mass);
+  double m[9], I[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) m[k] = double(iit[k]);
+  m3inv(m, I);
+  const V3<double> wd{double(w.x), double(w.y), double(w.z)};
+  const V3<double> Iw = m3v(I, wd);
+  const double vv = double(v.x) * double(v.x) + double(v.y) * double(v.y) + double(v.z) * double(v.z);
+  return 0.5 * (mass * vv + dot(wd, Iw));
+}
+
 template <typename R>
 __device__ __forceinline__ V3<R> lds3(const R* a, int i) {
   return V3<R>{a[i], a[kRegBodies + i], a[2 * kRegBodies + i]};
@@ -839,13 +863,20 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
   used = 0;
   if (do_vel) {
     bool polishing = KIND == KIND_POLISH && a.world_mask[w] == 0;
+    double ke_before = 0.0, world_mass = 0.0;
     if constexpr (KIND == KIND_POLISH) {
       if (polishing) {
       // take the predicted changes: snapshot, then add them to the bodies
       const float* dl = a.delta;
+      double ke = 0.0, mass = 0.0;
       for (int i = lane; i < n; i += 64) {
         const int g = b0w + i;
         const V3<R> v = ld3(b.vel, S, g), r = ld3(b.rot, S, g);
+        R iw[9];
+        ld9(b.iit_world, S, g, iw);
+        double mi;
+        ke += kinetic_energy(v, r, b.inv_mass[g], iw, mi);
+        mass += mi;
         st3(a.vel0, S, g, v);
         st3(a.vel0 + 3 * S, S, g, r);
         a.vel0[6 * S + g] = b.motion[g];
@@ -887,6 +918,8 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
         }
       }
       __syncthreads();
+      ke_before = wave_sum_f64(ke);
+      world_mass = wave_sum_f64(mass);
       }
     }
     // The reference's loop.  A polished world first runs it to
@@ -907,9 +940,36 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
         }
       }
       wave_argmax_dpp(best, bi);
-      if (bi == INT_MAX) break;
-      if (it >= budget) {
+      bool reject = false;  // the polish did not give an acceptable result
+      if (bi == INT_MAX) {
+        if constexpr (KIND == KIND_POLISH) {
+          // Every contact is within velocityEpsilon + tol of closing.  The
+          // plain phase applies only compressive impulses inside the
+          // friction cone with restitution below 1, so it never adds kinetic
+          // energy; a polished world that ends with more than it started
+          // with, beyond a slack of 1/2 tol^2 per kilogram (which also covers
+          // the acceleration term of the restitution target,
+          // contact.go:87-112), is rejected.  This is what stops a wrong
+          // prediction from pushing bodies apart.
+          if (polishing) {
+            double ke = 0.0, mass = 0.0;
+            for (int i = lane; i < n; i += 64) {
+              const int g = b0w + i;
+              R iw[9];
+              ld9(b.iit_world, S, g, iw);
+              ke += kinetic_energy(ld3(b.vel, S, g), ld3(b.rot, S, g), b.inv_mass[g], iw, mass);
+            }
+            const double ke_after = wave_sum_f64(ke);
+            const double tol = double(a.polish_tol);
+            reject = !(ke_after <= ke_before * (1.0 + kEnergyRel) + 0.5 * tol * tol * world_mass);
+          }
+        }
+        if (!reject) break;
+      } else if (it >= budget) {
         if (KIND != KIND_POLISH || !polishing) break;
+        reject = true;
+      }
+      if (reject) {
         for (int i = lane; i < n; i += 64) {
           const int g = b0w + i;
           st3(b.vel, S, g, ld3(a.vel0, S, g));

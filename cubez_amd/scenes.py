@@ -125,6 +125,7 @@ class Scene:
     dt: float = 1.0 / 60.0
     steps: int = 600
     iter_mult: int = 4
+    meta: dict = field(default_factory=dict)
 
     @property
     def n_worlds(self):
@@ -330,5 +331,6 @@ def towers(seed=0, n_worlds=16384, pitch=4.0):
     _finish(f, np.zeros(n, np.int32), np.ones((n, 3)), np.full(n, 5.0), pos, quat,
             np.zeros((n, 3)), np.zeros((n, 3)))
     po, pn, pof = _ground(n_worlds)
+    tops = np.cumsum(heights[:int(n_towers.sum())]) - 1  # the top cube of every tower
     return Scene(body_off, po, pn, pof, f, name=f"towers-{n_worlds}-s{seed}",
-                 dt=1.0 / 120.0, steps=1200, iter_mult=4)
+                 dt=1.0 / 120.0, steps=1200, iter_mult=4, meta={"tower_top": tops})

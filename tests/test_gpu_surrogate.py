@@ -201,6 +201,43 @@ def test_infinite_tolerance_accepts_every_world():
     assert nf == 0
 
 
+def test_prediction_that_pushes_bodies_apart_is_rejected():
+    """A prediction that drives contacts apart (5 m/s up for every contact's
+    first body) leaves no contact closing, so the closing-speed test alone
+    would accept it; the separation bound rejects it.  Rejected worlds are
+    the plain resolver's result bit for bit, and no accepted world keeps the
+    push."""
+    norm = np.concatenate([np.zeros(40), np.ones(40), np.zeros(12), np.ones(12)]).astype(np.float32)
+    norm[80 + 1] = 5.0  # out_mean of dv_body0.y
+    sc, steps = _settled(11, worlds=16, steps=120)
+    rejected = 0
+    with World(sc) as p, World(sc) as s:
+        p.step(sc.dt, steps, 4)
+        s.step(sc.dt, steps, 4)
+        s.sur_set_norm(norm)
+        s.sur_set_params(np.zeros(s.sur_num_params(), np.float32))
+        for _ in (0, 1, 2):
+            p.integrate(sc.dt)
+            s.integrate(sc.dt)
+            p.narrowphase()
+            s.narrowphase()
+            p.resolve(sc.dt, 4)
+            s.sur_resolve(sc.dt, 4, 0.06, polish=1)
+            mask = s.sur_world_mask()
+            ps, ss = p.state(), s.state()
+            for w in range(sc.n_worlds):
+                b0, b1 = sc.body_off[w], sc.body_off[w + 1]
+                if mask[w]:
+                    rejected += 1
+                    for k in ps:
+                        assert np.array_equal(ps[k][b0:b1], ss[k][b0:b1]), (w, k)
+                else:
+                    assert ss["vel"][b0:b1, 1].max() < 1.0, w
+            for k in set(ps) | {"last_acc", "motion", "flags"}:
+                s.set(k, p.get(k))
+    assert rejected > 0
+
+
 def test_zero_prediction_at_zero_tolerance_is_the_plain_loop():
     """Polishing from a zero prediction to velocityEpsilon with the plain
     budget is the reference's velocity loop: bit-identical to plain."""
