@@ -44,7 +44,13 @@ FIELDS = {
     "lin_damp": (15, 1, "f"),
     "ang_damp": (16, 1, "f"),
     "flags": (17, 1, "u"),
+    "friction": (18, 1, "f"),
+    "restitution": (19, 1, "f"),
 }
+
+# cbz_set_materials flags
+MATERIALS = 1
+MATERIALS_FRICTIONLESS_TWO_BODY = 2
 
 # every symbol include/cubez_amd.h declares
 EXPORTS = (
@@ -57,6 +63,7 @@ EXPORTS = (
     "cbz_sur_num_params", "cbz_sur_set_params", "cbz_sur_get_params", "cbz_sur_set_norm", "cbz_sur_get_norm",
     "cbz_sur_reserve", "cbz_sur_collect", "cbz_sur_dataset", "cbz_sur_load_dataset", "cbz_sur_init",
     "cbz_sur_train", "cbz_sur_grad", "cbz_sur_predict", "cbz_sur_resolve", "cbz_sur_world_mask", "cbz_sur_phase_times", "cbz_step_sur", "cbz_integrate_timed",
+    "cbz_set_materials", "cbz_set_plane_materials", "cbz_set_contact_materials",
 )
 
 
@@ -67,7 +74,8 @@ class Config(ctypes.Structure):
 
 class ContactView(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in
-                ("body0", "body1", "point", "normal", "pen", "c2w", "rel", "cvel", "ddv")]
+                ("body0", "body1", "point", "normal", "pen", "c2w", "rel", "cvel", "ddv", "friction",
+                 "restitution")]
 
 
 class CubezError(RuntimeError):
@@ -133,6 +141,9 @@ def load():
         "cbz_step_sur": ([P, f64, i32, i64, f64, i64, i32, P, P], i32),
         "cbz_sur_phase_times": ([P, P, P], i32),
         "cbz_integrate_timed": ([P, f64, i32, P, P], i32),
+        "cbz_set_materials": ([P, i32], i32),
+        "cbz_set_plane_materials": ([P, P, P], i32),
+        "cbz_set_contact_materials": ([P, P, P], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -25,6 +25,7 @@ struct FieldInfo {
 const FieldInfo kFields[CBZ_NUM_FIELDS] = {
     {1, 1},  {3, 0}, {3, 0}, {4, 0}, {3, 0},  {3, 0}, {3, 0}, {1, 0}, {9, 0},
     {9, 0},  {12, 0}, {12, 0}, {12, 0}, {3, 0}, {1, 0}, {1, 3}, {1, 3}, {1, 2},
+    {1, 0},  {1, 0},
 };
 
 constexpr uint8_t kUserFlags = CBZ_FLAG_AWAKE | CBZ_FLAG_CAN_SLEEP | CBZ_FLAG_STATIC;
@@ -59,6 +60,10 @@ struct cbz_context {
   double* brad = nullptr;
   void* plane_normal = nullptr;
   void* plane_offset = nullptr;
+  // contact materials (cbz_set_materials)
+  int32_t mat_flags = 0;
+  void* plane_fric = nullptr;  // [np]
+  void* plane_rest = nullptr;  // [np]
   int32_t* d_body_off = nullptr;
   int32_t* d_plane_off = nullptr;
 
@@ -91,7 +96,7 @@ struct cbz_context {
   int64_t cap = 0;
   void* c_body0 = nullptr;
   void* c_body1 = nullptr;
-  void* c_real = nullptr;  // point 3, normal 3, pen 1, c2w 9, rel 6, cvel 3, ddv 1  (26 planes)
+  void* c_real = nullptr;  // point 3, normal 3, pen 1, c2w 9, rel 6, cvel 3, ddv 1, friction 1, restitution 1 (28 planes)
   int32_t* world_cnt = nullptr;
   int64_t* world_coff = nullptr;
   int64_t* total = nullptr;
@@ -278,6 +283,8 @@ void free_scene(cbz_ctx* c) {
   dfree(c->brad);
   dfree(c->plane_normal);
   dfree(c->plane_offset);
+  dfree(c->plane_fric);
+  dfree(c->plane_rest);
   dfree(c->d_body_off);
   dfree(c->d_plane_off);
   dfree(c->d_tile_world);
@@ -355,6 +362,9 @@ Contacts<R> contacts_view(cbz_ctx* c) {
   v.rel = base + 16 * S;
   v.cvel = base + 22 * S;
   v.ddv = base + 25 * S;
+  const bool mat = (c->mat_flags & CBZ_MATERIALS) != 0;
+  v.fric = mat ? base + 26 * S : nullptr;
+  v.rest = mat ? base + 27 * S : nullptr;
   return v;
 }
 
@@ -476,6 +486,10 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles, hipEvent_t* ev = nullptr) {
   L.n_cand = c->n_cand;
   L.cand_cap = c->cand_cap;
   L.n_worlds = c->nw;
+  L.mat.body_fric = (const R*)c->field[CBZ_FRICTION];
+  L.mat.body_rest = (const R*)c->field[CBZ_RESTITUTION];
+  L.mat.plane_fric = (const R*)c->plane_fric;
+  L.mat.plane_rest = (const R*)c->plane_rest;
   HIPCHK(c, launch_fill_i32(c->overflow, 1, 0, c->stream));
   HIPCHK(c, launch_narrowphase<R>(L, c->stream, ev));
   c->have_contacts = true;
@@ -487,6 +501,7 @@ int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode 
                uint8_t* mask = nullptr, float* labels = nullptr, double polish_tol = 0, int64_t polish_mult = 0) {
   ResolveLaunch<R> L;
   L.mode = mode;
+  if (c->mat_flags & CBZ_MATERIALS_FRICTIONLESS_TWO_BODY) L.mode |= RM_FRICTIONLESS2;
   L.world_mask = mask;
   L.labels = labels;
   if (mode & RM_POLISH) {
@@ -955,9 +970,14 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc(c, &c->brad, (size_t)nb))) return rc;
   if ((rc = dalloc_bytes(c, &c->plane_normal, rs * 3 * (size_t)std::max(np, 1)))) return rc;
   if ((rc = dalloc_bytes(c, &c->plane_offset, rs * (size_t)std::max(np, 1)))) return rc;
+  if ((rc = dalloc_bytes(c, &c->plane_fric, rs * (size_t)std::max(np, 1)))) return rc;
+  if ((rc = dalloc_bytes(c, &c->plane_rest, rs * (size_t)std::max(np, 1)))) return rc;
   if (np > 0) {
     if ((rc = upload_real(c, c->plane_normal, plane_normal, np, 3))) return rc;
     if ((rc = upload_real(c, c->plane_offset, plane_offset, np, 1))) return rc;
+    const std::vector<double> pf((size_t)np, kFriction), pr((size_t)np, kRestitution);
+    if ((rc = upload_real(c, c->plane_fric, pf.data(), np, 1))) return rc;
+    if ((rc = upload_real(c, c->plane_rest, pr.data(), np, 1))) return rc;
   }
   if ((rc = dalloc(c, &c->d_body_off, (size_t)n_worlds + 1))) return rc;
   if ((rc = dalloc(c, &c->d_plane_off, (size_t)n_worlds + 1))) return rc;
@@ -1002,7 +1022,7 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   c->cap = c->cap_cfg > 0 ? c->cap_cfg : 16 * nb + 4096;
   if ((rc = dalloc_bytes(c, &c->c_body0, sizeof(int32_t) * (size_t)c->cap))) return rc;
   if ((rc = dalloc_bytes(c, &c->c_body1, sizeof(int32_t) * (size_t)c->cap))) return rc;
-  if ((rc = dalloc_bytes(c, &c->c_real, rs * 26 * (size_t)c->cap))) return rc;
+  if ((rc = dalloc_bytes(c, &c->c_real, rs * 28 * (size_t)c->cap))) return rc;
   if ((rc = dalloc(c, &c->hits, (size_t)c->cap))) return rc;
   if ((rc = dalloc(c, &c->n_hits, 1))) return rc;
   // candidate pairs of the cull: 4 per contact slot (64 per body by default)
@@ -1065,6 +1085,11 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = upload_real(c, c->field[CBZ_OFFSET], t.data(), nb, 12))) return rc;
   for (int64_t i = 0; i < nb; ++i) t[i] = 0.6;  // SetAwake(true): motion = 2*sleepEpsilon
   if ((rc = upload_real(c, c->field[CBZ_MOTION], t.data(), nb, 1))) return rc;
+  // contact materials: the reference's constants (colliders.go:201-202)
+  for (int64_t i = 0; i < nb; ++i) t[i] = kFriction;
+  if ((rc = upload_real(c, c->field[CBZ_FRICTION], t.data(), nb, 1))) return rc;
+  for (int64_t i = 0; i < nb; ++i) t[i] = kRestitution;
+  if ((rc = upload_real(c, c->field[CBZ_RESTITUTION], t.data(), nb, 1))) return rc;
   std::vector<int32_t> sh((size_t)nb, CBZ_CUBE);
   if (nb > 0) HIPCHK(c, hipMemcpy(c->field[CBZ_SHAPE], sh.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice));
   std::vector<uint8_t> fl((size_t)nb, CBZ_FLAG_AWAKE | CBZ_FLAG_CAN_SLEEP | F_OFFSET_ID);
@@ -1301,6 +1326,19 @@ int cbz_get_contacts(cbz_ctx* c, int32_t world, cbz_contact_view* out) {
     if (!p.dst) continue;
     if ((rc = download_real(c, base + (size_t)p.plane0 * S * rs, S, off, n, p.k, p.dst))) return rc;
   }
+  // Contact.Friction / Contact.Restitution: per contact with materials on,
+  // else the constants every contact carries
+  const bool mat = (c->mat_flags & CBZ_MATERIALS) != 0;
+  const Part mats[] = {{out->friction, 26, 1}, {out->restitution, 27, 1}};
+  const double consts[] = {kFriction, kRestitution};
+  for (int m = 0; m < 2; ++m) {
+    if (!mats[m].dst) continue;
+    if (mat) {
+      if ((rc = download_real(c, base + (size_t)mats[m].plane0 * S * rs, S, off, n, 1, mats[m].dst))) return rc;
+    } else {
+      for (int32_t k = 0; k < n; ++k) mats[m].dst[k] = consts[m];
+    }
+  }
   return CBZ_OK;
 }
 
@@ -1341,12 +1379,68 @@ int cbz_set_contacts(cbz_ctx* c, const int64_t* counts, const int32_t* body0, co
       if ((rc = upload_real(c, base + ((size_t)3 + k) * S * rs, t.data(), tot, 1))) return rc;
     }
     if ((rc = upload_real(c, base + (size_t)6 * S * rs, pen, tot, 1))) return rc;
+    if (c->mat_flags & CBZ_MATERIALS) {  // the constants, until cbz_set_contact_materials
+      const std::vector<double> f((size_t)tot, kFriction), e((size_t)tot, kRestitution);
+      if ((rc = upload_real(c, base + (size_t)26 * S * rs, f.data(), tot, 1))) return rc;
+      if ((rc = upload_real(c, base + (size_t)27 * S * rs, e.data(), tot, 1))) return rc;
+    }
   }
   HIPCHK(c, hipMemcpy(c->world_cnt, cnt.data(), cnt.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->world_coff, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->total, &tot, sizeof tot, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemset(c->overflow, 0, 2 * sizeof(int32_t)));
   c->have_contacts = true;
+  return CBZ_OK;
+}
+
+int cbz_set_materials(cbz_ctx* c, int32_t flags) {
+  if (!c) return CBZ_ERR_ARG;
+  if (flags & ~(CBZ_MATERIALS | CBZ_MATERIALS_FRICTIONLESS_TWO_BODY))
+    return fail(c, CBZ_ERR_ARG, "set_materials: unknown flag");
+  if ((flags & CBZ_MATERIALS_FRICTIONLESS_TWO_BODY) && !(flags & CBZ_MATERIALS))
+    return fail(c, CBZ_ERR_ARG, "set_materials: CBZ_MATERIALS_FRICTIONLESS_TWO_BODY needs CBZ_MATERIALS");
+  if (flags == c->mat_flags) return CBZ_OK;
+  if (c->stream) {
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  c->mat_flags = flags;
+  drop_graph(c);  // the captured kernels carry the contact view
+  c->have_contacts = false;
+  return CBZ_OK;
+}
+
+int cbz_set_plane_materials(cbz_ctx* c, const double* friction, const double* restitution) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  for (int32_t k = 0; friction && k < c->np; ++k)
+    if (!(friction[k] >= 0.0) || !std::isfinite(friction[k]))
+      return fail(c, CBZ_ERR_ARG, "set_plane_materials: friction must be finite and >= 0");
+  for (int32_t k = 0; restitution && k < c->np; ++k)
+    if (!std::isfinite(restitution[k])) return fail(c, CBZ_ERR_ARG, "set_plane_materials: restitution not finite");
+  if (c->np == 0) return CBZ_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (friction && (rc = upload_real(c, c->plane_fric, friction, c->np, 1))) return rc;
+  if (restitution && (rc = upload_real(c, c->plane_rest, restitution, c->np, 1))) return rc;
+  return CBZ_OK;
+}
+
+int cbz_set_contact_materials(cbz_ctx* c, const double* friction, const double* restitution) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (!(c->mat_flags & CBZ_MATERIALS)) return fail(c, CBZ_ERR_STATE, "set_contact_materials: materials are off");
+  if (!c->have_contacts) return fail(c, CBZ_ERR_STATE, "set_contact_materials: no contacts");
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = check_overflow(c))) return rc;
+  int64_t tot = 0;
+  HIPCHK(c, hipMemcpy(&tot, c->total, sizeof tot, hipMemcpyDeviceToHost));
+  if (tot <= 0) return CBZ_OK;
+  const size_t rs = real_size(c);
+  char* base = (char*)c->c_real;
+  const int64_t S = c->cap;
+  if (friction && (rc = upload_real(c, base + (size_t)26 * S * rs, friction, tot, 1))) return rc;
+  if (restitution && (rc = upload_real(c, base + (size_t)27 * S * rs, restitution, tot, 1))) return rc;
   return CBZ_OK;
 }
 
@@ -1658,6 +1752,7 @@ int cbz_sur_reserve(cbz_ctx* c, int64_t rows) {
 int cbz_sur_collect(cbz_ctx* c, double dt, int64_t iter_per_contact, int64_t* added) {
   int rc = need_scene(c);
   if (rc) return rc;
+  if (c->mat_flags & CBZ_MATERIALS) return fail(c, CBZ_ERR_STATE, "sur_collect: contact materials are on");
   if (!c->have_contacts) return fail(c, CBZ_ERR_STATE, "sur_collect: no contacts (run a narrowphase first)");
   HIPCHK(c, hipSetDevice(c->device));
   auto& u = c->sur;
@@ -1851,6 +1946,7 @@ int cbz_sur_resolve(cbz_ctx* c, double dt, int64_t iter_per_contact, double tol,
                     int32_t flags, int64_t* n_fallback) {
   int rc = need_scene(c);
   if (rc) return rc;
+  if (c->mat_flags & CBZ_MATERIALS) return fail(c, CBZ_ERR_STATE, "sur_resolve: contact materials are on");
   if (iter_per_contact < 0 || polish_per_contact < 0 || !(tol == tol))
     return fail(c, CBZ_ERR_ARG, "sur_resolve: bad arguments");
   if (!c->have_contacts) return fail(c, CBZ_ERR_STATE, "sur_resolve: no contacts (run a narrowphase first)");
@@ -1883,6 +1979,7 @@ int cbz_step_sur(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contac
                  int64_t polish_per_contact, int32_t flags, int64_t* counts, int64_t* fallbacks) {
   int rc = need_scene(c);
   if (rc) return rc;
+  if (c->mat_flags & CBZ_MATERIALS) return fail(c, CBZ_ERR_STATE, "step_sur: contact materials are on");
   if (n_steps < 0 || iter_per_contact < 0 || polish_per_contact < 0 || !(tol == tol))
     return fail(c, CBZ_ERR_ARG, "step_sur: bad arguments");
   if (!c->sur.have_model) return fail(c, CBZ_ERR_STATE, "step_sur: no model");

@@ -19,7 +19,10 @@ namespace cbz {
 // (velocities, rotations, motion, awake flags) and re-solved by the plain
 // velocity phase, and its mask byte set to 1.  Worlds whose mask byte is 1
 // run the plain velocity phase.
-enum : int32_t { RM_POS = 1, RM_VEL = 2, RM_LABEL = 4, RM_POLISH = 8 };
+// RM_FRICTIONLESS2: the frictionless impulse of a two-body contact includes
+// its second body (the reference omits it, contact.go:512); only contacts
+// whose friction is 0 (per-contact materials) take that impulse.
+enum : int32_t { RM_POS = 1, RM_VEL = 2, RM_LABEL = 4, RM_POLISH = 8, RM_FRICTIONLESS2 = 16 };
 
 template <typename R>
 struct NarrowLaunch {
@@ -41,6 +44,7 @@ struct NarrowLaunch {
   int32_t* n_cand;
   int64_t cand_cap;
   int32_t n_worlds;
+  Materials<R> mat{};  // per-body and per-plane materials (read when c.fric is set)
 };
 
 template <typename R>

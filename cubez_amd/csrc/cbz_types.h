@@ -72,6 +72,22 @@ struct Contacts {
   R* rel;          // [6][cap] relativeContactPosition[0], [1]
   R* cvel;         // [3][cap] contactVelocity
   R* ddv;          // [cap] desiredDeltaVelocity
+  // Contact.Friction / Contact.Restitution (contact.go:21-25) per contact
+  // when materials are on; null: every contact carries kFriction and
+  // kRestitution, as in the reference.
+  R* fric;         // [cap]
+  R* rest;         // [cap]
+};
+
+// Sources of the per-contact materials the narrowphase writes: friction and
+// restitution per body and per plane.  A contact takes the mean of its two
+// sides' values.
+template <typename R>
+struct Materials {
+  const R* body_fric;   // [n_bodies]
+  const R* body_rest;   // [n_bodies]
+  const R* plane_fric;  // [n_planes]
+  const R* plane_rest;  // [n_planes]
 };
 
 // Pair-space description for the narrowphase: the ordered pairs of world w

@@ -72,11 +72,18 @@ struct NP {
   int32_t* n_cand;
   int64_t cand_cap;
   int32_t n_worlds;
+  Materials<R> mat;  // read when c.fric is set
 };
 
+// Writes one contact.  m1 names the contact's second material: body b1 for
+// a body pair, -(k+1) for plane k (Bodies[1] is nil there).  With materials
+// on, the contact's friction and restitution are the means of body b0's and
+// m1's values (the reference gives every contact 0.9 and 0.1,
+// colliders.go:201-202, 248-249, 360-361, 437-438, 511-512, 704-705).
 template <typename R>
-__device__ __forceinline__ void put_contact(const Contacts<R>& c, int64_t slot, int b0, int b1, const D3& p,
+__device__ __forceinline__ void put_contact(const NP<R>& a, int64_t slot, int b0, int b1, int m1, const D3& p,
                                             const D3& n, double pen) {
+  const Contacts<R>& c = a.c;
   if (slot >= c.cap) return;
   const int64_t S = c.cap;
   c.body0[slot] = b0;
@@ -88,6 +95,13 @@ __device__ __forceinline__ void put_contact(const Contacts<R>& c, int64_t slot, 
   c.normal[S + slot] = R(n.y);
   c.normal[2 * S + slot] = R(n.z);
   c.pen[slot] = R(pen);
+  if (c.fric) {
+    const Materials<R>& m = a.mat;
+    const R f1 = m1 >= 0 ? m.body_fric[m1] : m.plane_fric[-m1 - 1];
+    const R e1 = m1 >= 0 ? m.body_rest[m1] : m.plane_rest[-m1 - 1];
+    c.fric[slot] = (m.body_fric[b0] + f1) * R(0.5);
+    c.rest[slot] = (m.body_rest[b0] + e1) * R(0.5);
+  }
 }
 
 // GetAxis / HalfSize[i] with a run-time index, re-read from the body arrays
@@ -113,14 +127,15 @@ __device__ __forceinline__ double transform_to_axis(const Col& c, const D3& a) {
 
 // CollisionSphere.CheckAgainstHalfSpace, colliders.go:180-207
 template <typename R, bool EMIT>
-__device__ __forceinline__ int sphere_half_space(const NP<R>& a, const Col& s, int si, const D3& pn, double po, int64_t slot) {
+__device__ __forceinline__ int sphere_half_space(const NP<R>& a, const Col& s, int si, int pk, const D3& pn, double po,
+                                                 int64_t slot) {
   D3 positionAxis = axis(s.T, 3);
   double r = s.h[0];
   double distance = dot(pn, positionAxis) - r;
   if ((distance <= po) == false) return 0;
   if (EMIT) {
     D3 p = add(mul(pn, distance + r * -1.0), positionAxis);
-    put_contact(a.c, slot, si, -1, p, pn, -distance);
+    put_contact(a, slot, si, -1, -pk - 1, p, pn, -distance);
   }
   return 1;
 }
@@ -128,7 +143,8 @@ __device__ __forceinline__ int sphere_half_space(const NP<R>& a, const Col& s, i
 // CollisionCube.CheckAgainstHalfSpace, colliders.go:308-366, with
 // intersectCubeAndHalfSpace (:750-760) as the early-out.
 template <typename R, bool EMIT>
-__device__ __forceinline__ int cube_half_space(const NP<R>& a, const Col& c, int ci, const D3& pn, double po, int64_t slot) {
+__device__ __forceinline__ int cube_half_space(const NP<R>& a, const Col& c, int ci, int pk, const D3& pn, double po,
+                                               int64_t slot) {
   double projectedRadius = transform_to_axis(c, pn);
   double cubeDistance = dot(pn, axis(c.T, 3)) - projectedRadius;
   if (!(cubeDistance <= po)) return 0;
@@ -142,7 +158,7 @@ __device__ __forceinline__ int cube_half_space(const NP<R>& a, const Col& c, int
     if (vertexDistance <= po) {
       if (EMIT) {
         D3 p = add(mul(pn, vertexDistance - po), vertexPos);
-        put_contact(a.c, slot + k, ci, -1, p, pn, po - vertexDistance);
+        put_contact(a, slot + k, ci, -1, -pk - 1, p, pn, po - vertexDistance);
       }
       ++k;
     }
@@ -160,7 +176,7 @@ __device__ __forceinline__ int sphere_sphere(const NP<R>& a, const Col& s1, int 
   if (EMIT) {
     D3 p = add(mul(midline, 0.5), p1);
     D3 n = mul(midline, 1.0 / size);
-    put_contact(a.c, slot, i1, i2, p, n, s1.h[0] + s2.h[0] - size);
+    put_contact(a, slot, i1, i2, i2, p, n, s1.h[0] + s2.h[0] - size);
   }
   return 1;
 }
@@ -201,7 +217,7 @@ __device__ __forceinline__ int cube_sphere(const NP<R>& a, const Col& c, int ci,
     } else {
       pen = 0.0;
     }
-    put_contact(a.c, slot, ci, si, cw, n, pen);
+    put_contact(a, slot, ci, si, si, cw, n, pen);
   }
   return 1;
 }
@@ -275,10 +291,10 @@ __device__ __forceinline__ int cube_cube(const NP<R>& a, const Col& one, int i1,
   D3 n, p;
   if (best < 3) {
     point_face(a, one, i1, two, toCenter, best, n, p);
-    put_contact(a.c, slot, i1, i2, p, n, pen);
+    put_contact(a, slot, i1, i2, i2, p, n, pen);
   } else if (best < 6) {
     point_face(a, two, i2, one, mul(toCenter, -1.0), best - 3, n, p);
-    put_contact(a.c, slot, i2, i1, p, n, pen);
+    put_contact(a, slot, i2, i1, i1, p, n, pen);
   } else {
     best -= 6;
     const int oneAxisIndex = best / 3;
@@ -309,7 +325,7 @@ __device__ __forceinline__ int cube_cube(const NP<R>& a, const Col& one, int i1,
     const double oneSize = oneAxisIndex < 3 ? half_g(a, i1, oneAxisIndex) : 0.0;
     const double twoSize = twoAxisIndex < 3 ? half_g(a, i2, twoAxisIndex) : 0.0;
     D3 cv = contact_point(ptOnOneEdge, oneAxis, oneSize, ptOnTwoEdge, twoAxis, twoSize, useOne);
-    put_contact(a.c, slot, i1, i2, cv, ax, pen);
+    put_contact(a, slot, i1, i2, i2, cv, ax, pen);
   }
   return 1;
 }
@@ -346,8 +362,8 @@ __device__ __forceinline__ int pair_eval(const NP<R>& a, int one, int two, int64
     double po = double(a.pl.offset[pk]);
     Col c;
     load_col(a.b, bi, c);
-    if (c.shape == SHAPE_SPHERE) return sphere_half_space<R, EMIT>(a, c, bi, pn, po, slot);
-    return cube_half_space<R, EMIT>(a, c, bi, pn, po, slot);
+    if (c.shape == SHAPE_SPHERE) return sphere_half_space<R, EMIT>(a, c, bi, pk, pn, po, slot);
+    return cube_half_space<R, EMIT>(a, c, bi, pk, pn, po, slot);
   }
   const int s1 = a.b.shape[one], s2 = a.b.shape[two];
   if (!(s1 == SHAPE_SPHERE && s2 == SHAPE_SPHERE) && far_apart(a, one, two, s1, s2)) return 0;
@@ -660,7 +676,10 @@ hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st, hipEvent
   a.n_cand = L.n_cand;
   a.cand_cap = L.cand_cap;
   a.n_worlds = L.n_worlds;
+  a.mat = L.mat;
   if (a.n_worlds <= 0) return hipSuccess;
+  if (a.c.fric && (!a.c.rest || !a.mat.body_fric || !a.mat.body_rest || !a.mat.plane_fric || !a.mat.plane_rest))
+    return hipErrorInvalidValue;
   const bool list = a.ps.list_one != nullptr;
   if (!list && (!a.hits || !a.n_hits || !a.cand || !a.n_cand)) return hipErrorInvalidValue;
   // ev (optional): recorded before the count, after it, after the scans

@@ -31,10 +31,12 @@ constexpr double kEnergyRel = 1e-5;
 
 // Compile-time variants of the resolver kernel, so that the plain
 // resolver's hot loops carry no code (and no register pressure) of the
-// other two: KIND_PLAIN (ResolveContacts, or one of its phases),
-// KIND_LABEL (also records every impulse's body changes per contact),
-// KIND_POLISH (the surrogate's polish with the plain fallback).
-enum : int { KIND_PLAIN = 0, KIND_LABEL = 1, KIND_POLISH = 2 };
+// others: KIND_PLAIN (ResolveContacts, or one of its phases), KIND_LABEL
+// (also records every impulse's body changes per contact), KIND_POLISH (the
+// surrogate's polish with the plain fallback), KIND_MAT (KIND_PLAIN with
+// per-contact friction and restitution, and the frictionless impulse for
+// contacts whose friction is 0).
+enum : int { KIND_PLAIN = 0, KIND_LABEL = 1, KIND_POLISH = 2, KIND_MAT = 3 };
 
 template <typename R>
 struct RS {
@@ -106,14 +108,28 @@ __device__ __forceinline__ V3<R> local_velocity(const Bodies<R>& b, int body, co
   return add(cv, accv);
 }
 
-// calculateDesiredDeltaVelocity, contact.go:87-112
+// Contact.Restitution and Contact.Friction of contact g: the reference's
+// constants (colliders.go:201-202) unless the kernel is KIND_MAT.
+template <typename R, int KIND>
+__device__ __forceinline__ R restitution_of(const Contacts<R>& c, int64_t g) {
+  if constexpr (KIND == KIND_MAT) return c.rest[g];
+  return R(kRestitution);
+}
+template <typename R, int KIND>
+__device__ __forceinline__ R friction_of(const Contacts<R>& c, int64_t g) {
+  if constexpr (KIND == KIND_MAT) return c.fric[g];
+  return R(kFriction);
+}
+
+// calculateDesiredDeltaVelocity, contact.go:87-112, with the contact's
+// restitution e
 template <typename R>
-__device__ __forceinline__ R desired_dv(const Bodies<R>& b, int b0, int b1, const V3<R>& n, R cvx, R dt) {
+__device__ __forceinline__ R desired_dv(const Bodies<R>& b, int b0, int b1, const V3<R>& n, R cvx, R dt, R e) {
   const int64_t S = b.stride;
   R vfa = R(0);
   if (awake(b, b0)) vfa += dot(mul(ld3(b.last_acc, S, b0), dt), n);
   if (b1 >= 0 && awake(b, b1)) vfa -= dot(mul(ld3(b.last_acc, S, b1), dt), n);
-  R restitution = R(kRestitution);
+  R restitution = e;
   if (dabs(cvx) < R(0.25)) restitution = R(0);
   return -cvx - restitution * (cvx - vfa);
 }
@@ -152,7 +168,7 @@ __device__ __forceinline__ void contact_basis(const V3<R>& n, R* m) {
 }
 
 // calculateInternals, contact.go:59-85, for contact g.
-template <typename R>
+template <typename R, int KIND>
 __device__ void prepare_contact(const RS<R>& a, int64_t g) {
   const Bodies<R>& b = a.b;
   const Contacts<R>& c = a.c;
@@ -177,7 +193,7 @@ __device__ void prepare_contact(const RS<R>& a, int64_t g) {
     r1 = sub(p, ld3(b.pos, S, b1));
     cv = sub(cv, local_velocity(b, b1, r1, m, a.dt));
   }
-  const R ddv = desired_dv(b, b0, b1, n, cv.x, a.dt);
+  const R ddv = desired_dv(b, b0, b1, n, cv.x, a.dt, restitution_of<R, KIND>(c, g));
 #pragma unroll
   for (int k = 0; k < 9; ++k) c.c2w[k * CS + g] = m[k];
   st3(c.rel, CS, g, r0);
@@ -295,13 +311,38 @@ __device__ __forceinline__ void skew(const V3<R>& v, R* m) {
   m[2] = -v.y; m[5] = v.x;  m[8] = R(0);
 }
 
-// calculateFrictionImpulse, contact.go:535-606.  (Every contact carries
-// Friction 0.9, so applyVelocityChange never takes the frictionless branch
-// of contact.go:459-461.)
+// calculateFrictionlessImpulse, contact.go:498-531, taken by contacts
+// whose friction is 0 (contact.go:459-461; only KIND_MAT has such
+// contacts).  ContactNormal is column 0 of contactToWorld.  The reference's
+// test at :512 is inverted (Bodies[1] == nil): a two-body contact never adds
+// its second body's terms, and a half-space contact dereferences nil there
+// (a Go panic; here the absent body adds nothing).  two_body
+// (RM_FRICTIONLESS2) adds the second body's terms when it exists.
+template <typename R>
+__device__ __forceinline__ V3<R> frictionless_impulse(const Bodies<R>& b, const int kb[2], const R* iit0,
+                                                      const R* iit1, const V3<R> rel[2], const R* c2w, R ddv,
+                                                      bool two_body) {
+  const V3<R> n{c2w[0], c2w[1], c2w[2]};
+  V3<R> dvw = cross(rel[0], n);
+  dvw = m3v(iit0, dvw);
+  dvw = cross(dvw, rel[0]);
+  R deltaVelocity = dot(dvw, n);
+  deltaVelocity += b.inv_mass[kb[0]];
+  if (two_body && kb[1] >= 0) {
+    dvw = cross(rel[1], n);
+    dvw = m3v(iit1, dvw);
+    dvw = cross(dvw, rel[1]);
+    deltaVelocity += dot(dvw, n);
+    deltaVelocity += b.inv_mass[kb[1]];
+  }
+  return V3<R>{ddv / deltaVelocity, R(0), R(0)};
+}
+
+// calculateFrictionImpulse, contact.go:535-606
 template <typename R>
 __device__ __forceinline__ V3<R> friction_impulse(const Bodies<R>& b, const int kb[2], const R* iit0, const R* iit1,
-                                                  const V3<R> rel[2], const R* c2w, R ddv, const V3<R>& cvel) {
-  const R friction = R(kFriction);
+                                                  const V3<R> rel[2], const R* c2w, R ddv, const V3<R>& cvel,
+                                                  R friction) {
   R inverseMass = b.inv_mass[kb[0]];
   R sk[9], t[9], dvw[9];
   skew(rel[0], sk);
@@ -343,10 +384,11 @@ __device__ __forceinline__ V3<R> friction_impulse(const Bodies<R>& b, const int 
   return ic;
 }
 
-// applyVelocityChange, contact.go:448-494
-template <typename R>
+// applyVelocityChange, contact.go:448-494, for a contact of the given
+// friction (the frictionless branch of :459-461 exists in KIND_MAT only)
+template <typename R, int KIND>
 __device__ void apply_velocity_change(const RS<R>& a, const int kb[2], const V3<R> rel[2], const R* c2w, R ddv,
-                                      const V3<R>& cvel, V3<R> vc[2], V3<R> rc[2], int lane) {
+                                      const V3<R>& cvel, V3<R> vc[2], V3<R> rc[2], int lane, R friction) {
   const Bodies<R>& b = a.b;
   const int64_t S = b.stride;
   R iit0[9], iit1[9];
@@ -354,7 +396,11 @@ __device__ void apply_velocity_change(const RS<R>& a, const int kb[2], const V3<
 #pragma unroll
   for (int k = 0; k < 9; ++k) iit1[k] = R(0);
   if (kb[1] >= 0) ld9(b.iit_world, S, kb[1], iit1);
-  const V3<R> ic = friction_impulse(b, kb, iit0, iit1, rel, c2w, ddv, cvel);
+  V3<R> ic;
+  if (KIND == KIND_MAT && friction == R(0))
+    ic = frictionless_impulse(b, kb, iit0, iit1, rel, c2w, ddv, (a.mode & RM_FRICTIONLESS2) != 0);
+  else
+    ic = friction_impulse(b, kb, iit0, iit1, rel, c2w, ddv, cvel, friction);
   const V3<R> impulse = m3v(c2w, ic);
   const V3<R> zero{R(0), R(0), R(0)};
   V3<R> torque = cross(rel[0], impulse);
@@ -395,7 +441,7 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
   if (KIND == KIND_POLISH && lane == 0) a.world_mask[w] = 1;
   // prepareContacts, contact.go:225-229
   for (int j = lane; j < C; j += 64) {
-    if (do_pos) prepare_contact(a, base + j);
+    if (do_pos) prepare_contact<R, KIND>(a, base + j);
     if (LDS) {
       key[j] = c.pen[base + j];
       kb0[j] = c.body0[base + j];
@@ -489,7 +535,7 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
     bool aw[2];
     match_awake(a, kb[0], kb[1], aw[0], aw[1], lane);
     V3<R> vc[2], rc[2];
-    apply_velocity_change(a, kb, rel, m, best, cv, vc, rc, lane);
+    apply_velocity_change<R, KIND>(a, kb, rel, m, best, cv, vc, rc, lane, friction_of<R, KIND>(c, g));
     if constexpr (KIND == KIND_LABEL)
       if (lane == 0) add_label(a, g, vc, rc);
     for (int j = lane; j < C; j += 64) {
@@ -521,7 +567,7 @@ __device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key
       }
       st3(c.cvel, CS, gj, cvj);
       const V3<R> nj = ld3(c.normal, CS, gj);
-      dkey[j] = desired_dv(b, jb[0], jb[1], nj, cvj.x, a.dt);
+      dkey[j] = desired_dv(b, jb[0], jb[1], nj, cvj.x, a.dt, restitution_of<R, KIND>(c, gj));
     }
     __syncthreads();
     ++used;
@@ -601,14 +647,14 @@ __device__ __forceinline__ V3<R> lds3(const R* a, int i) {
 }
 
 // calculateDesiredDeltaVelocity (contact.go:87-112) with the awake flags and
-// lastFrameAccelleration*dt from LDS.
+// lastFrameAccelleration*dt from LDS, for a contact of restitution e.
 template <typename R>
 __device__ __forceinline__ R desired_dv_lds(const uint8_t* fl, const R* lad, int l0, int l1, const V3<R>& n,
-                                            R cvx) {
+                                            R cvx, R e) {
   R vfa = R(0);
   if (fl[l0] & F_AWAKE) vfa += dot(lds3(lad, l0), n);
   if (l1 >= 0 && (fl[l1] & F_AWAKE)) vfa -= dot(lds3(lad, l1), n);
-  R restitution = R(kRestitution);
+  R restitution = e;
   if (dabs(cvx) < R(0.25)) restitution = R(0);
   return -cvx - restitution * (cvx - vfa);
 }
@@ -626,14 +672,23 @@ __device__ __forceinline__ V3<R> local_velocity_lds(const Bodies<R>& b, int body
   return add(cv, accv);
 }
 
+// Restitution of world-local contact j: from LDS in KIND_MAT, else the
+// reference's constant.
+template <typename R, int KIND>
+__device__ __forceinline__ R lds_restitution(const R* s_rest, int j) {
+  if constexpr (KIND == KIND_MAT) return s_rest[j];
+  return R(kRestitution);
+}
+
 __device__ __forceinline__ int id_lo(uint32_t id) { return int(id & 0xFFFFu); }
 __device__ __forceinline__ int id_hi(uint32_t id) { return (id >> 16) == kNoBody ? -1 : int(id >> 16); }
 
-template <typename R, int SLOTS>
+template <typename R, int SLOTS, int KIND>
 struct RegLds {
   // the widest variants keep contact points in LDS (in VGPRs they would spill)
   static constexpr bool PT_LDS = SLOTS >= 12;
   static constexpr int CAP = 64 * SLOTS;
+  static constexpr bool MAT = KIND == KIND_MAT;
   static constexpr size_t pos0 = 0;                                   // R[3][kRegBodies]
   static constexpr size_t lad = pos0 + 3 * kRegBodies * sizeof(R);    // R[3][kRegBodies]
   static constexpr size_t bs = lad + 3 * kRegBodies * sizeof(R);      // double[CAP] basis scale
@@ -641,12 +696,13 @@ struct RegLds {
   static constexpr size_t pt = bc + 16 * sizeof(R);                   // R[3][CAP] (PT_LDS)
   static constexpr size_t flags = pt + (PT_LDS ? 3 * CAP * sizeof(R) : 0);  // uint8[kRegBodies]
   static constexpr size_t bcid = flags + kRegBodies;                  // uint32
-  static constexpr size_t bytes = bcid + 16;
+  static constexpr size_t rest = bcid + 16;                           // R[CAP] restitution (MAT)
+  static constexpr size_t bytes = rest + (MAT ? CAP * sizeof(R) : 0);
 };
 
 template <typename R, int SLOTS, int KIND>
 __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* lds, int lane) {
-  using L = RegLds<R, SLOTS>;
+  using L = RegLds<R, SLOTS, KIND>;
   constexpr bool PT_LDS = L::PT_LDS;
   constexpr int CAP = L::CAP;
   R* s_pos0 = (R*)(lds + L::pos0);
@@ -656,6 +712,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
   R* s_pt = (R*)(lds + L::pt);
   uint8_t* s_flags = (uint8_t*)(lds + L::flags);
   uint32_t& s_bcid = *(uint32_t*)(lds + L::bcid);
+  R* s_rest = (R*)(lds + L::rest);  // KIND_MAT: restitution of contact j (world-local)
   const bool do_pos = (a.mode & RM_POS) != 0;
   const bool do_vel = (a.mode & RM_VEL) != 0 && (KIND == KIND_POLISH || !a.world_mask || a.world_mask[w]);
   if (a.world_iters && lane == 0) {
@@ -716,6 +773,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
       const V3<R> nn = ld3(c.normal, CS, g);
       const V3<R> p = ld3(c.point, CS, g);
       s_bs[j] = basis_scale(nn);
+      if constexpr (KIND == KIND_MAT) s_rest[j] = c.rest[g];
       ids[s] = uint32_t(c0 - b0w) | (uint32_t(c1 >= 0 ? c1 - b0w : int(kNoBody)) << 16);
       nrm[s] = nn;
       if constexpr (PT_LDS) {
@@ -750,7 +808,8 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
         r1 = sub(p, lds3(s_pos0, l1));
         cv = sub(cv, local_velocity_lds(b, c1, lds3(s_lad, l1), r1, m));
       }
-      const R ddv = desired_dv_lds(s_flags, s_lad, l0, l1, nn, cv.x);
+      if constexpr (KIND == KIND_MAT) s_rest[j] = c.rest[g];
+      const R ddv = desired_dv_lds(s_flags, s_lad, l0, l1, nn, cv.x, lds_restitution<R, KIND>(s_rest, j));
 #pragma unroll
       for (int k = 0; k < 9; ++k) c.c2w[k * CS + g] = m[k];
       st3(c.rel, CS, g, r0);
@@ -914,7 +973,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
         // gave it, as in the plain loop
         if (moved) {
           cvl[s] = cvj;
-          key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x);
+          key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x, lds_restitution<R, KIND>(s_rest, 64 * s + lane));
         }
       }
       __syncthreads();
@@ -1027,7 +1086,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
       R m[9];
       basis_from_scale(n, s_bs[bi], m);
       V3<R> vc[2], rc[2];
-      apply_velocity_change(a, kb, rel, m, best, cvk, vc, rc, lane);
+      apply_velocity_change<R, KIND>(a, kb, rel, m, best, cvk, vc, rc, lane, friction_of<R, KIND>(c, base + bi));
       if constexpr (KIND == KIND_LABEL)
         if (lane == 0) add_label(a, base + bi, vc, rc);
 #pragma unroll
@@ -1058,7 +1117,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
           }
         }
         cvl[s] = cvj;
-        key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x);
+        key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x, lds_restitution<R, KIND>(s_rest, 64 * s + lane));
       }
       __syncthreads();
       ++it;
@@ -1081,17 +1140,17 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
 }
 
 constexpr size_t cmax(size_t x, size_t y) { return x > y ? x : y; }
-template <typename R>
+template <typename R, int KIND>
 constexpr size_t resolve_lds_bytes() {
-  return cmax(cmax(RegLds<R, 16>::bytes, RegLds<R, 12>::bytes),
-              cmax(RegLds<R, 8>::bytes, kKeyCap * (sizeof(R) + 2 * sizeof(int32_t))));
+  return cmax(cmax(RegLds<R, 16, KIND>::bytes, RegLds<R, 12, KIND>::bytes),
+              cmax(RegLds<R, 8, KIND>::bytes, kKeyCap * (sizeof(R) + 2 * sizeof(int32_t))));
 }
 
 // One launch for every world: each workgroup takes the world order[blockIdx]
 // (larger worlds first) and runs the variant for its contact count.
 template <typename R, int KIND>
 __global__ void __launch_bounds__(64) k_resolve(RS<R> a) {
-  __shared__ __align__(16) char lds[resolve_lds_bytes<R>()];
+  __shared__ __align__(16) char lds[resolve_lds_bytes<R, KIND>()];
   const int lane = threadIdx.x;
   const int w = a.order ? a.order[blockIdx.x] : int(blockIdx.x);
   if (w >= a.n_worlds) return;
@@ -1139,8 +1198,12 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   a.polish_mult = L.polish_mult;
   if ((a.mode & RM_POLISH) && (!a.world_mask || !a.delta || !a.vel0)) return hipErrorInvalidValue;
   if ((a.mode & RM_LABEL) && !a.labels) return hipErrorInvalidValue;
+  // per-contact materials: the plain resolver only
+  if (a.c.fric && (!a.c.rest || (a.mode & (RM_POLISH | RM_LABEL)))) return hipErrorInvalidValue;
   if (a.n_worlds <= 0) return hipSuccess;
-  if (a.mode & RM_POLISH)
+  if (a.c.fric)
+    k_resolve<R, KIND_MAT><<<a.n_worlds, 64, 0, st>>>(a);
+  else if (a.mode & RM_POLISH)
     k_resolve<R, KIND_POLISH><<<a.n_worlds, 64, 0, st>>>(a);
   else if (a.mode & RM_LABEL)
     k_resolve<R, KIND_LABEL><<<a.n_worlds, 64, 0, st>>>(a);

@@ -116,12 +116,14 @@ class World:
         out = {
             "body0": np.empty(n, np.int32), "body1": np.empty(n, np.int32),
             "point": np.empty((n, 3)), "normal": np.empty((n, 3)), "pen": np.empty(n),
+            "friction": np.empty(n), "restitution": np.empty(n),
         }
         if internals:
             out.update({"c2w": np.empty((n, 9)), "rel": np.empty((n, 6)), "cvel": np.empty((n, 3)),
                         "ddv": np.empty(n)})
         v = N.ContactView(*[out[k].ctypes.data if k in out else None for k in
-                            ("body0", "body1", "point", "normal", "pen", "c2w", "rel", "cvel", "ddv")])
+                            ("body0", "body1", "point", "normal", "pen", "c2w", "rel", "cvel", "ddv",
+                             "friction", "restitution")])
         self._chk(self._lib.cbz_get_contacts(self._h, int(world), ctypes.byref(v)))
         return out
 
@@ -131,6 +133,27 @@ class World:
                 np.ascontiguousarray(point, np.float64).reshape(-1),
                 np.ascontiguousarray(normal, np.float64).reshape(-1), np.ascontiguousarray(pen, np.float64)]
         self._chk(self._lib.cbz_set_contacts(self._h, _ptr(counts), *[_ptr(a) for a in arrs]))
+
+    def set_materials(self, on=True, frictionless_two_body=False):
+        """Per-body, per-plane and per-contact friction and restitution
+        (cbz_set_materials); off, every contact carries 0.9 and 0.1 as in the
+        reference.  Drops the current contacts."""
+        flags = (N.MATERIALS if on else 0) | (N.MATERIALS_FRICTIONLESS_TWO_BODY if frictionless_two_body else 0)
+        self._chk(self._lib.cbz_set_materials(self._h, flags))
+
+    def set_plane_materials(self, friction=None, restitution=None):
+        f = None if friction is None else np.ascontiguousarray(friction, np.float64)
+        e = None if restitution is None else np.ascontiguousarray(restitution, np.float64)
+        self._chk(self._lib.cbz_set_plane_materials(self._h, None if f is None else _ptr(f),
+                                                    None if e is None else _ptr(e)))
+
+    def set_contact_materials(self, friction=None, restitution=None):
+        """Friction / restitution of every current contact (all worlds
+        concatenated in emission order)."""
+        f = None if friction is None else np.ascontiguousarray(friction, np.float64)
+        e = None if restitution is None else np.ascontiguousarray(restitution, np.float64)
+        self._chk(self._lib.cbz_set_contact_materials(self._h, None if f is None else _ptr(f),
+                                                      None if e is None else _ptr(e)))
 
     def resolve(self, dt, iter_per_contact=4, iter_fixed=0):
         self._chk(self._lib.cbz_resolve(self._h, float(dt), int(iter_per_contact), int(iter_fixed)))

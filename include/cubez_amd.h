@@ -69,7 +69,9 @@ enum cbz_field {
     CBZ_LIN_DAMP = 15,   /* 1 double LinearDamping */
     CBZ_ANG_DAMP = 16,   /* 1 double AngularDamping */
     CBZ_FLAGS = 17,      /* 1 uint8  CBZ_FLAG_* */
-    CBZ_NUM_FIELDS = 18
+    CBZ_FRICTION = 18,   /* 1 double contact friction of the body's collider (default 0.9; see cbz_set_materials) */
+    CBZ_RESTITUTION = 19,/* 1 double contact restitution of the body's collider (default 0.1) */
+    CBZ_NUM_FIELDS = 20
 };
 
 typedef struct cbz_config {
@@ -91,6 +93,8 @@ typedef struct cbz_contact_view {
     double *rel;     /* [n*6] relativeContactPosition[0], [1] */
     double *cvel;    /* [n*3] contactVelocity */
     double *ddv;     /* [n]   desiredDeltaVelocity */
+    double *friction;    /* [n] Friction (0.9 on every contact unless materials are on) */
+    double *restitution; /* [n] Restitution (0.1 unless materials are on) */
 } cbz_contact_view;
 
 typedef struct cbz_context cbz_ctx;
@@ -150,6 +154,39 @@ int cbz_get_contacts(cbz_ctx *ctx, int32_t world, cbz_contact_view *out);
  * appends its own *Contact values before ResolveContacts. */
 int cbz_set_contacts(cbz_ctx *ctx, const int64_t *counts, const int32_t *body0, const int32_t *body1,
                      const double *point, const double *normal, const double *pen);
+/* --- contact materials ------------------------------------------------------
+ * The reference gives every contact Friction 0.9 and Restitution 0.1
+ * (colliders.go:201-202, 248-249, 360-361, 437-438, 511-512, 704-705); its
+ * README lists per-contact values as a missing feature (README.md:89).
+ * With CBZ_MATERIALS on:
+ *  - each body (fields CBZ_FRICTION, CBZ_RESTITUTION) and each plane
+ *    (cbz_set_plane_materials) has a friction and a restitution, 0.9 and 0.1
+ *    by default, and every contact the narrowphase writes carries the means
+ *    of its two sides' values (so the defaults reproduce the reference
+ *    exactly);
+ *  - cbz_set_contact_materials overrides the values of the current contacts,
+ *    as a Go caller assigns Contact.Friction / Contact.Restitution
+ *    (contact.go:21-25) between CheckForCollisions and ResolveContacts;
+ *  - ResolveContacts uses each contact's values, and a contact of friction 0
+ *    takes calculateFrictionlessImpulse (contact.go:459-461, :498-531) with
+ *    the reference's own behaviour: the test at :512 is inverted, so a
+ *    two-body contact's impulse leaves out its second body's terms (a
+ *    half-space contact, on which Go would dereference nil, gets the
+ *    one-body impulse).  CBZ_MATERIALS_FRICTIONLESS_TWO_BODY adds the second
+ *    body's terms.
+ * Changing the flags drops the current contacts (run a narrowphase again).
+ * The learned resolver (cbz_sur_*) refuses contexts with materials on. */
+#define CBZ_MATERIALS 1
+#define CBZ_MATERIALS_FRICTIONLESS_TWO_BODY 2
+int cbz_set_materials(cbz_ctx *ctx, int32_t flags);
+/* friction[n_planes] (>= 0) and restitution[n_planes] of the planes; either
+ * may be NULL */
+int cbz_set_plane_materials(cbz_ctx *ctx, const double *friction, const double *restitution);
+/* friction and restitution of every current contact, all worlds
+ * concatenated in emission order ([total] each; either may be NULL);
+ * materials must be on */
+int cbz_set_contact_materials(cbz_ctx *ctx, const double *friction, const double *restitution);
+
 /* ResolveContacts(maxIterations, contacts, dt) (contact.go:208-222) on every
  * world's list, with maxIterations = iter_per_contact * C_w when iter_fixed
  * is 0 (the examples pass 8*len(contacts), examples/cubedrop.go:73), else

@@ -998,17 +998,28 @@ static void adjust_positions(int64_t maxIterations, Contact *contacts, int64_t n
     if (cnt) cnt->pos_iters += iterationsUsed;
 }
 
-/* calculateFrictionlessImpulse, contact.go:498-531.  Unreachable while every
- * contact carries Friction 0.9 (colliders.go:201 etc.).  The reference's test
- * at :512 is inverted (Bodies[1] == nil) and would dereference nil; the
- * oracle counts that as a panic and treats the missing body as zero. */
-static Vector3 frictionless_impulse(Contact *c, const Matrix3 iit[2], cbzo_counters *cnt) {
+/* calculateFrictionlessImpulse, contact.go:498-531.  Unreachable in the
+ * reference while every contact carries Friction 0.9 (colliders.go:201 etc.);
+ * reached here by contacts whose friction a caller set to 0.  The
+ * reference's test at :512 is inverted (Bodies[1] == nil): a two-body
+ * contact never adds its second body's terms, and a half-space contact would
+ * dereference nil (the oracle counts that as a panic and treats the missing
+ * body as zero).  With CBZO_FRICTIONLESS_TWO_BODY the second body's terms
+ * are added when it exists (the evident intent of :512-524); this option has
+ * no counterpart in the reference. */
+static Vector3 frictionless_impulse(Contact *c, const Matrix3 iit[2], int32_t flags, cbzo_counters *cnt) {
     Vector3 dvw = v3_cross(&c->relativeContactPosition[0], &c->ContactNormal);
     dvw = m3_mul_vec(&iit[0], &dvw);
     dvw = v3_cross(&dvw, &c->relativeContactPosition[0]);
     Real deltaVelocity = v3_dot(&dvw, &c->ContactNormal);
     deltaVelocity += c->Bodies[0]->inverseMass;
-    if (c->Bodies[1] == NULL) {
+    if ((flags & CBZO_FRICTIONLESS_TWO_BODY) && c->Bodies[1] != NULL) {
+        dvw = v3_cross(&c->relativeContactPosition[1], &c->ContactNormal);
+        dvw = m3_mul_vec(&iit[1], &dvw);
+        dvw = v3_cross(&dvw, &c->relativeContactPosition[1]);
+        deltaVelocity += v3_dot(&dvw, &c->ContactNormal);
+        deltaVelocity += c->Bodies[1]->inverseMass;
+    } else if (c->Bodies[1] == NULL) {
         if (cnt) cnt->panics++;
         dvw = v3_cross(&c->relativeContactPosition[1], &c->ContactNormal);
         dvw = m3_mul_vec(&iit[1], &dvw);
@@ -1066,7 +1077,7 @@ static Vector3 friction_impulse(Contact *c, const Matrix3 iit[2]) {
 
 /* applyVelocityChange, contact.go:448-494 */
 static void apply_velocity_change(Contact *c, Vector3 velocityChange[2], Vector3 rotationChange[2],
-                                  cbzo_counters *cnt) {
+                                  int32_t flags, cbzo_counters *cnt) {
     Matrix3 iit[2];
     memset(iit, 0, sizeof iit);
     memset(velocityChange, 0, 2 * sizeof(Vector3));
@@ -1075,7 +1086,7 @@ static void apply_velocity_change(Contact *c, Vector3 velocityChange[2], Vector3
     if (c->Bodies[1] != NULL) iit[1] = c->Bodies[1]->inverseInertiaTensorWorld;
     Vector3 impulseContact;
     if (c->Friction == 0.0)
-        impulseContact = frictionless_impulse(c, iit, cnt);
+        impulseContact = frictionless_impulse(c, iit, flags, cnt);
     else
         impulseContact = friction_impulse(c, iit);
     Vector3 impulse = m3_mul_vec(&c->contactToWorld, &impulseContact);
@@ -1097,7 +1108,7 @@ static void apply_velocity_change(Contact *c, Vector3 velocityChange[2], Vector3
 
 /* adjustVelocities, contact.go:390-445 */
 static void adjust_velocities(int64_t maxIterations, Contact *contacts, int64_t n, Real duration,
-                              cbzo_counters *cnt) {
+                              int32_t flags, cbzo_counters *cnt) {
     int64_t iterationsUsed = 0;
     while (iterationsUsed < maxIterations) {
         Real max = velocityEpsilon;
@@ -1112,7 +1123,7 @@ static void adjust_velocities(int64_t maxIterations, Contact *contacts, int64_t 
         Contact *contact = &contacts[index];
         match_awake_state(contact);
         Vector3 velocityChange[2], rotationChange[2];
-        apply_velocity_change(contact, velocityChange, rotationChange, cnt);
+        apply_velocity_change(contact, velocityChange, rotationChange, flags, cnt);
         for (int64_t i = 0; i < n; i++) {
             Contact *c2 = &contacts[i];
             for (int b = 0; b < 2; b++) {
@@ -1138,11 +1149,11 @@ static void adjust_velocities(int64_t maxIterations, Contact *contacts, int64_t 
 
 /* ResolveContacts, contact.go:208-222 */
 static void resolve_contacts(int64_t maxIterations, Contact *contacts, int64_t n, Real duration,
-                             cbzo_counters *cnt) {
+                             int32_t flags, cbzo_counters *cnt) {
     if (duration <= 0.0 || contacts == NULL || n == 0) return;
     for (int64_t i = 0; i < n; i++) calculate_internals(&contacts[i], duration); /* prepareContacts :225 */
     adjust_positions(maxIterations, contacts, n, cnt);
-    adjust_velocities(maxIterations, contacts, n, duration, cnt);
+    adjust_velocities(maxIterations, contacts, n, duration, flags, cnt);
 }
 
 /* ===================================================================== */
@@ -1338,11 +1349,16 @@ int64_t cbzo_narrowphase_pairs(cbzo_scene *s, int64_t npairs, const int32_t *one
 }
 
 void cbzo_resolve(cbzo_scene *s, cbzo_contact *c, int64_t nc, int64_t max_iter, double dt, cbzo_counters *cnt) {
+    cbzo_resolve_ex(s, c, nc, max_iter, dt, 0, cnt);
+}
+
+void cbzo_resolve_ex(cbzo_scene *s, cbzo_contact *c, int64_t nc, int64_t max_iter, double dt, int32_t flags,
+                     cbzo_counters *cnt) {
     World w;
     world_load(&w, s);
     Contact *cs = (Contact *)calloc((size_t)(nc > 0 ? nc : 1), sizeof(Contact));
     for (int64_t k = 0; k < nc; k++) record_to_contact(&w, &c[k], &cs[k]);
-    resolve_contacts(max_iter, cs, nc, dt, cnt);
+    resolve_contacts(max_iter, cs, nc, dt, flags, cnt);
     for (int64_t k = 0; k < nc; k++) contact_to_record(&w, &cs[k], &c[k]);
     world_store(&w, s);
     free(cs);
@@ -1377,7 +1393,7 @@ int cbzo_step(cbzo_scene *s, double dt, double sleep_bias, int32_t iter_mult, in
         }
         if (counts) counts[step] = l.n;
         if (cnt) cnt->contacts += l.n;
-        if (l.n > 0) resolve_contacts((int64_t)iter_mult * l.n, l.c, l.n, dt, cnt);
+        if (l.n > 0) resolve_contacts((int64_t)iter_mult * l.n, l.c, l.n, dt, 0, cnt);
     }
     world_store(&w, s);
     free(l.c);

@@ -96,6 +96,12 @@ int64_t cbzo_narrowphase_pairs(cbzo_scene *s, int64_t npairs, const int32_t *one
                                cbzo_counters *cnt);
 void cbzo_resolve(cbzo_scene *s, cbzo_contact *c, int64_t nc, int64_t max_iter, double dt,
                   cbzo_counters *cnt);
+/* cbzo_resolve with options: CBZO_FRICTIONLESS_TWO_BODY adds the second
+ * body's terms to the frictionless impulse of a two-body contact (the
+ * reference omits them, contact.go:512). */
+#define CBZO_FRICTIONLESS_TWO_BODY 1
+void cbzo_resolve_ex(cbzo_scene *s, cbzo_contact *c, int64_t nc, int64_t max_iter, double dt, int32_t flags,
+                     cbzo_counters *cnt);
 /* n_steps of: integrate, all-pairs narrowphase, ResolveContacts(mult*C).
  * counts[k] receives the contact count of step k (may be NULL). Returns 0, or
  * -1 if a step exceeded the scratch contact capacity. */

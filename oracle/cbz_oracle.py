@@ -25,6 +25,10 @@ CONTACT_DTYPE = np.dtype([
     ("rel", np.float64, 6), ("cvel", np.float64, 3), ("ddv", np.float64),
 ], align=True)
 
+# cbzo_resolve_ex option: the frictionless impulse of a two-body contact
+# includes its second body (the reference omits it, contact.go:512)
+FRICTIONLESS_TWO_BODY = 1
+
 COUNTER_FIELDS = ("pairs", "plane_checks", "sat_axes", "sat_skipped", "contacts", "pos_iters",
                   "vel_iters", "integrated", "panics")
 
@@ -67,6 +71,7 @@ def load():
         L.cbzo_narrowphase_pairs.argtypes = [sp, i64, P, P, P, i64, cp]
         L.cbzo_narrowphase_pairs.restype = i64
         L.cbzo_resolve.argtypes = [sp, P, i64, i64, f64, cp]
+        L.cbzo_resolve_ex.argtypes = [sp, P, i64, i64, f64, i32, cp]
         L.cbzo_step.argtypes = [sp, f64, f64, i32, i32, P, cp]
         L.cbzo_step.restype = ctypes.c_int
         L.cbzo_real_equal.argtypes = [f64, f64]
@@ -168,9 +173,12 @@ class OracleWorld:
             return self.narrowphase_pairs(one, two, cap=m)
         return out[:m].copy()
 
-    def resolve(self, contacts, dt, max_iter):
+    def resolve(self, contacts, dt, max_iter, flags=0):
+        """ResolveContacts on the given records (their friction and
+        restitution included); flags: FRICTIONLESS_TWO_BODY."""
         c = np.ascontiguousarray(contacts, CONTACT_DTYPE).copy()
-        self.L.cbzo_resolve(ctypes.byref(self.s), _p(c), len(c), int(max_iter), dt, ctypes.byref(self.cnt))
+        self.L.cbzo_resolve_ex(ctypes.byref(self.s), _p(c), len(c), int(max_iter), dt, int(flags),
+                               ctypes.byref(self.cnt))
         return c
 
     def step(self, dt, n_steps, iter_mult):

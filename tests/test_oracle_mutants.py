@@ -31,6 +31,10 @@ MUTANTS = {
     "relative_velocity_operands": (
         "Vector3 velocity = v3_cross(&body->Rotation, &c->relativeContactPosition[bodyIndex]);",
         "Vector3 velocity = v3_cross(&c->relativeContactPosition[bodyIndex], &body->Rotation);"),
+    "frictionless_linear_term": ("deltaVelocity += c->Bodies[0]->inverseMass;", "deltaVelocity += 0.0;"),
+    "frictionless_angular_operands": ("dvw = v3_cross(&dvw, &c->relativeContactPosition[0]);",
+                                      "dvw = v3_cross(&c->relativeContactPosition[0], &dvw);"),
+    "friction_cone_coefficient": ("ic.v[2] *= c->Friction * ic.v[0];", "ic.v[2] *= ic.v[0];"),
 }
 
 
@@ -46,6 +50,7 @@ def test_oracle_checks_catch_planted_mistake(name, tmp_path):
     env = dict(os.environ, CBZ_ORACLE_LIB=str(lib))
     r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider", "-m", "not gpu",
                         os.path.join(ROOT, "tests", "test_oracle_math.py"),
-                        os.path.join(ROOT, "tests", "test_oracle_physics.py")],
+                        os.path.join(ROOT, "tests", "test_oracle_physics.py"),
+                        os.path.join(ROOT, "tests", "test_oracle_materials.py")],
                        cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
     assert r.returncode != 0, f"mutant {name} passed every oracle check:\n{r.stdout[-2000:]}"
