@@ -295,7 +295,7 @@ def main():
         tfl = flop * sur["contacts_total"] / (sur["network_ms_total"] / 1000.0) / 1e12 if sur["network_ms_total"] else 0
         out["mlp"] = {"flop_per_contact": flop, "ms_per_step": sur["resolver_ms_per_step"]["network"],
                       "achieved_tflops": tfl, "peak_tflops": 2500.0, "mfma_frac": tfl / 2500.0,
-                      "bound": "mfma", "kernel": "k_gemm (4 layers, forward)"}
+                      "bound": "mfma", "kernel": "k_mlp_forward (fused 4-layer forward)"}
         out["zero_prediction"] = {k: zero[k] for k in ("value", "ms_per_step", "fallback_rate")}
     else:
         out["contacts_per_step"] = exact32["contacts_per_step"]

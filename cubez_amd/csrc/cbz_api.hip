@@ -2022,3 +2022,209 @@ int cbz_step_sur(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contac
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// State checkpoints: header, then the payload (little-endian).
+//   header (64 bytes): magic "CBZSTAT1", u32 version, u32 header bytes,
+//     i32 n_worlds, i32 n_planes, i64 n_bodies, i32 material flags,
+//     i32 field count, i32 precision of the saving context, i32 0,
+//     u64 payload bytes, u64 FNV-1a hash of the payload
+//   payload: i32 body_off[n_worlds + 1], i32 plane_off[n_worlds + 1]
+//     (zero-padded to 8 bytes), f64 plane normal [n_planes][3], offset,
+//     friction, restitution [n_planes]; then per field: i32 id, i32 width,
+//     i32 kind (0 f64, 1 i32, 2 u8), i32 0, i64 data bytes, data in the host
+//     layout of cbz_get_field (zero-padded to 8 bytes)
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr char kCkptMagic[8] = {'C', 'B', 'Z', 'S', 'T', 'A', 'T', '1'};
+constexpr uint32_t kCkptVersion = 1;
+constexpr size_t kCkptHeader = 64;
+
+struct CkptHeader {
+  char magic[8];
+  uint32_t version;
+  uint32_t header_bytes;
+  int32_t n_worlds;
+  int32_t n_planes;
+  int64_t n_bodies;
+  int32_t mat_flags;
+  int32_t n_fields;
+  int32_t precision;
+  int32_t zero;
+  uint64_t payload_bytes;
+  uint64_t hash;
+};
+static_assert(sizeof(CkptHeader) == kCkptHeader, "checkpoint header layout");
+
+size_t pad8(size_t n) { return (n + 7) & ~size_t(7); }
+
+uint64_t fnv1a(const uint8_t* p, size_t n) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= p[i];
+    h *= 0x100000001b3ull;
+  }
+  return h;
+}
+
+size_t field_bytes(int f, int64_t nb) {
+  const FieldInfo& fi = kFields[f];
+  const size_t el = (fi.kind == 1) ? sizeof(int32_t) : (fi.kind == 2) ? 1 : sizeof(double);
+  return el * (size_t)fi.width * (size_t)nb;
+}
+
+int field_kind(int f) { return kFields[f].kind == 1 ? 1 : kFields[f].kind == 2 ? 2 : 0; }
+
+size_t ckpt_payload_bytes(const cbz_ctx* c) {
+  size_t n = pad8(2 * sizeof(int32_t) * (size_t)(c->nw + 1));
+  n += sizeof(double) * 6 * (size_t)c->np;
+  for (int f = 0; f < CBZ_NUM_FIELDS; ++f) n += 24 + pad8(field_bytes(f, c->nb));
+  return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cbz_checkpoint_size(cbz_ctx* c, int64_t* bytes) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (!bytes) return fail(c, CBZ_ERR_ARG, "checkpoint_size: null output");
+  *bytes = (int64_t)(kCkptHeader + ckpt_payload_bytes(c));
+  return CBZ_OK;
+}
+
+int cbz_checkpoint_save(cbz_ctx* c, void* buf, int64_t bytes) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  const size_t payload = ckpt_payload_bytes(c);
+  if (!buf || bytes < (int64_t)(kCkptHeader + payload))
+    return fail(c, CBZ_ERR_ARG, "checkpoint_save: buffer smaller than cbz_checkpoint_size");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  uint8_t* out = (uint8_t*)buf;
+  std::memset(out, 0, kCkptHeader + payload);
+  uint8_t* p = out + kCkptHeader;
+  std::memcpy(p, c->body_off.data(), sizeof(int32_t) * (c->nw + 1));
+  std::memcpy(p + sizeof(int32_t) * (c->nw + 1), c->plane_off.data(), sizeof(int32_t) * (c->nw + 1));
+  p += pad8(2 * sizeof(int32_t) * (size_t)(c->nw + 1));
+  const int64_t np = c->np;
+  if (np > 0) {
+    std::vector<double> t((size_t)np * 3);
+    if ((rc = download_real(c, c->plane_normal, np, 0, np, 3, t.data()))) return rc;
+    std::memcpy(p, t.data(), sizeof(double) * 3 * np);
+    p += sizeof(double) * 3 * np;
+    void* const one[3] = {c->plane_offset, c->plane_fric, c->plane_rest};
+    for (void* src : one) {
+      if ((rc = download_real(c, src, np, 0, np, 1, t.data()))) return rc;
+      std::memcpy(p, t.data(), sizeof(double) * np);
+      p += sizeof(double) * np;
+    }
+  }
+  std::vector<uint8_t> tmp;
+  for (int f = 0; f < CBZ_NUM_FIELDS; ++f) {
+    const size_t fb = field_bytes(f, c->nb);
+    const int32_t hdr[4] = {f, kFields[f].width, field_kind(f), 0};
+    const int64_t len = (int64_t)fb;
+    std::memcpy(p, hdr, sizeof hdr);
+    std::memcpy(p + 16, &len, sizeof len);
+    p += 24;
+    if (fb > 0) {
+      tmp.resize(fb);
+      if ((rc = cbz_get_field(c, f, tmp.data()))) return rc;
+      std::memcpy(p, tmp.data(), fb);
+    }
+    p += pad8(fb);
+  }
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, sizeof kCkptMagic);
+  h.version = kCkptVersion;
+  h.header_bytes = (uint32_t)kCkptHeader;
+  h.n_worlds = c->nw;
+  h.n_planes = c->np;
+  h.n_bodies = c->nb;
+  h.mat_flags = c->mat_flags;
+  h.n_fields = CBZ_NUM_FIELDS;
+  h.precision = c->precision;
+  h.payload_bytes = payload;
+  h.hash = fnv1a(out + kCkptHeader, payload);
+  std::memcpy(out, &h, sizeof h);
+  return CBZ_OK;
+}
+
+int cbz_checkpoint_load(cbz_ctx* c, const void* buf, int64_t bytes) {
+  if (!c) return CBZ_ERR_ARG;
+  if (!buf || bytes < (int64_t)kCkptHeader) return fail(c, CBZ_ERR_ARG, "checkpoint_load: truncated header");
+  const uint8_t* in = (const uint8_t*)buf;
+  CkptHeader h;
+  std::memcpy(&h, in, sizeof h);
+  if (std::memcmp(h.magic, kCkptMagic, sizeof kCkptMagic) != 0)
+    return fail(c, CBZ_ERR_ARG, "checkpoint_load: not a cubez_amd state checkpoint");
+  if (h.version != kCkptVersion || h.header_bytes != kCkptHeader)
+    return fail(c, CBZ_ERR_ARG, "checkpoint_load: unsupported version " + std::to_string(h.version));
+  if (h.n_worlds <= 0 || h.n_planes < 0 || h.n_bodies < 0 || h.n_fields != CBZ_NUM_FIELDS ||
+      bytes - (int64_t)kCkptHeader < (int64_t)h.payload_bytes)
+    return fail(c, CBZ_ERR_ARG, "checkpoint_load: truncated or inconsistent header");
+  const uint8_t* p = in + kCkptHeader;
+  if (fnv1a(p, h.payload_bytes) != h.hash) return fail(c, CBZ_ERR_ARG, "checkpoint_load: payload hash mismatch");
+  const int32_t nw = h.n_worlds;
+  const size_t off_bytes = pad8(2 * sizeof(int32_t) * (size_t)(nw + 1));
+  if (h.payload_bytes < off_bytes + sizeof(double) * 6 * (size_t)h.n_planes)
+    return fail(c, CBZ_ERR_ARG, "checkpoint_load: payload too short");
+  std::vector<int32_t> bo((size_t)nw + 1), po((size_t)nw + 1);
+  std::memcpy(bo.data(), p, sizeof(int32_t) * (nw + 1));
+  std::memcpy(po.data(), p + sizeof(int32_t) * (nw + 1), sizeof(int32_t) * (nw + 1));
+  if (bo[nw] != h.n_bodies || po[nw] != h.n_planes)
+    return fail(c, CBZ_ERR_ARG, "checkpoint_load: offsets disagree with the header");
+  const uint8_t* q = p + off_bytes;
+  const int64_t np = h.n_planes;
+  std::vector<double> pn((size_t)np * 3), pofs((size_t)np), pf((size_t)np), pr((size_t)np);
+  std::memcpy(pn.data(), q, sizeof(double) * 3 * np);
+  q += sizeof(double) * 3 * np;
+  std::memcpy(pofs.data(), q, sizeof(double) * np);
+  q += sizeof(double) * np;
+  std::memcpy(pf.data(), q, sizeof(double) * np);
+  q += sizeof(double) * np;
+  std::memcpy(pr.data(), q, sizeof(double) * np);
+  q += sizeof(double) * np;
+  // every field record must be present and whole before the context changes
+  const uint8_t* end = p + h.payload_bytes;
+  const uint8_t* fields[CBZ_NUM_FIELDS] = {};
+  for (int f = 0; f < CBZ_NUM_FIELDS; ++f) {
+    if (end - q < 24) return fail(c, CBZ_ERR_ARG, "checkpoint_load: truncated field record");
+    int32_t hdr[4];
+    int64_t len;
+    std::memcpy(hdr, q, sizeof hdr);
+    std::memcpy(&len, q + 16, sizeof len);
+    if (hdr[0] != f || hdr[1] != kFields[f].width || hdr[2] != field_kind(f) ||
+        len != (int64_t)field_bytes(f, h.n_bodies) || end - (q + 24) < (ptrdiff_t)pad8((size_t)len))
+      return fail(c, CBZ_ERR_ARG, "checkpoint_load: field record " + std::to_string(f) + " does not match");
+    fields[f] = q + 24;
+    q += 24 + pad8((size_t)len);
+  }
+  for (int64_t i = 0; i < h.n_bodies; ++i) {
+    int32_t sh;
+    std::memcpy(&sh, fields[CBZ_SHAPE] + 4 * i, sizeof sh);
+    if (sh != CBZ_CUBE && sh != CBZ_SPHERE) return fail(c, CBZ_ERR_ARG, "checkpoint_load: unknown shape");
+  }
+  if (h.mat_flags & ~(CBZ_MATERIALS | CBZ_MATERIALS_FRICTIONLESS_TWO_BODY))
+    return fail(c, CBZ_ERR_ARG, "checkpoint_load: unknown material flags");
+  int rc = cbz_set_scene(c, nw, bo.data(), po.data(), pn.data(), pofs.data());
+  if (rc) return rc;
+  if ((rc = cbz_set_materials(c, h.mat_flags))) return rc;
+  if ((rc = cbz_set_plane_materials(c, pf.data(), pr.data()))) return rc;
+  // OFFSET before FLAGS: the flags upload keeps the identity-offset bit the
+  // offset upload derives
+  std::vector<int> order;
+  for (int f = 0; f < CBZ_NUM_FIELDS; ++f)
+    if (f != CBZ_FLAGS) order.push_back(f);
+  order.push_back(CBZ_FLAGS);
+  for (int f : order) {
+    if (h.n_bodies == 0) break;
+    if ((rc = cbz_set_field(c, f, fields[f]))) return rc;
+  }
+  return CBZ_OK;
+}
+
+}  // extern "C"

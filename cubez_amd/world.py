@@ -155,6 +155,23 @@ class World:
         self._chk(self._lib.cbz_set_contact_materials(self._h, None if f is None else _ptr(f),
                                                       None if e is None else _ptr(e)))
 
+    # -- checkpoints ------------------------------------------------------------
+    def checkpoint(self):
+        """The whole state as bytes (cbz_checkpoint_save)."""
+        n = ctypes.c_int64(0)
+        self._chk(self._lib.cbz_checkpoint_size(self._h, ctypes.byref(n)))
+        buf = np.zeros(n.value, np.uint8)
+        self._chk(self._lib.cbz_checkpoint_save(self._h, _ptr(buf), n.value))
+        return buf.tobytes()
+
+    def restore(self, data):
+        """Replace the state with a checkpoint (cbz_checkpoint_load)."""
+        buf = np.frombuffer(bytes(data), np.uint8).copy()
+        self._chk(self._lib.cbz_checkpoint_load(self._h, _ptr(buf), len(buf)))
+        self.scene = None
+        self.n_worlds = int(self._lib.cbz_num_worlds(self._h))
+        self.n_bodies = int(self._lib.cbz_num_bodies(self._h))
+
     def resolve(self, dt, iter_per_contact=4, iter_fixed=0):
         self._chk(self._lib.cbz_resolve(self._h, float(dt), int(iter_per_contact), int(iter_fixed)))
 

@@ -187,6 +187,29 @@ int cbz_set_plane_materials(cbz_ctx *ctx, const double *friction, const double *
  * materials must be on */
 int cbz_set_contact_materials(cbz_ctx *ctx, const double *friction, const double *restitution);
 
+/* --- state checkpoints -------------------------------------------------------
+ * The whole simulation state of a context as one byte string, for resume and
+ * for moving a run between machines.  The reference keeps this state in
+ * RigidBody's exported and unexported fields (rigidbody.go:23-101) and has no
+ * serialisation of its own.  Saved: the scene tables (world and plane
+ * offsets, planes with their materials), the material flags and every field
+ * of enum cbz_field in cbz_get_field's host layout, which includes what the
+ * next step reads besides the pose: motion, IsAwake (CBZ_FLAGS),
+ * lastFrameAccelleration and the derived transforms and world inverse
+ * inertia.  Real fields are stored as doubles whatever the precision (an
+ * FP32 state round-trips exactly).  Loading replaces the context's scene
+ * (as cbz_set_scene) with that state, so stepping on gives the same result,
+ * bit for bit, as never having stopped.  Contacts are not saved: the next
+ * step's narrowphase makes them.  The layout (little-endian): a 64-byte
+ * header (magic "CBZSTAT1", version 1, sizes, material flags, field count,
+ * saving precision, payload size and FNV-1a hash), the world and plane
+ * offsets, the plane data, then one record (id, width, kind, byte count,
+ * data) per field; a mismatched magic, version, size or hash is rejected
+ * with CBZ_ERR_ARG before the context changes. */
+int cbz_checkpoint_size(cbz_ctx *ctx, int64_t *bytes);
+int cbz_checkpoint_save(cbz_ctx *ctx, void *buf, int64_t bytes);
+int cbz_checkpoint_load(cbz_ctx *ctx, const void *buf, int64_t bytes);
+
 /* ResolveContacts(maxIterations, contacts, dt) (contact.go:208-222) on every
  * world's list, with maxIterations = iter_per_contact * C_w when iter_fixed
  * is 0 (the examples pass 8*len(contacts), examples/cubedrop.go:73), else
