@@ -130,6 +130,7 @@ struct cbz_context {
     int32_t* nfall = nullptr;
     float* delta = nullptr;  // [6][n_bodies] predicted change per body
     void* vel0 = nullptr;    // [7][n_bodies] velocity, rotation, motion before it
+    void* cvel0 = nullptr;   // [3][cap] contact velocities as prepare left them
     // training buffers
     int batch_cap = 0;
     int64_t* idx = nullptr;
@@ -271,6 +272,7 @@ void free_sur_buffers(cbz_ctx* c) {
   dfree(u.nfall);
   dfree(u.delta);
   dfree(u.vel0);
+  dfree(u.cvel0);
   u.inf_cap = 0;
 }
 
@@ -507,6 +509,7 @@ int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode 
   if (mode & RM_POLISH) {
     L.delta = c->sur.delta;
     L.vel0 = (R*)c->sur.vel0;
+    L.cvel0 = (R*)c->sur.cvel0;
     L.polish_tol = (R)polish_tol;
     L.polish_mult = polish_mult;
   }
@@ -784,6 +787,7 @@ int sur_ensure_inference(cbz_ctx* c, int64_t rows) {
   const size_t nb = (size_t)std::max<int64_t>(c->nb, 1);
   if ((rc = dalloc(c, &u.delta, 6 * nb))) return rc;
   if ((rc = dalloc_bytes(c, &u.vel0, 7 * nb * real_size(c)))) return rc;
+  if ((rc = dalloc_bytes(c, &u.cvel0, 3 * (size_t)std::max<int64_t>(c->cap, 1) * real_size(c)))) return rc;
   u.inf_cap = Mpad;
   return CBZ_OK;
 }

@@ -67,6 +67,7 @@ struct ResolveLaunch {
   float* labels = nullptr;               // [cap][12] velocity-phase changes per contact
   const float* delta = nullptr;          // RM_POLISH: [6][n_bodies] velocity, rotation changes
   R* vel0 = nullptr;                     // RM_POLISH: [7][n_bodies] snapshot (vel, rot, motion)
+  R* cvel0 = nullptr;                    // RM_POLISH: [3][cap] contact velocities after prepare
   R polish_tol = R(0);
   int64_t polish_mult = 0;
 };

@@ -58,6 +58,7 @@ struct RS {
   float* labels;               // [cap][12] accumulated velocity-phase changes (null: off)
   const float* delta;          // RM_POLISH: [6][n_bodies] velocity and rotation changes
   R* vel0;                     // RM_POLISH: [7][n_bodies] vel, rot, motion before them
+  R* cvel0;                    // RM_POLISH: [3][cap] contact velocities as prepare left them
   R polish_tol;
   int64_t polish_mult;
 };
@@ -169,7 +170,7 @@ __device__ __forceinline__ void contact_basis(const V3<R>& n, R* m) {
 
 // calculateInternals, contact.go:59-85, for contact g.
 template <typename R, int KIND>
-__device__ void prepare_contact(const RS<R>& a, int64_t g) {
+__device__ __forceinline__ void prepare_contact(const RS<R>& a, int64_t g) {
   const Bodies<R>& b = a.b;
   const Contacts<R>& c = a.c;
   const int64_t S = b.stride, CS = c.cap;
@@ -239,7 +240,7 @@ __device__ __forceinline__ void rederive(const Bodies<R>& b, int i, const V3<R>&
 
 // applyPositionChange, contact.go:286-386
 template <typename R>
-__device__ void apply_position_change(const RS<R>& a, const int kb[2], const bool aw[2], const V3<R>& n,
+__device__ __forceinline__ void apply_position_change(const RS<R>& a, const int kb[2], const bool aw[2], const V3<R>& n,
                                       const V3<R> rel[2], R penetration, V3<R> lin[2], V3<R> ang[2], int lane) {
   const Bodies<R>& b = a.b;
   const int64_t S = b.stride;
@@ -387,7 +388,7 @@ __device__ __forceinline__ V3<R> friction_impulse(const Bodies<R>& b, const int 
 // applyVelocityChange, contact.go:448-494, for a contact of the given
 // friction (the frictionless branch of :459-461 exists in KIND_MAT only)
 template <typename R, int KIND>
-__device__ void apply_velocity_change(const RS<R>& a, const int kb[2], const V3<R> rel[2], const R* c2w, R ddv,
+__device__ __forceinline__ void apply_velocity_change(const RS<R>& a, const int kb[2], const V3<R> rel[2], const R* c2w, R ddv,
                                       const V3<R>& cvel, V3<R> vc[2], V3<R> rc[2], int lane, R friction) {
   const Bodies<R>& b = a.b;
   const int64_t S = b.stride;
@@ -428,7 +429,7 @@ __device__ void apply_velocity_change(const RS<R>& a, const int kb[2], const V3<
 }
 
 template <typename R, bool LDS, int KIND>
-__device__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key, int32_t* kb0, int32_t* kb1,
+__device__ __forceinline__ void resolve_world(const RS<R>& a, int w, int C, int64_t base, R* key, int32_t* kb0, int32_t* kb1,
                               int lane) {
   const Bodies<R>& b = a.b;
   const Contacts<R>& c = a.c;
@@ -589,7 +590,7 @@ __device__ __forceinline__ bool fits_registers(int C, int n) { return C <= 64 * 
 // HBM-resident path (worlds too large for the register kernel): selection
 // keys and body ids in LDS when the world has at most kKeyCap contacts.
 template <typename R, int KIND>
-__device__ void resolve_hbm(const RS<R>& a, int w, int C, char* lds, int lane) {
+__device__ __forceinline__ void resolve_hbm(const RS<R>& a, int w, int C, char* lds, int lane) {
   if (a.world_iters && lane == 0) {
     if (a.mode & RM_POS) a.world_iters[2 * w] = 0;
     if (a.mode & RM_VEL) a.world_iters[2 * w + 1] = 0;
@@ -701,7 +702,7 @@ struct RegLds {
 };
 
 template <typename R, int SLOTS, int KIND>
-__device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* lds, int lane) {
+__device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* lds, int lane) {
   using L = RegLds<R, SLOTS, KIND>;
   constexpr bool PT_LDS = L::PT_LDS;
   constexpr int CAP = L::CAP;
@@ -907,17 +908,15 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
   }
   const int64_t pos_iters = used;
 
-  // adjustVelocities, contact.go:390-445
-  V3<R> cvl[SLOTS];
+  // adjustVelocities, contact.go:390-445.  The contact velocities stay in
+  // HBM (c.cvel, as prepare stored them): an iteration touches only the
+  // resolved contact's and the one or two slots that share its bodies, and
+  // keeping them out of VGPRs keeps the wide classes from spilling.
 #pragma unroll
   for (int s = 0; s < SLOTS; ++s) {
     const int j = 64 * s + lane;
     if (do_pos && j < C) c.pen[base + j] = key[s];
-    cvl[s] = V3<R>{R(0), R(0), R(0)};
-    if (do_vel && j < C) {  // selection keys and contact velocities as prepare stored them
-      key[s] = c.ddv[base + j];
-      cvl[s] = ld3(c.cvel, CS, base + j);
-    }
+    if (do_vel && j < C) key[s] = c.ddv[base + j];  // selection keys as prepare stored them
   }
   used = 0;
   if (do_vel) {
@@ -927,6 +926,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
       if (polishing) {
       // take the predicted changes: snapshot, then add them to the bodies
       const float* dl = a.delta;
+      for (int j = lane; j < C; j += 64) st3(a.cvel0, CS, base + j, ld3(c.cvel, CS, base + j));
       double ke = 0.0, mass = 0.0;
       for (int i = lane; i < n; i += 64) {
         const int g = b0w + i;
@@ -954,7 +954,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
         const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
         R mj[9];
         basis_from_scale(nrm[s], s_bs[64 * s + lane], mj);
-        V3<R> cvj = cvl[s];
+        V3<R> cvj = ld3(c.cvel, CS, base + 64 * s + lane);
         bool moved = false;
 #pragma unroll
         for (int bb = 0; bb < 2; ++bb) {
@@ -972,7 +972,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
         // a contact neither of whose bodies changed keeps the key prepare
         // gave it, as in the plain loop
         if (moved) {
-          cvl[s] = cvj;
+          st3(c.cvel, CS, base + 64 * s + lane, cvj);
           key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x, lds_restitution<R, KIND>(s_rest, 64 * s + lane));
         }
       }
@@ -1041,7 +1041,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
           const int j = 64 * s + lane;
           if (j < C) {
             key[s] = c.ddv[base + j];
-            cvl[s] = ld3(c.cvel, CS, base + j);
+            st3(c.cvel, CS, base + j, ld3(a.cvel0, CS, base + j));
           }
         }
         __syncthreads();
@@ -1059,7 +1059,6 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
             const V3<R> q = get_pt(s);
             s_bc[0] = nrm[s].x; s_bc[1] = nrm[s].y; s_bc[2] = nrm[s].z;
             s_bc[3] = q.x;      s_bc[4] = q.y;      s_bc[5] = q.z;
-            s_bc[6] = cvl[s].x; s_bc[7] = cvl[s].y; s_bc[8] = cvl[s].z;
             s_bcid = ids[s];
           }
         }
@@ -1067,7 +1066,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
       __syncthreads();
       const V3<R> n{s_bc[0], s_bc[1], s_bc[2]};
       const V3<R> p{s_bc[3], s_bc[4], s_bc[5]};
-      const V3<R> cvk{s_bc[6], s_bc[7], s_bc[8]};
+      const V3<R> cvk = ld3(c.cvel, CS, base + bi);
       const uint32_t kid = s_bcid;
       const int kl[2] = {id_lo(kid), id_hi(kid)};
       const int kb[2] = {b0w + kl[0], kl[1] >= 0 ? b0w + kl[1] : -1};
@@ -1101,7 +1100,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
         if (!any) continue;
         R mj[9];
         basis_from_scale(nrm[s], s_bs[64 * s + lane], mj);
-        V3<R> cvj = cvl[s];
+        V3<R> cvj = ld3(c.cvel, CS, base + 64 * s + lane);
 #pragma unroll
         for (int bb = 0; bb < 2; ++bb) {
           if (jb[bb] < 0) continue;
@@ -1116,7 +1115,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
             }
           }
         }
-        cvl[s] = cvj;
+        st3(c.cvel, CS, base + 64 * s + lane, cvj);
         key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], nrm[s], cvj.x, lds_restitution<R, KIND>(s_rest, 64 * s + lane));
       }
       __syncthreads();
@@ -1126,10 +1125,7 @@ __device__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* 
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
       const int j = 64 * s + lane;
-      if (j < C) {
-        c.ddv[base + j] = key[s];
-        st3(c.cvel, CS, base + j, cvl[s]);
-      }
+      if (j < C) c.ddv[base + j] = key[s];
     }
   }
   for (int i = lane; i < n; i += 64) b.flags[b0w + i] = s_flags[i];
@@ -1149,7 +1145,7 @@ constexpr size_t resolve_lds_bytes() {
 // One launch for every world: each workgroup takes the world order[blockIdx]
 // (larger worlds first) and runs the variant for its contact count.
 template <typename R, int KIND>
-__global__ void __launch_bounds__(64) k_resolve(RS<R> a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) k_resolve(RS<R> a) {
   __shared__ __align__(16) char lds[resolve_lds_bytes<R, KIND>()];
   const int lane = threadIdx.x;
   const int w = a.order ? a.order[blockIdx.x] : int(blockIdx.x);
@@ -1194,9 +1190,10 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   a.labels = L.labels;
   a.delta = L.delta;
   a.vel0 = L.vel0;
+  a.cvel0 = L.cvel0;
   a.polish_tol = L.polish_tol;
   a.polish_mult = L.polish_mult;
-  if ((a.mode & RM_POLISH) && (!a.world_mask || !a.delta || !a.vel0)) return hipErrorInvalidValue;
+  if ((a.mode & RM_POLISH) && (!a.world_mask || !a.delta || !a.vel0 || !a.cvel0)) return hipErrorInvalidValue;
   if ((a.mode & RM_LABEL) && !a.labels) return hipErrorInvalidValue;
   // per-contact materials: the plain resolver only
   if (a.c.fric && (!a.c.rest || (a.mode & (RM_POLISH | RM_LABEL)))) return hipErrorInvalidValue;
