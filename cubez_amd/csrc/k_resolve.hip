@@ -22,7 +22,7 @@
 
 namespace cbz {
 
-constexpr int kKeyCap = 2048;
+constexpr int kKeyCap = 1280;
 
 // Relative energy slack of the surrogate's acceptance test (rounding); the
 // absolute slack is 1/2 tol^2 per kilogram: the kinetic energy of the
@@ -686,16 +686,17 @@ __device__ __forceinline__ int id_hi(uint32_t id) { return (id >> 16) == kNoBody
 
 template <typename R, int SLOTS, int KIND>
 struct RegLds {
-  // the widest variants keep contact points in LDS (in VGPRs they would spill)
-  static constexpr bool PT_LDS = SLOTS >= 12;
+  // the widest variants read contact points from HBM (c.point) where they
+  // need them: in VGPRs they would spill, and in LDS they would cut the
+  // worlds resident per CU
+  static constexpr bool PT_HBM = SLOTS >= 12;
   static constexpr int CAP = 64 * SLOTS;
   static constexpr bool MAT = KIND == KIND_MAT;
   static constexpr size_t pos0 = 0;                                   // R[3][kRegBodies]
   static constexpr size_t lad = pos0 + 3 * kRegBodies * sizeof(R);    // R[3][kRegBodies]
   static constexpr size_t bs = lad + 3 * kRegBodies * sizeof(R);      // double[CAP] basis scale
   static constexpr size_t bc = bs + CAP * sizeof(double);             // R[9] broadcast
-  static constexpr size_t pt = bc + 16 * sizeof(R);                   // R[3][CAP] (PT_LDS)
-  static constexpr size_t flags = pt + (PT_LDS ? 3 * CAP * sizeof(R) : 0);  // uint8[kRegBodies]
+  static constexpr size_t flags = bc + 16 * sizeof(R);                // uint8[kRegBodies]
   static constexpr size_t bcid = flags + kRegBodies;                  // uint32
   static constexpr size_t rest = bcid + 16;                           // R[CAP] restitution (MAT)
   static constexpr size_t bytes = rest + (MAT ? CAP * sizeof(R) : 0);
@@ -704,13 +705,12 @@ struct RegLds {
 template <typename R, int SLOTS, int KIND>
 __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0w, int n, char* lds, int lane) {
   using L = RegLds<R, SLOTS, KIND>;
-  constexpr bool PT_LDS = L::PT_LDS;
+  constexpr bool PT_HBM = L::PT_HBM;
   constexpr int CAP = L::CAP;
   R* s_pos0 = (R*)(lds + L::pos0);
   R* s_lad = (R*)(lds + L::lad);
   double* s_bs = (double*)(lds + L::bs);
   R* s_bc = (R*)(lds + L::bc);
-  R* s_pt = (R*)(lds + L::pt);
   uint8_t* s_flags = (uint8_t*)(lds + L::flags);
   uint32_t& s_bcid = *(uint32_t*)(lds + L::bcid);
   R* s_rest = (R*)(lds + L::rest);  // KIND_MAT: restitution of contact j (world-local)
@@ -750,11 +750,10 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
   R key[SLOTS];
   uint32_t ids[SLOTS];
   V3<R> nrm[SLOTS];
-  V3<R> ptr[PT_LDS ? 1 : SLOTS];
+  V3<R> ptr[PT_HBM ? 1 : SLOTS];
   auto get_pt = [&](int s) -> V3<R> {
-    if constexpr (PT_LDS) {
-      const int j = 64 * s + lane;
-      return V3<R>{s_pt[j], s_pt[CAP + j], s_pt[2 * CAP + j]};
+    if constexpr (PT_HBM) {
+      return ld3(c.point, CS, base + 64 * s + lane);
     } else {
       return ptr[s];
     }
@@ -767,7 +766,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
     key[s] = R(0);
     ids[s] = 0xFFFFFFFFu;
     nrm[s] = V3<R>{R(0), R(0), R(0)};
-    if constexpr (!PT_LDS) ptr[s] = nrm[s];
+    if constexpr (!PT_HBM) ptr[s] = nrm[s];
     if (j < C && !do_pos) {  // velocity phase only: reload what prepare stored
       const int64_t g = base + j;
       const int c0 = c.body0[g], c1 = c.body1[g];
@@ -777,13 +776,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
       if constexpr (KIND == KIND_MAT) s_rest[j] = c.rest[g];
       ids[s] = uint32_t(c0 - b0w) | (uint32_t(c1 >= 0 ? c1 - b0w : int(kNoBody)) << 16);
       nrm[s] = nn;
-      if constexpr (PT_LDS) {
-        s_pt[j] = p.x;
-        s_pt[CAP + j] = p.y;
-        s_pt[2 * CAP + j] = p.z;
-      } else {
-        ptr[s] = p;
-      }
+      if constexpr (!PT_HBM) ptr[s] = p;
     } else if (j < C) {
       const int64_t g = base + j;
       int c0 = c.body0[g], c1 = c.body1[g];
@@ -820,13 +813,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
       key[s] = c.pen[g];
       ids[s] = uint32_t(l0) | (uint32_t(l1 >= 0 ? l1 : int(kNoBody)) << 16);
       nrm[s] = nn;
-      if constexpr (PT_LDS) {
-        s_pt[j] = p.x;
-        s_pt[CAP + j] = p.y;
-        s_pt[2 * CAP + j] = p.z;
-      } else {
-        ptr[s] = p;
-      }
+      if constexpr (!PT_HBM) ptr[s] = p;
     }
   }
   __syncthreads();
@@ -1145,7 +1132,7 @@ constexpr size_t resolve_lds_bytes() {
 // One launch for every world: each workgroup takes the world order[blockIdx]
 // (larger worlds first) and runs the variant for its contact count.
 template <typename R, int KIND>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) k_resolve(RS<R> a) {
+__device__ __forceinline__ void resolve_block(const RS<R>& a) {
   __shared__ __align__(16) char lds[resolve_lds_bytes<R, KIND>()];
   const int lane = threadIdx.x;
   const int w = a.order ? a.order[blockIdx.x] : int(blockIdx.x);
@@ -1167,6 +1154,27 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   } else {
     resolve_reg<R, 16, KIND>(a, w, C, b0w, n, lds, lane);
   }
+}
+
+// The FP64 kernel keeps a full wave's registers (its working set spills
+// heavily below that); its LDS (about 21 KB) lets 4 worlds share a CU.  The
+// FP32 kernel is held to two waves per SIMD: its few spills cost less than
+// the second wave gains.
+template <int KIND>
+__global__ void __launch_bounds__(64) k_resolve_f64(RS<double> a) {
+  resolve_block<double, KIND>(a);
+}
+template <int KIND>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) k_resolve_f32(RS<float> a) {
+  resolve_block<float, KIND>(a);
+}
+
+template <typename R, int KIND>
+void launch_kind(const RS<R>& a, hipStream_t st) {
+  if constexpr (sizeof(R) == 8)
+    k_resolve_f64<KIND><<<a.n_worlds, 64, 0, st>>>(a);
+  else
+    k_resolve_f32<KIND><<<a.n_worlds, 64, 0, st>>>(a);
 }
 
 template <typename R>
@@ -1199,13 +1207,13 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   if (a.c.fric && (!a.c.rest || (a.mode & (RM_POLISH | RM_LABEL)))) return hipErrorInvalidValue;
   if (a.n_worlds <= 0) return hipSuccess;
   if (a.c.fric)
-    k_resolve<R, KIND_MAT><<<a.n_worlds, 64, 0, st>>>(a);
+    launch_kind<R, KIND_MAT>(a, st);
   else if (a.mode & RM_POLISH)
-    k_resolve<R, KIND_POLISH><<<a.n_worlds, 64, 0, st>>>(a);
+    launch_kind<R, KIND_POLISH>(a, st);
   else if (a.mode & RM_LABEL)
-    k_resolve<R, KIND_LABEL><<<a.n_worlds, 64, 0, st>>>(a);
+    launch_kind<R, KIND_LABEL>(a, st);
   else
-    k_resolve<R, KIND_PLAIN><<<a.n_worlds, 64, 0, st>>>(a);
+    launch_kind<R, KIND_PLAIN>(a, st);
   return hipGetLastError();
 }
 
