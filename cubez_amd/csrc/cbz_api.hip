@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "cbz_broad.h"
 #include "cbz_launch.h"
 #include "cbz_mlp.h"
 #include "cbz_types.h"
@@ -86,6 +87,27 @@ struct cbz_context {
   int32_t* l_off = nullptr;
   int32_t* l_one = nullptr;
   int32_t* l_two = nullptr;
+  // sort-and-sweep broadphase (cbz_narrowphase_broad), allocated on first use
+  struct Broad {
+    bool ready = false;
+    int32_t* body_world = nullptr;     // [nb]
+    double* lo = nullptr;              // [2][nb]
+    double* hi = nullptr;              // [2][nb]
+    float* hix = nullptr;              // [nb]
+    uint8_t* wild = nullptr;           // [nb]
+    uint64_t* key_in = nullptr;        // [nb]
+    uint64_t* key_out = nullptr;
+    int32_t* val_in = nullptr;
+    int32_t* val_out = nullptr;
+    int64_t* plane_base = nullptr;     // [nw]
+    int64_t plane_pairs = 0;
+    uint64_t* pkey_in = nullptr;       // [pcap]
+    uint64_t* pkey = nullptr;
+    int64_t pcap = 0;
+    int64_t* pcount = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+  } bp;
   // tile scratch, sized for the larger pair space
   int32_t tiles_cap = 0;
   int32_t* tile_cnt = nullptr;
@@ -297,6 +319,24 @@ void free_scene(cbz_ctx* c) {
   dfree(c->n_hits);
   dfree(c->cand);
   dfree(c->n_cand);
+  {
+    auto& bp = c->bp;
+    dfree(bp.body_world);
+    dfree(bp.lo);
+    dfree(bp.hi);
+    dfree(bp.hix);
+    dfree(bp.wild);
+    dfree(bp.key_in);
+    dfree(bp.key_out);
+    dfree(bp.val_in);
+    dfree(bp.val_out);
+    dfree(bp.plane_base);
+    dfree(bp.pkey_in);
+    dfree(bp.pkey);
+    dfree(bp.pcount);
+    dfree(bp.tmp);
+    bp = cbz_context::Broad{};
+  }
   dfree(c->l_tile_world);
   dfree(c->l_world_tile0);
   dfree(c->l_world_q);
@@ -530,12 +570,166 @@ int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode 
   return CBZ_OK;
 }
 
+// List buffers for n_pairs explicit pairs.
+int ensure_list(cbz_ctx* c, int64_t n_pairs) {
+  int rc;
+  if (n_pairs > c->l_pairs_cap) {
+    if ((rc = dalloc(c, &c->l_one, (size_t)n_pairs))) return rc;
+    if ((rc = dalloc(c, &c->l_two, (size_t)n_pairs))) return rc;
+    c->l_pairs_cap = (int32_t)n_pairs;
+  }
+  if (!c->l_world_tile0) {
+    if ((rc = dalloc(c, &c->l_world_tile0, (size_t)c->nw + 1))) return rc;
+    if ((rc = dalloc(c, &c->l_world_q, (size_t)c->nw))) return rc;
+    if ((rc = dalloc(c, &c->l_off, (size_t)c->nw + 1))) return rc;
+  }
+  return CBZ_OK;
+}
+
+int check_overflow(cbz_ctx* c);
+
+// The explicit-list narrowphase over l_one/l_two (world w's pairs from
+// l_off[w], wq[w] of them): tiles of 64 consecutive pairs per world.
+int run_list_narrowphase(cbz_ctx* c, const std::vector<int32_t>& wq, int64_t* total) {
+  std::vector<int32_t> wt0(c->nw + 1), tw;
+  int64_t nt = 0;
+  for (int32_t w = 0; w < c->nw; ++w) {
+    wt0[w] = (int32_t)nt;
+    nt += (wq[w] + 63) / 64;
+  }
+  wt0[c->nw] = (int32_t)nt;
+  if (nt > INT32_MAX / 64) return fail(c, CBZ_ERR_ARG, "narrowphase: pair list too long");
+  tw.resize((size_t)nt);
+  for (int32_t w = 0; w < c->nw; ++w)
+    for (int32_t t = wt0[w]; t < wt0[w + 1]; ++t) tw[t] = w;
+  int rc;
+  if (nt > c->l_tiles_cap) {
+    if ((rc = dalloc(c, &c->l_tile_world, (size_t)nt))) return rc;
+    c->l_tiles_cap = (int32_t)nt;
+  }
+  if ((rc = ensure_tiles(c, (int32_t)std::max<int64_t>(nt, 1)))) return rc;
+  if (nt > 0) HIPCHK(c, hipMemcpy(c->l_tile_world, tw.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->l_world_tile0, wt0.data(), wt0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->l_world_q, wq.data(), wq.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, launch_fill_i32(c->overflow, 2, 0, c->stream));
+  rc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, (int32_t)nt) : do_narrowphase<float>(c, (int32_t)nt);
+  if (rc) return rc;
+  if ((rc = check_overflow(c))) return rc;
+  if (total) HIPCHK(c, hipMemcpy(total, c->total, sizeof(int64_t), hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+// Broadphase buffers sized for the scene (and pair keys for pcap pairs).
+int bp_ensure(cbz_ctx* c, int64_t pcap) {
+  auto& bp = c->bp;
+  int rc;
+  const size_t nb = (size_t)std::max<int64_t>(c->nb, 1);
+  if (!bp.ready) {
+    std::vector<int32_t> bw(nb, 0);
+    std::vector<int64_t> base((size_t)c->nw, 0);
+    int64_t acc = 0;
+    for (int32_t w = 0; w < c->nw; ++w) {
+      for (int32_t i = c->body_off[w]; i < c->body_off[w + 1]; ++i) bw[i] = w;
+      base[w] = acc;
+      acc += (int64_t)(c->body_off[w + 1] - c->body_off[w]) * (c->plane_off[w + 1] - c->plane_off[w]);
+    }
+    if ((rc = dalloc(c, &bp.body_world, nb))) return rc;
+    if ((rc = dalloc(c, &bp.lo, 2 * nb))) return rc;
+    if ((rc = dalloc(c, &bp.hi, 2 * nb))) return rc;
+    if ((rc = dalloc(c, &bp.hix, nb))) return rc;
+    if ((rc = dalloc(c, &bp.wild, nb))) return rc;
+    if ((rc = dalloc(c, &bp.key_in, nb))) return rc;
+    if ((rc = dalloc(c, &bp.key_out, nb))) return rc;
+    if ((rc = dalloc(c, &bp.val_in, nb))) return rc;
+    if ((rc = dalloc(c, &bp.val_out, nb))) return rc;
+    if ((rc = dalloc(c, &bp.plane_base, (size_t)std::max(c->nw, 1)))) return rc;
+    if ((rc = dalloc(c, &bp.pcount, 1))) return rc;
+    if (c->nb > 0) HIPCHK(c, hipMemcpy(bp.body_world, bw.data(), c->nb * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(bp.plane_base, base.data(), base.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    bp.plane_pairs = acc;
+    bp.ready = true;
+  }
+  if (pcap > bp.pcap) {
+    if ((rc = dalloc(c, &bp.pkey_in, (size_t)pcap))) return rc;
+    if ((rc = dalloc(c, &bp.pkey, (size_t)pcap))) return rc;
+    bp.pcap = pcap;
+  }
+  size_t need = 0;
+  HIPCHK(c, bp_sort_bytes((int32_t)c->nb, bp.pcap, &need));
+  if (need > bp.tmp_bytes) {
+    dfree(bp.tmp);
+    HIPCHK(c, hipMalloc(&bp.tmp, need));
+    bp.tmp_bytes = need;
+  }
+  return CBZ_OK;
+}
+
+template <typename R>
+BroadArgs<R> bp_args(cbz_ctx* c) {
+  auto& bp = c->bp;
+  BroadArgs<R> a;
+  a.nb = (int32_t)c->nb;
+  a.stride = c->nb;
+  a.ctransform = (const R*)c->field[CBZ_CTRANSFORM];
+  a.brad = c->brad;
+  a.body_world = bp.body_world;
+  a.body_off = c->d_body_off;
+  a.plane_off = c->d_plane_off;
+  a.lo = bp.lo;
+  a.hi = bp.hi;
+  a.hix = bp.hix;
+  a.wild = bp.wild;
+  a.key_in = bp.key_in;
+  a.key_out = bp.key_out;
+  a.val_in = bp.val_in;
+  a.val_out = bp.val_out;
+  a.pkey = bp.pkey_in;
+  a.pcap = bp.pcap;
+  a.plane_pairs = bp.plane_pairs;
+  a.pcount = bp.pcount;
+  return a;
+}
+
+// Broadphase into the explicit list: returns the number of listed pairs.
+template <typename R>
+int do_broadphase(cbz_ctx* c, int64_t* n_pairs) {
+  auto& bp = c->bp;
+  int rc;
+  if ((rc = bp_ensure(c, bp.plane_pairs + 16 * c->nb + 1024))) return rc;
+  if ((rc = refresh_brad(c))) return rc;
+  HIPCHK(c, launch_bp_bounds_sort<R>(bp_args<R>(c), bp.tmp, bp.tmp_bytes, c->stream));
+  BroadPlanes pp{(int32_t)c->nb, bp.body_world, c->d_body_off, c->d_plane_off, bp.plane_base, bp.pkey_in};
+  int64_t found = 0;
+  for (int round = 0; round < 2; ++round) {
+    pp.pkey = bp.pkey_in;
+    HIPCHK(c, launch_bp_sweep<R>(bp_args<R>(c), pp, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&found, bp.pcount, sizeof found, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (bp.plane_pairs + found <= bp.pcap) break;
+    if (round == 1) return fail(c, CBZ_ERR_HIP, "broadphase: pair count changed between sweeps");
+    if ((rc = bp_ensure(c, bp.plane_pairs + found + found / 4 + 1024))) return rc;
+  }
+  const int64_t n = bp.plane_pairs + found;
+  if (n > INT32_MAX / 2) return fail(c, CBZ_ERR_CAPACITY, "broadphase: more than 2^30 candidate pairs");
+  if ((rc = ensure_list(c, n))) return rc;
+  BroadList L{n, c->nw, bp.pkey_in, bp.pkey, bp.body_world, c->d_body_off, c->d_plane_off, c->l_one, c->l_two,
+              c->l_off};
+  HIPCHK(c, launch_bp_list(L, bp.tmp, bp.tmp_bytes, c->stream));
+  *n_pairs = n;
+  return CBZ_OK;
+}
+
 int check_overflow(cbz_ctx* c) {
-  int32_t of = 0;
+  int32_t of = 0, nc = 0;
   int64_t tot = 0;
   HIPCHK(c, hipMemcpyAsync(&of, c->overflow + 1, sizeof of, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(&tot, c->total, sizeof tot, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&nc, c->n_cand, sizeof nc, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (of && nc > c->cand_cap)
+    return fail(c, CBZ_ERR_CAPACITY,
+                "narrowphase candidate list exceeded: " + std::to_string(nc) + " candidate pairs for " +
+                    std::to_string(c->cand_cap) + " slots (raise cbz_config.contact_capacity)");
   if (of)
     return fail(c, CBZ_ERR_CAPACITY,
                 "contact capacity exceeded: " + std::to_string(tot) + " contacts for " + std::to_string(c->cap) +
@@ -1029,8 +1223,14 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc_bytes(c, &c->c_real, rs * 28 * (size_t)c->cap))) return rc;
   if ((rc = dalloc(c, &c->hits, (size_t)c->cap))) return rc;
   if ((rc = dalloc(c, &c->n_hits, 1))) return rc;
-  // candidate pairs of the cull: 4 per contact slot (64 per body by default)
-  c->cand_cap = std::min<int64_t>(4 * c->cap, INT32_MAX);
+  // candidate pairs of the cull: 4 per contact slot plus the scene's ordered
+  // pairs up to 32 M (a dense pile keeps many pairs inside the cull's bound)
+  int64_t pairs_total = 0;
+  for (int32_t w = 0; w < n_worlds; ++w) {
+    const int64_t n = body_off[w + 1] - body_off[w], npw = plane_off[w + 1] - plane_off[w];
+    pairs_total += n * (n - 1 + npw);
+  }
+  c->cand_cap = std::min<int64_t>(4 * c->cap + std::min<int64_t>(pairs_total, int64_t(32) << 20), INT32_MAX);
   if ((rc = dalloc(c, &c->cand, (size_t)c->cand_cap))) return rc;
   if ((rc = dalloc(c, &c->n_cand, 1))) return rc;
   if ((rc = dalloc(c, &c->world_cnt, (size_t)n_worlds))) return rc;
@@ -1248,47 +1448,30 @@ int cbz_narrowphase_pairs(cbz_ctx* c, int64_t n_pairs, const int32_t* one, const
     prev = wa;
     wq[wa]++;
   }
-  std::vector<int32_t> wt0(c->nw + 1), tw;
-  int64_t nt = 0;
-  for (int32_t w = 0; w < c->nw; ++w) {
-    loff[w + 1] = loff[w] + wq[w];
-    wt0[w] = (int32_t)nt;
-    nt += (wq[w] + 63) / 64;
-  }
-  wt0[c->nw] = (int32_t)nt;
-  tw.resize((size_t)nt);
-  for (int32_t w = 0; w < c->nw; ++w)
-    for (int32_t t = wt0[w]; t < wt0[w + 1]; ++t) tw[t] = w;
-  if (nt > c->l_tiles_cap) {
-    if ((rc = dalloc(c, &c->l_tile_world, (size_t)nt))) return rc;
-    c->l_tiles_cap = (int32_t)nt;
-  }
-  if (n_pairs > c->l_pairs_cap) {
-    if ((rc = dalloc(c, &c->l_one, (size_t)n_pairs))) return rc;
-    if ((rc = dalloc(c, &c->l_two, (size_t)n_pairs))) return rc;
-    c->l_pairs_cap = (int32_t)n_pairs;
-  }
-  if (!c->l_world_tile0) {
-    if ((rc = dalloc(c, &c->l_world_tile0, (size_t)c->nw + 1))) return rc;
-    if ((rc = dalloc(c, &c->l_world_q, (size_t)c->nw))) return rc;
-    if ((rc = dalloc(c, &c->l_off, (size_t)c->nw + 1))) return rc;
-  }
-  if ((rc = ensure_tiles(c, (int32_t)std::max<int64_t>(nt, 1)))) return rc;
+  for (int32_t w = 0; w < c->nw; ++w) loff[w + 1] = loff[w] + wq[w];
+  if ((rc = ensure_list(c, n_pairs))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (nt > 0) HIPCHK(c, hipMemcpy(c->l_tile_world, tw.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice));
   if (n_pairs > 0) {
     HIPCHK(c, hipMemcpy(c->l_one, one, n_pairs * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->l_two, two, n_pairs * sizeof(int32_t), hipMemcpyHostToDevice));
   }
-  HIPCHK(c, hipMemcpy(c->l_world_tile0, wt0.data(), wt0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(c->l_world_q, wq.data(), wq.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->l_off, loff.data(), loff.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-  HIPCHK(c, launch_fill_i32(c->overflow, 2, 0, c->stream));
-  rc = c->precision == CBZ_F64 ? do_narrowphase<double>(c, (int32_t)nt) : do_narrowphase<float>(c, (int32_t)nt);
+  return run_list_narrowphase(c, wq, total);
+}
+
+int cbz_narrowphase_broad(cbz_ctx* c, int64_t* total, int64_t* pairs_checked) {
+  int rc = need_scene(c);
   if (rc) return rc;
-  if ((rc = check_overflow(c))) return rc;
-  if (total) HIPCHK(c, hipMemcpy(total, c->total, sizeof(int64_t), hipMemcpyDeviceToHost));
-  return CBZ_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  int64_t n = 0;
+  rc = c->precision == CBZ_F64 ? do_broadphase<double>(c, &n) : do_broadphase<float>(c, &n);
+  if (rc) return rc;
+  std::vector<int32_t> loff((size_t)c->nw + 1), wq((size_t)c->nw);
+  HIPCHK(c, hipMemcpyAsync(loff.data(), c->l_off, loff.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int32_t w = 0; w < c->nw; ++w) wq[w] = loff[w + 1] - loff[w];
+  if (pairs_checked) *pairs_checked = n;
+  return run_list_narrowphase(c, wq, total);
 }
 
 int cbz_contact_counts(cbz_ctx* c, int64_t* counts) {

@@ -99,6 +99,14 @@ class World:
         self._chk(self._lib.cbz_narrowphase(self._h, ctypes.byref(t)))
         return t.value
 
+    def narrowphase_broad(self):
+        """The narrowphase behind the sort-and-sweep broadphase: (contacts,
+        pairs checked); the contacts equal narrowphase()'s."""
+        t = ctypes.c_int64(0)
+        n = ctypes.c_int64(0)
+        self._chk(self._lib.cbz_narrowphase_broad(self._h, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
     def narrowphase_pairs(self, one, two):
         one = np.ascontiguousarray(one, np.int32)
         two = np.ascontiguousarray(two, np.int32)

@@ -145,6 +145,19 @@ int cbz_narrowphase(cbz_ctx *ctx, int64_t *total);
  * non-decreasing world order; both operands of a pair in the same world. */
 int cbz_narrowphase_pairs(cbz_ctx *ctx, int64_t n_pairs, const int32_t *one, const int32_t *two,
                           int64_t *total);
+/* cbz_narrowphase with a sort-and-sweep broadphase in front, the first item
+ * of the reference's roadmap (README.md:98-99): per world, the bodies are
+ * sorted by the lower x bound of a conservative box (sqrt(3.003) times the
+ * bounding radius, which covers every pair the narrowphase's own far-pair
+ * test keeps; a body whose state is not finite overlaps everything), swept
+ * along x with y and z tested, and only the overlapping ordered pairs plus
+ * every body-plane pair are checked.  The pairs are sorted into the
+ * all-pairs order of examples/cubedrop.go:47-64, so the contacts are
+ * exactly cbz_narrowphase's, in the same order.  Worth it for worlds of
+ * thousands of bodies, where the all-pairs space grows as n^2.
+ * *pairs_checked receives the number of pairs the narrowphase evaluated
+ * (either output may be NULL). */
+int cbz_narrowphase_broad(cbz_ctx *ctx, int64_t *total, int64_t *pairs_checked);
 /* contact count of every world after the last narrowphase ([n_worlds]) */
 int cbz_contact_counts(cbz_ctx *ctx, int64_t *counts);
 /* all contacts of one world, in emission order */
