@@ -25,6 +25,7 @@
 namespace cbz {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 template <typename R>
@@ -32,7 +33,9 @@ __device__ __forceinline__ V3<R> ld3c(const R* a, int64_t s, int64_t i) {
   return V3<R>{a[i], a[s + i], a[2 * s + i]};
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU with the hardware reciprocal (about 1 ulp): its result is rounded to
+// BF16 for the next layer
+__device__ __forceinline__ float silu(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float silu_grad(float z) {
   const float s = 1.0f / (1.0f + __expf(-z));
   return s * (1.0f + z * (1.0f - s));
@@ -298,23 +301,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 31, h = lane >> 5;
   const int64_t m0 = int64_t(blockIdx.x) * kFuseRows;
+  // The weights are the MFMA's first operand and the rows its second, so
+  // acc[t][u][j] is feature wave*64 + 32u + 8(j/4) + 4h + j%4 of tile row
+  // 32t + r: a lane holds runs of 4 consecutive features of one row, stored
+  // to the LDS tile 8 bytes at a time.
   f32x16 acc[2][2];
   // hidden-layer epilogue: bias, SiLU, BF16 into the LDS tile
   auto store_hidden = [&](const float* bias) {
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      const int col = wave * 64 + 32 * u + r;
-      const float bv = bias[col];
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
+      for (int q = 0; q < 4; ++q) {
+        const int col = wave * 64 + 32 * u + 8 * q + 4 * h;
+        const float4 bv{bias[col], bias[col + 1], bias[col + 2], bias[col + 3]};
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          const int row = 32 * t + 8 * (j >> 2) + 4 * h + (j & 3);
-          H[fswz(row, col)] = __bf16(silu(acc[t][u][j] + bv));
+        for (int t = 0; t < 2; ++t) {
+          const int row = 32 * t + r;
+          bf16x4 v;
+          v[0] = __bf16(silu(acc[t][u][4 * q + 0] + bv.x));
+          v[1] = __bf16(silu(acc[t][u][4 * q + 1] + bv.y));
+          v[2] = __bf16(silu(acc[t][u][4 * q + 2] + bv.z));
+          v[3] = __bf16(silu(acc[t][u][4 * q + 3] + bv.w));
+          *(bf16x4*)(H + fswz(row, col)) = v;
         }
+      }
     }
   };
-  // layer 0: A from the features in HBM
+  // layer 0: rows from the features in HBM
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -332,11 +345,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+      for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[u], a[t], acc[t][u], 0, 0, 0);
   }
   store_hidden(f.bias[0]);
   __syncthreads();
-  // layers 1 and 2: A from the LDS tile, results back into it
+  // layers 1 and 2: rows from the LDS tile, results back into it
   for (int l = 1; l < kLayers - 1; ++l) {
 #pragma unroll
     for (int t = 0; t < 2; ++t)
@@ -353,13 +366,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
       for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+        for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[u], a[t], acc[t][u], 0, 0, 0);
     }
     __syncthreads();  // every wave has read the tile
     store_hidden(f.bias[l]);
     __syncthreads();
   }
-  // layer 3: 12 outputs (one 32-column tile); waves 0 and 1 take a row tile each
+  // layer 3: 12 outputs (one 32-feature tile); waves 0 and 1 take a row tile
+  // each; lane (r, h) holds features 8q + 4h .. +3 of row 32*wave + r
   if (wave < 2) {
     f32x16 y = {};
     const __bf16* W = f.W[kLayers - 1];
@@ -367,15 +381,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     for (int k = 0; k < kHidden; k += 16) {
       const bf16x8 a = *(const bf16x8*)(H + fswz(32 * wave + r, k + 8 * h));
       const bf16x8 b = *(const bf16x8*)(W + r * kHidden + k + 8 * h);
-      y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, y, 0, 0, 0);
+      y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, y, 0, 0, 0);
     }
-    const int col = r;
-    if (col < kOut) {
-      const float bv = f.bias[kLayers - 1][col], sc = f.y_scale[col], sh = f.y_shift[col];
+    const int64_t row = m0 + 32 * wave + r;
+    if (row < f.M) {
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int64_t row = m0 + 32 * wave + 8 * (j >> 2) + 4 * h + (j & 3);
-        if (row < f.M) f.Y[row * kOut + col] = (y[j] + bv) * sc + sh;
+      for (int q = 0; q < 2; ++q) {
+        const int col = 8 * q + 4 * h;
+        if (col + 4 <= kOut) {
+          float4 o;
+          o.x = (y[4 * q + 0] + f.bias[kLayers - 1][col + 0]) * f.y_scale[col + 0] + f.y_shift[col + 0];
+          o.y = (y[4 * q + 1] + f.bias[kLayers - 1][col + 1]) * f.y_scale[col + 1] + f.y_shift[col + 1];
+          o.z = (y[4 * q + 2] + f.bias[kLayers - 1][col + 2]) * f.y_scale[col + 2] + f.y_shift[col + 2];
+          o.w = (y[4 * q + 3] + f.bias[kLayers - 1][col + 3]) * f.y_scale[col + 3] + f.y_shift[col + 3];
+          *(float4*)(f.Y + row * kOut + col) = o;
+        }
       }
     }
   }
