@@ -284,15 +284,16 @@ __global__ void k_pack_weights(const float* W, int N, int K, __bf16* Wnk, int kp
 // fused forward (inference)
 // ---------------------------------------------------------------------------
 // One workgroup (4 waves) per block of kFuseRows rows runs all four layers:
-// the block's activations stay in LDS (one 64 x 256 BF16 tile, updated in
+// the block's activations stay in LDS (one 128 x 256 BF16 tile, updated in
 // place), so HBM sees only the BF16 features in and the 12 outputs out; the
-// weights stream from the L2.  Each wave computes a 64-row x 64-column slab
-// of a hidden layer as 2 x 2 MFMA 32x32x16 tiles (four independent
-// accumulators).  The k order of every dot product is that of k_gemm, so the
-// results are the same bits.  LDS element (row, k) sits at
+// weights stream from the L2, once per 128 rows.  Each wave computes a
+// 128-row x 64-column slab of a hidden layer as 4 x 2 MFMA 32x32x16 tiles
+// (eight independent accumulators).  The k order of every dot product is
+// that of k_gemm, so the results are the same bits.  LDS element (row, k) sits at
 // row * 256 + (((k / 8) ^ (row % 32)) * 8) + k % 8: the 16-byte fragments
 // that the 32 rows of a tile read at one k land in different banks.
-constexpr int kFuseRows = 64;
+constexpr int kFuseRows = 128;
+constexpr int kRowTiles = kFuseRows / 32;  // 32-row MFMA tiles per block (at most 4: one per wave in layer 3)
 
 __device__ __forceinline__ int fswz(int row, int k) { return row * kHidden + ((((k >> 3) ^ (row & 31))) << 3) + (k & 7); }
 
@@ -305,7 +306,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   // acc[t][u][j] is feature wave*64 + 32u + 8(j/4) + 4h + j%4 of tile row
   // 32t + r: a lane holds runs of 4 consecutive features of one row, stored
   // to the LDS tile 8 bytes at a time.
-  f32x16 acc[2][2];
+  f32x16 acc[kRowTiles][2];
   // hidden-layer epilogue: bias, SiLU, BF16 into the LDS tile
   auto store_hidden = [&](const float* bias) {
 #pragma unroll
@@ -315,7 +316,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         const int col = wave * 64 + 32 * u + 8 * q + 4 * h;
         const float4 bv{bias[col], bias[col + 1], bias[col + 2], bias[col + 3]};
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int t = 0; t < kRowTiles; ++t) {
           const int row = 32 * t + r;
           bf16x4 v;
           v[0] = __bf16(silu(acc[t][u][4 * q + 0] + bv.x));
@@ -329,21 +330,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   };
   // layer 0: rows from the features in HBM
 #pragma unroll
-  for (int t = 0; t < 2; ++t)
+  for (int t = 0; t < kRowTiles; ++t)
 #pragma unroll
     for (int u = 0; u < 2; ++u) acc[t][u] = f32x16{};
 #pragma unroll
   for (int k = 0; k < kFeatPad; k += 16) {
-    bf16x8 a[2], b[2];
+    bf16x8 a[kRowTiles], b[2];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < kRowTiles; ++t) {
       const int64_t row = m0 + 32 * t + r;
       a[t] = row < f.M ? *(const bf16x8*)(f.X + row * kFeatPad + k + 8 * h) : bf16x8{};
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u) b[u] = *(const bf16x8*)(f.W[0] + (wave * 64 + 32 * u + r) * kFeatPad + k + 8 * h);
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < kRowTiles; ++t)
 #pragma unroll
       for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[u], a[t], acc[t][u], 0, 0, 0);
   }
@@ -352,19 +353,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   // layers 1 and 2: rows from the LDS tile, results back into it
   for (int l = 1; l < kLayers - 1; ++l) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < kRowTiles; ++t)
 #pragma unroll
       for (int u = 0; u < 2; ++u) acc[t][u] = f32x16{};
     const __bf16* W = f.W[l];
 #pragma unroll 4
     for (int k = 0; k < kHidden; k += 16) {
-      bf16x8 a[2], b[2];
+      bf16x8 a[kRowTiles], b[2];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) a[t] = *(const bf16x8*)(H + fswz(32 * t + r, k + 8 * h));
+      for (int t = 0; t < kRowTiles; ++t) a[t] = *(const bf16x8*)(H + fswz(32 * t + r, k + 8 * h));
 #pragma unroll
       for (int u = 0; u < 2; ++u) b[u] = *(const bf16x8*)(W + (wave * 64 + 32 * u + r) * kHidden + k + 8 * h);
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
+      for (int t = 0; t < kRowTiles; ++t)
 #pragma unroll
         for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[u], a[t], acc[t][u], 0, 0, 0);
     }
@@ -372,9 +373,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     store_hidden(f.bias[l]);
     __syncthreads();
   }
-  // layer 3: 12 outputs (one 32-feature tile); waves 0 and 1 take a row tile
-  // each; lane (r, h) holds features 8q + 4h .. +3 of row 32*wave + r
-  if (wave < 2) {
+  // layer 3: 12 outputs (one 32-feature tile); wave t takes row tile t; lane
+  // (r, h) holds features 8q + 4h .. +3 of row 32*wave + r
+  if (wave < kRowTiles) {
     f32x16 y = {};
     const __bf16* W = f.W[kLayers - 1];
 #pragma unroll 4
