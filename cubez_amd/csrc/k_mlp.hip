@@ -293,27 +293,29 @@ __global__ void k_pack_weights(const float* W, int N, int K, __bf16* Wnk, int kp
 // row * 256 + (((k / 8) ^ (row % 32)) * 8) + k % 8: the 16-byte fragments
 // that the 32 rows of a tile read at one k land in different banks.
 constexpr int kFuseRows = 128;
-constexpr int kRowTiles = kFuseRows / 32;  // 32-row MFMA tiles per block (at most 4: one per wave in layer 3)
+constexpr int kRowTiles = kFuseRows / 32;  // 32-row MFMA tiles per block (one per wave in layer 3)
+constexpr int kFuseWaves = 4;
+constexpr int kColTiles = kHidden / 32 / kFuseWaves;  // 32-column tiles per wave
 
 __device__ __forceinline__ int fswz(int row, int k) { return row * kHidden + ((((k >> 3) ^ (row & 31))) << 3) + (k & 7); }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_mlp_forward(MlpArgs f) {
+__global__ void __launch_bounds__(64 * kFuseWaves) __attribute__((amdgpu_waves_per_eu(2, 2))) k_mlp_forward(MlpArgs f) {
   __shared__ __align__(16) __bf16 H[kFuseRows * kHidden];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 31, h = lane >> 5;
   const int64_t m0 = int64_t(blockIdx.x) * kFuseRows;
   // The weights are the MFMA's first operand and the rows its second, so
-  // acc[t][u][j] is feature wave*64 + 32u + 8(j/4) + 4h + j%4 of tile row
+  // acc[t][u][j] is feature 32 (wave kColTiles + u) + 8(j/4) + 4h + j%4 of tile row
   // 32t + r: a lane holds runs of 4 consecutive features of one row, stored
   // to the LDS tile 8 bytes at a time.
-  f32x16 acc[kRowTiles][2];
+  f32x16 acc[kRowTiles][kColTiles];
   // hidden-layer epilogue: bias, SiLU, BF16 into the LDS tile
   auto store_hidden = [&](const float* bias) {
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kColTiles; ++u) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int col = wave * 64 + 32 * u + 8 * q + 4 * h;
+        const int col = wave * 32 * kColTiles + 32 * u + 8 * q + 4 * h;
         const float4 bv{bias[col], bias[col + 1], bias[col + 2], bias[col + 3]};
 #pragma unroll
         for (int t = 0; t < kRowTiles; ++t) {
@@ -332,21 +334,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
   for (int t = 0; t < kRowTiles; ++t)
 #pragma unroll
-    for (int u = 0; u < 2; ++u) acc[t][u] = f32x16{};
+    for (int u = 0; u < kColTiles; ++u) acc[t][u] = f32x16{};
 #pragma unroll
   for (int k = 0; k < kFeatPad; k += 16) {
-    bf16x8 a[kRowTiles], b[2];
+    bf16x8 a[kRowTiles], b[kColTiles];
 #pragma unroll
     for (int t = 0; t < kRowTiles; ++t) {
       const int64_t row = m0 + 32 * t + r;
       a[t] = row < f.M ? *(const bf16x8*)(f.X + row * kFeatPad + k + 8 * h) : bf16x8{};
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u) b[u] = *(const bf16x8*)(f.W[0] + (wave * 64 + 32 * u + r) * kFeatPad + k + 8 * h);
+    for (int u = 0; u < kColTiles; ++u) b[u] = *(const bf16x8*)(f.W[0] + (wave * 32 * kColTiles + 32 * u + r) * kFeatPad + k + 8 * h);
 #pragma unroll
     for (int t = 0; t < kRowTiles; ++t)
 #pragma unroll
-      for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[u], a[t], acc[t][u], 0, 0, 0);
+      for (int u = 0; u < kColTiles; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[u], a[t], acc[t][u], 0, 0, 0);
   }
   store_hidden(f.bias[0]);
   __syncthreads();
@@ -355,19 +357,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
     for (int t = 0; t < kRowTiles; ++t)
 #pragma unroll
-      for (int u = 0; u < 2; ++u) acc[t][u] = f32x16{};
+      for (int u = 0; u < kColTiles; ++u) acc[t][u] = f32x16{};
     const __bf16* W = f.W[l];
 #pragma unroll 4
     for (int k = 0; k < kHidden; k += 16) {
-      bf16x8 a[kRowTiles], b[2];
+      bf16x8 a[kRowTiles], b[kColTiles];
 #pragma unroll
       for (int t = 0; t < kRowTiles; ++t) a[t] = *(const bf16x8*)(H + fswz(32 * t + r, k + 8 * h));
 #pragma unroll
-      for (int u = 0; u < 2; ++u) b[u] = *(const bf16x8*)(W + (wave * 64 + 32 * u + r) * kHidden + k + 8 * h);
+      for (int u = 0; u < kColTiles; ++u) b[u] = *(const bf16x8*)(W + (wave * 32 * kColTiles + 32 * u + r) * kHidden + k + 8 * h);
 #pragma unroll
       for (int t = 0; t < kRowTiles; ++t)
 #pragma unroll
-        for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[u], a[t], acc[t][u], 0, 0, 0);
+        for (int u = 0; u < kColTiles; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[u], a[t], acc[t][u], 0, 0, 0);
     }
     __syncthreads();  // every wave has read the tile
     store_hidden(f.bias[l]);
@@ -405,7 +407,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 hipError_t launch_mlp_forward(const MlpArgs& f, hipStream_t st) {
   if (f.M <= 0) return hipSuccess;
   const int64_t blocks = (int64_t(f.M) + kFuseRows - 1) / kFuseRows;
-  k_mlp_forward<<<int(blocks), 256, 0, st>>>(f);
+  k_mlp_forward<<<int(blocks), 64 * kFuseWaves, 0, st>>>(f);
   return hipGetLastError();
 }
 
