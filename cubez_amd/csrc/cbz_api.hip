@@ -695,7 +695,10 @@ template <typename R>
 int do_broadphase(cbz_ctx* c, int64_t* n_pairs) {
   auto& bp = c->bp;
   int rc;
-  if ((rc = bp_ensure(c, bp.plane_pairs + 16 * c->nb + 1024))) return rc;
+  if ((rc = bp_ensure(c, 0))) return rc;  // scene tables and the body-plane pair count
+  if (bp.plane_pairs > INT32_MAX / 2) return fail(c, CBZ_ERR_CAPACITY, "broadphase: too many body-plane pairs");
+  // the body-plane keys always fit: the capacity is at least plane_pairs
+  if ((rc = bp_ensure(c, std::min<int64_t>(bp.plane_pairs + 16 * c->nb + 1024, INT32_MAX / 2)))) return rc;
   if ((rc = refresh_brad(c))) return rc;
   HIPCHK(c, launch_bp_bounds_sort<R>(bp_args<R>(c), bp.tmp, bp.tmp_bytes, c->stream));
   BroadPlanes pp{(int32_t)c->nb, bp.body_world, c->d_body_off, c->d_plane_off, bp.plane_base, bp.pkey_in};
@@ -705,12 +708,13 @@ int do_broadphase(cbz_ctx* c, int64_t* n_pairs) {
     HIPCHK(c, launch_bp_sweep<R>(bp_args<R>(c), pp, c->stream));
     HIPCHK(c, hipMemcpyAsync(&found, bp.pcount, sizeof found, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (bp.plane_pairs + found > INT32_MAX / 2)
+      return fail(c, CBZ_ERR_CAPACITY, "broadphase: more than 2^30 candidate pairs");
     if (bp.plane_pairs + found <= bp.pcap) break;
     if (round == 1) return fail(c, CBZ_ERR_HIP, "broadphase: pair count changed between sweeps");
-    if ((rc = bp_ensure(c, bp.plane_pairs + found + found / 4 + 1024))) return rc;
+    if ((rc = bp_ensure(c, std::min<int64_t>(bp.plane_pairs + found + found / 4 + 1024, INT32_MAX / 2)))) return rc;
   }
   const int64_t n = bp.plane_pairs + found;
-  if (n > INT32_MAX / 2) return fail(c, CBZ_ERR_CAPACITY, "broadphase: more than 2^30 candidate pairs");
   if ((rc = ensure_list(c, n))) return rc;
   BroadList L{n, c->nw, bp.pkey_in, bp.pkey, bp.body_world, c->d_body_off, c->d_plane_off, c->l_one, c->l_two,
               c->l_off};

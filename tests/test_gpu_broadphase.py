@@ -87,3 +87,25 @@ def test_broadphase_steps_equal_fused_steps():
             b.resolve(sc.dt, 4)
         for k in STATE:
             assert same(a.get(k), b.get(k)), k
+
+
+def test_broadphase_with_many_planes_per_world():
+    """20 planes per world: more body-plane pairs than the first sizing of
+    the pair buffer assumes per body (a regression test for that sizing)."""
+    base = scenes.batched(5, 4, 8, 64)
+    nw = base.n_worlds
+    normals, offsets = [], []
+    for w in range(nw):
+        normals.append([0.0, 1.0, 0.0])
+        offsets.append(0.0)
+        for k in range(19):  # tilted planes far below the scene: pairs but no contacts
+            a = 0.3 * (k + 1)
+            normals.append([np.sin(a) * 0.2, np.sqrt(1 - 0.04 * np.sin(a) ** 2), np.cos(a) * 0.2])
+            offsets.append(-200.0 - k)
+    sc = scenes.Scene(base.body_off.copy(), np.arange(nw + 1, dtype=np.int32) * 20, np.array(normals),
+                      np.array(offsets), base.fields, name="many-planes", dt=base.dt)
+    with World(sc) as w:
+        w.step(sc.dt, 120, 4)
+        w.integrate(sc.dt)
+        total, checked = _compare(w)
+        assert total > 0 and checked >= 20 * sc.n_bodies
