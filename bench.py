@@ -128,9 +128,12 @@ def run_path(World, sc, precision, steps, warmup):
     # K1 alone: back-to-back launches on the initial state between two events
     w.load(sc)
     k1_ms, k1_bodies = w.integrate_timed(sc.dt, K1_LAUNCHES)
+    per_cu, n_cu = w.resolver_occupancy()
     w.close()
     sp, integrated, contacts, iters = (int(x) for x in cnt)
     names = ("integrate", "np_count", "np_scans", "np_emit", "resolve", "step_profiled")
+    resident = min(per_cu * n_cu, sc.n_worlds)
+    resolve_ns = ms[4] / sp * 1e6
     return {
         "value": sc.n_bodies * steps / wall,
         "ms_per_step": 1000.0 * wall / steps,
@@ -140,6 +143,16 @@ def run_path(World, sc, precision, steps, warmup):
         "contacts_per_step": contacts / sp,
         "contacts_per_s": contacts / (ms[5] / 1000.0) if ms[5] > 0 else None,
         "iters_per_step": iters / sp,
+        # the resolver's latency measures (its phase has no roofline): one
+        # wavefront per world, worlds_per_cu of them resident per CU
+        "resolver": {
+            "iterations_per_world_per_step": iters / sp / sc.n_worlds,
+            "worlds_per_cu": per_cu,
+            "resident_worlds": resident,
+            "waves_per_simd": resident / (4.0 * n_cu),
+            "ns_per_iteration_aggregate": resolve_ns / (iters / sp) if iters else None,
+            "ns_per_iteration_per_world": resolve_ns * resident / (iters / sp) if iters else None,
+        },
         "k1_ms_per_launch": k1_ms,
         "k1_bodies_per_launch": k1_bodies / K1_LAUNCHES,
     }
@@ -300,8 +313,8 @@ def main():
     else:
         out["contacts_per_step"] = exact32["contacts_per_step"]
         out["contacts_per_s"] = exact32["contacts_per_s"]
-    out["exact_f64"] = {k: exact64[k] for k in ("value", "ms_per_step", "kernels", "iters_per_step")}
-    out["exact_f32"] = {k: exact32[k] for k in ("value", "ms_per_step", "kernels", "iters_per_step")}
+    out["exact_f64"] = {k: exact64[k] for k in ("value", "ms_per_step", "kernels", "iters_per_step", "resolver")}
+    out["exact_f32"] = {k: exact32[k] for k in ("value", "ms_per_step", "kernels", "iters_per_step", "resolver")}
     out["kernels_ms_per_step"] = kern
     out["dominant_kernel"] = max(("integrate", "np_count", "np_scans", "np_emit", "resolve"), key=lambda k: kern[k])
     ratio = traffic_ratio("f32")

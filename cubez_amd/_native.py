@@ -65,6 +65,7 @@ EXPORTS = (
     "cbz_sur_train", "cbz_sur_grad", "cbz_sur_predict", "cbz_sur_resolve", "cbz_sur_world_mask", "cbz_sur_phase_times", "cbz_step_sur", "cbz_integrate_timed",
     "cbz_set_materials", "cbz_set_plane_materials", "cbz_set_contact_materials",
     "cbz_checkpoint_size", "cbz_checkpoint_save", "cbz_checkpoint_load", "cbz_narrowphase_broad",
+    "cbz_resolver_occupancy",
 )
 
 
@@ -149,6 +150,7 @@ def load():
         "cbz_checkpoint_save": ([P, P, i64], i32),
         "cbz_checkpoint_load": ([P, P, i64], i32),
         "cbz_narrowphase_broad": ([P, P, P], i32),
+        "cbz_resolver_occupancy": ([P, P, P], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

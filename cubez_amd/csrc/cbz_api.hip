@@ -1648,6 +1648,17 @@ int cbz_resolve(cbz_ctx* c, double dt, int64_t iter_per_contact, int64_t iter_fi
   return CBZ_OK;
 }
 
+int cbz_resolver_occupancy(cbz_ctx* c, int32_t* worlds_per_cu, int32_t* n_cu) {
+  if (!c) return CBZ_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  int w = 0, cu = 0;
+  HIPCHK(c, resolver_occupancy(c->precision == CBZ_F64, &w));
+  HIPCHK(c, hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, c->device));
+  if (worlds_per_cu) *worlds_per_cu = w;
+  if (n_cu) *n_cu = cu;
+  return CBZ_OK;
+}
+
 int cbz_resolver_iterations(cbz_ctx* c, int64_t* iters) {
   int rc = need_scene(c);
   if (rc) return rc;

@@ -80,6 +80,7 @@ template <typename R>
 hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st, hipEvent_t* ev = nullptr);
 template <typename R>
 hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st);
+hipError_t resolver_occupancy(bool f64, int* worlds_per_cu);
 
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);

@@ -1217,6 +1217,13 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Worlds (workgroups) of the plain resolver kernel resident per CU, from
+// its register and LDS use.
+hipError_t resolver_occupancy(bool f64, int* worlds_per_cu) {
+  return f64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(worlds_per_cu, k_resolve_f64<KIND_PLAIN>, 64, 0)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(worlds_per_cu, k_resolve_f32<KIND_PLAIN>, 64, 0);
+}
+
 template hipError_t launch_resolve<double>(const ResolveLaunch<double>&, hipStream_t);
 template hipError_t launch_resolve<float>(const ResolveLaunch<float>&, hipStream_t);
 

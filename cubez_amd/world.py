@@ -183,6 +183,12 @@ class World:
     def resolve(self, dt, iter_per_contact=4, iter_fixed=0):
         self._chk(self._lib.cbz_resolve(self._h, float(dt), int(iter_per_contact), int(iter_fixed)))
 
+    def resolver_occupancy(self):
+        """(worlds resident per CU for the plain resolver, CU count)."""
+        a, b = ctypes.c_int32(0), ctypes.c_int32(0)
+        self._chk(self._lib.cbz_resolver_occupancy(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def resolver_iterations(self):
         it = np.empty(2 * self.n_worlds, np.int64)
         self._chk(self._lib.cbz_resolver_iterations(self._h, _ptr(it)))

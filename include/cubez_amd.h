@@ -231,6 +231,11 @@ int cbz_resolve(cbz_ctx *ctx, double dt, int64_t iter_per_contact, int64_t iter_
 /* iterations the last resolve used per world: [2*n_worlds] = positions,
  * velocities (the loop counters of contact.go:235 and :392) */
 int cbz_resolver_iterations(cbz_ctx *ctx, int64_t *iters);
+/* Worlds the resolver keeps resident per CU (the plain kernel of the
+ * context's precision: one wavefront per world, bounded by its registers
+ * and LDS) and the device's CU count; the bench derives the time per
+ * serial iteration from them. */
+int cbz_resolver_occupancy(cbz_ctx *ctx, int32_t *worlds_per_cu, int32_t *n_cu);
 
 /* --- fused stepping --------------------------------------------------------
  * n_steps of updateCallback (examples/cubedrop.go:69-75): integrate, the
