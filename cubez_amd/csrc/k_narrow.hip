@@ -409,8 +409,11 @@ __device__ __forceinline__ int64_t block_reserve(int mine, int32_t* counter, int
   return base + incl - mine;
 }
 
-// Cull: one wave per task (world w, column tile c, rows r0 .. r0+15).  Lane
-// l owns column 64c + l for the whole task: a plane, or body j whose centre,
+// tasks per wave of the cull: their candidates share one block reservation
+constexpr int kCullTasks = 4;
+
+// Cull: each wave takes kCullTasks consecutive tasks (world w, column tile
+// c, rows r0 .. r0+15).  Lane l owns column 64c + l for the whole task: a plane, or body j whose centre,
 // bounding radius and shape stay in registers; the 16 row bodies are loaded
 // once (lane k holds row r0 + k) and broadcast.  Each tile is cleared here;
 // its plane pairs and the body pairs that pass the conservative bound (see
@@ -420,19 +423,30 @@ __global__ void __launch_bounds__(256) k_np_cull(NP<R> a) {
   __shared__ int s_wtot[4];
   __shared__ int s_base;
   const int lane = threadIdx.x & 63;
-  const int task = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const bool valid = task < a.ps.n_tasks;  // the other waves still join the block's reservation
-  uint32_t cm = 0;                         // bit k: row r0 + k is a candidate for this lane
-  int t0 = 0, r0 = 0, nch = 1, c = 0;
-  if (valid) {
+  const int task0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * kCullTasks;
+  // bit 16(q%4) + k of word q/4: row r0(q) + k of the wave's task q is a
+  // candidate for this lane
+  uint64_t cm[(kCullTasks + 3) / 4];
+#pragma unroll
+  for (int u = 0; u < (kCullTasks + 3) / 4; ++u) cm[u] = 0;
+  int t0[kCullTasks], r0[kCullTasks], nch[kCullTasks], cc[kCullTasks];
+#pragma unroll
+  for (int q = 0; q < kCullTasks; ++q) {
+    t0[q] = 0;
+    r0[q] = 0;
+    nch[q] = 1;
+    cc[q] = 0;
+    const int task = task0 + q;
+    if (task >= a.ps.n_tasks) continue;  // the wave still joins the block's reservation
     const int w = a.ps.task_world[task];
     const int b0 = a.ps.body_off[w], n = a.ps.body_off[w + 1] - b0;
     const int p0 = a.ps.plane_off[w], np = a.ps.plane_off[w + 1] - p0;
-    nch = (np + n + 63) >> 6;
+    nch[q] = (np + n + 63) >> 6;
     const int k = task - a.ps.world_task0[w];
-    c = k % nch;
-    r0 = (k / nch) * kRowBlock;
-    const int r1 = min(n, r0 + kRowBlock);
+    const int c = k % nch[q];
+    cc[q] = c;
+    r0[q] = (k / nch[q]) * kRowBlock;
+    const int r1 = min(n, r0[q] + kRowBlock);
     const int col = 64 * c + lane;
     const bool is_plane = col < np;
     const int j = col - np;
@@ -451,17 +465,17 @@ __global__ void __launch_bounds__(256) k_np_cull(NP<R> a) {
     // row bodies r0 .. r1-1: lane k < 16 loads row r0 + k
     double ti[3] = {0, 0, 0}, ri = 0;
     int si = 0;
-    if (lane < r1 - r0) {
-      const int g = b0 + r0 + lane;
+    if (lane < r1 - r0[q]) {
+      const int g = b0 + r0[q] + lane;
       ti[0] = double(a.b.ctransform[9 * S + g]);
       ti[1] = double(a.b.ctransform[10 * S + g]);
       ti[2] = double(a.b.ctransform[11 * S + g]);
       ri = a.brad[g];
       si = a.b.shape[g];
     }
-    t0 = a.ps.world_tile0[w];
-    for (int r = r0; r < r1; ++r) {
-      const int src = r - r0;
+    t0[q] = a.ps.world_tile0[w];
+    for (int r = r0[q]; r < r1; ++r) {
+      const int src = r - r0[q];
       const double x = __shfl(ti[0], src, 64), y = __shfl(ti[1], src, 64), z = __shfl(ti[2], src, 64);
       const double rr = __shfl(ri, src, 64) + rj;
       const int s_i = __shfl(si, src, 64);
@@ -473,18 +487,25 @@ __global__ void __launch_bounds__(256) k_np_cull(NP<R> a) {
         const double kf = (s_i == SHAPE_CUBE && sj == SHAPE_CUBE) ? 3.003 : 1.001;
         cand = spheres || !(d2 > kf * rr * rr);
       }
-      if (cand) cm |= 1u << src;
+      if (cand) cm[q >> 2] |= 1ull << (16 * (q & 3) + src);
       if (lane == 0) {
-        const int t = t0 + r * nch + c;
+        const int t = t0[q] + r * nch[q] + c;
         a.tile_cnt[t] = 0;
         a.tile_mask[t] = 0ull;
       }
     }
   }
-  int64_t pos = block_reserve(__popc(cm), a.n_cand, s_wtot, &s_base);
-  for (uint32_t m = cm; m; m &= m - 1u, ++pos) {
-    const int r = r0 + __ffs(m) - 1;
-    if (pos < a.cand_cap) a.cand[pos] = uint32_t(t0 + r * nch + c) * 64u + uint32_t(lane);
+  // one reservation per block for all of its waves' tasks
+  int ncand = 0;
+#pragma unroll
+  for (int u = 0; u < (kCullTasks + 3) / 4; ++u) ncand += __popcll(cm[u]);
+  int64_t pos = block_reserve(ncand, a.n_cand, s_wtot, &s_base);
+#pragma unroll
+  for (int q = 0; q < kCullTasks; ++q) {
+    for (uint32_t m = uint32_t(cm[q >> 2] >> (16 * (q & 3))) & 0xFFFFu; m; m &= m - 1u, ++pos) {
+      const int r = r0[q] + __ffs(m) - 1;
+      if (pos < a.cand_cap) a.cand[pos] = uint32_t(t0[q] + r * nch[q] + cc[q]) * 64u + uint32_t(lane);
+    }
   }
 }
 
@@ -692,7 +713,7 @@ hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st, hipEvent
     hipError_t e = launch_fill_i32(a.n_hits, 1, 0, st);
     if (e == hipSuccess) e = launch_fill_i32(a.n_cand, 1, 0, st);
     if (e != hipSuccess) return e;
-    const int nbt = (a.ps.n_tasks + 3) / 4;
+    const int nbt = (a.ps.n_tasks + 4 * kCullTasks - 1) / (4 * kCullTasks);
     if (nbt > 0) k_np_cull<R><<<nbt, 256, 0, st>>>(a);
     const int64_t want = (a.cand_cap + 255) / 256;
     k_np_eval<R><<<int(want < 8192 ? (want > 0 ? want : 1) : 8192), 256, 0, st>>>(a);
