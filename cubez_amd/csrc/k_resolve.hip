@@ -836,16 +836,18 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
 #pragma unroll
       for (int s = 0; s < SLOTS; ++s) {
         if (s == (bi >> 6)) {
-          const V3<R> q = get_pt(s);
           s_bc[0] = nrm[s].x; s_bc[1] = nrm[s].y; s_bc[2] = nrm[s].z;
-          s_bc[3] = q.x;      s_bc[4] = q.y;      s_bc[5] = q.z;
+          if constexpr (!PT_HBM) {  // the wide classes read the point from HBM below
+            const V3<R> q = get_pt(s);
+            s_bc[3] = q.x; s_bc[4] = q.y; s_bc[5] = q.z;
+          }
           s_bcid = ids[s];
         }
       }
     }
     __syncthreads();
     const V3<R> n{s_bc[0], s_bc[1], s_bc[2]};
-    const V3<R> p{s_bc[3], s_bc[4], s_bc[5]};
+    const V3<R> p = PT_HBM ? ld3(c.point, CS, base + bi) : V3<R>{s_bc[3], s_bc[4], s_bc[5]};
     const uint32_t kid = s_bcid;
     const int kl[2] = {id_lo(kid), id_hi(kid)};
     const int kb[2] = {b0w + kl[0], kl[1] >= 0 ? b0w + kl[1] : -1};
@@ -1043,16 +1045,18 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s) {
           if (s == (bi >> 6)) {
-            const V3<R> q = get_pt(s);
             s_bc[0] = nrm[s].x; s_bc[1] = nrm[s].y; s_bc[2] = nrm[s].z;
-            s_bc[3] = q.x;      s_bc[4] = q.y;      s_bc[5] = q.z;
+            if constexpr (!PT_HBM) {  // the wide classes read the point from HBM below
+              const V3<R> q = get_pt(s);
+              s_bc[3] = q.x; s_bc[4] = q.y; s_bc[5] = q.z;
+            }
             s_bcid = ids[s];
           }
         }
       }
       __syncthreads();
       const V3<R> n{s_bc[0], s_bc[1], s_bc[2]};
-      const V3<R> p{s_bc[3], s_bc[4], s_bc[5]};
+      const V3<R> p = PT_HBM ? ld3(c.point, CS, base + bi) : V3<R>{s_bc[3], s_bc[4], s_bc[5]};
       const V3<R> cvk = ld3(c.cvel, CS, base + bi);
       const uint32_t kid = s_bcid;
       const int kl[2] = {id_lo(kid), id_hi(kid)};
