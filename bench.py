@@ -34,6 +34,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "body-steps/s, 4096 worlds: surrogate vs exact GPU resolver at ≤1e-3 m error"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector, FMA counted as two
+SAT_AXIS_FLOP = 75
+PLANE_CHECK_FLOP = 40
 K1_LAUNCHES = 50
 
 # Algorithmic bytes of K1 per body-step: an integrated body reads 29 reals
@@ -70,7 +73,7 @@ def _cpu_world(args):
     o.calculate_derived()
     t0 = time.perf_counter()
     o.step(sc.dt, steps, sc.iter_mult)
-    return o.n * steps, time.perf_counter() - t0
+    return o.n * steps, time.perf_counter() - t0, o.counters()
 
 
 def cpu_baseline(scene, steps, budget_s):
@@ -97,6 +100,7 @@ def cpu_baseline(scene, steps, budget_s):
         res = pool.map(_cpu_world, [(w, steps) for w in worlds])
     wall = time.perf_counter() - t0
     bs = sum(r[0] for r in res)
+    counters = {k: sum(r[2][k] for r in res) for k in res[0][2]} if res else {}
     return {
         "value": bs / wall,
         "unit": "body-steps/s",
@@ -104,6 +108,30 @@ def cpu_baseline(scene, steps, budget_s):
         "kind": "port",
         "sample": f"{len(worlds)} of {n_worlds} worlds (every {stride}th), all {steps} steps, "
                   f"{bs} body-steps in {wall:.1f} s wall on {cores} processes",
+    }, counters, bs
+
+
+def narrowphase_rate(counters, sample_body_steps, n_bodies, kernels):
+    """K2's FP64 VALU rate: the reference's narrowphase work on the same
+    steps, from the oracle's op counters on the CPU sample (scaled to the
+    whole scene by body-steps), over the FP64 narrowphase kernels' time per
+    step.  Work model (per unit, the reference's arithmetic in
+    colliders.go:445-477 tryAxis / penetrationOnAxis, :762-770
+    transformToAxis, :308-367 the half-space check): 75 flops per SAT axis
+    tested (normalise, two projections, the distance), 40 per body-plane check;
+    contact generation is not counted, so the rate is a lower bound."""
+    scale = n_bodies / max(sample_body_steps, 1)
+    per_step = {k: counters[k] * scale for k in ("pairs", "plane_checks", "sat_axes", "sat_skipped", "contacts")}
+    flop = SAT_AXIS_FLOP * per_step["sat_axes"] + PLANE_CHECK_FLOP * per_step["plane_checks"]
+    ms = kernels["np_count"] + kernels["np_scans"] + kernels["np_emit"]
+    tf = flop / (ms / 1000.0) / 1e12 if ms > 0 else 0.0
+    return {
+        "kernels": "k_np_cull + k_np_eval + scans + k_np_emit_hits (FP64 state, same 600 steps)",
+        "bound": "fp64-valu", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": tf / FP64_PEAK_TFLOPS, "frac_of_unfused_peak": tf / (FP64_PEAK_TFLOPS / 2),
+        "flop_per_step": flop, "ms_per_step": ms, "reference_work_per_step": per_step,
+        "flop_model": f"{SAT_AXIS_FLOP} per SAT axis tested + {PLANE_CHECK_FLOP} per body-plane check "
+                      "(contact generation not counted: a lower bound)",
     }
 
 
@@ -329,7 +357,9 @@ def main():
         if ratio else None,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, args.steps, args.cpu_budget)
+        out["cpu_baseline"], counters, sample_bs = cpu_baseline(sc, args.steps, args.cpu_budget)
+        if counters:
+            out["narrowphase"] = narrowphase_rate(counters, sample_bs, nb, exact64["kernels"])
     print(json.dumps(out), flush=True)
 
 
