@@ -107,8 +107,22 @@ def cpu_baseline(scene, steps, budget_s):
         "cores": cores,
         "kind": "port",
         "sample": f"{len(worlds)} of {n_worlds} worlds (every {stride}th), all {steps} steps, "
-                  f"{bs} body-steps in {wall:.1f} s wall on {cores} processes",
+                  f"{bs} body-steps in {wall:.1f} s wall on {cores} processes; host: {_host_cpu()}",
     }, counters, bs
+
+
+def _host_cpu():
+    """nproc and the CPU model of the host the baseline ran on."""
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"nproc {os.cpu_count()}, {model}"
 
 
 def narrowphase_rate(counters, sample_body_steps, n_bodies, kernels):
