@@ -69,7 +69,7 @@ struct cbz_context {
   int32_t* d_plane_off = nullptr;
 
   // all-pairs pair space
-  int32_t n_tiles = 0, n_tasks = 0;
+  int32_t n_tiles = 0, n_tasks = 0, max_world_tiles = 0;
   int32_t* d_tile_world = nullptr;
   int32_t* d_world_tile0 = nullptr;
   int32_t* d_task_world = nullptr;
@@ -492,6 +492,7 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles, hipEvent_t* ev = nullptr) {
   L.ps.plane_off = c->d_plane_off;
   if (list_tiles < 0) {
     L.ps.n_tiles = c->n_tiles;
+    L.ps.max_world_tiles = c->max_world_tiles;
     L.ps.tile_world = c->d_tile_world;
     L.ps.world_tile0 = c->d_world_tile0;
     L.ps.world_q = nullptr;
@@ -503,6 +504,7 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles, hipEvent_t* ev = nullptr) {
     L.ps.list_two = nullptr;
   } else {
     L.ps.n_tiles = list_tiles;
+    L.ps.max_world_tiles = 0;  // list worlds are short: the one-wave scan
     L.ps.tile_world = c->l_tile_world;
     L.ps.world_tile0 = c->l_world_tile0;
     L.ps.world_q = c->l_world_q;
@@ -1189,13 +1191,14 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   // all-pairs tiles: row-aligned, nch = ceil((np + n) / 64) per row; count
   // tasks: nch column tiles x ceil(n / kRowBlock) row blocks per world
   std::vector<int32_t> wt0(n_worlds + 1), tw, wk0(n_worlds + 1), kw;
-  int64_t nt = 0, nk = 0;
+  int64_t nt = 0, nk = 0, max_wt = 0;
   for (int32_t w = 0; w < n_worlds; ++w) {
     const int64_t n = body_off[w + 1] - body_off[w], npw = plane_off[w + 1] - plane_off[w];
     const int64_t nch = (npw + n + 63) / 64;
     wt0[w] = (int32_t)nt;
     wk0[w] = (int32_t)nk;
     nt += n * nch;
+    max_wt = std::max<int64_t>(max_wt, n * nch);
     nk += nch * ((n + kRowBlock - 1) / kRowBlock);
     if (nt > (int64_t)(INT32_MAX / 64) || nk > INT32_MAX / 2)
       return fail(c, CBZ_ERR_ARG, "set_scene: pair space too large");
@@ -1210,6 +1213,7 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   }
   c->n_tiles = (int32_t)nt;
   c->n_tasks = (int32_t)nk;
+  c->max_world_tiles = (int32_t)max_wt;
   if ((rc = dalloc(c, &c->d_tile_world, (size_t)std::max<int64_t>(nt, 1)))) return rc;
   if ((rc = dalloc(c, &c->d_world_tile0, (size_t)n_worlds + 1))) return rc;
   if ((rc = dalloc(c, &c->d_task_world, (size_t)std::max<int64_t>(nk, 1)))) return rc;

@@ -107,6 +107,7 @@ constexpr int kAwakeSlots = 256;
 
 struct PairSpace {
   int32_t n_tiles;
+  int32_t max_world_tiles = 0;  // the largest world's tile count (host side: picks the scan's block size)
   const int32_t* tile_world;
   const int32_t* world_tile0;
   const int32_t* world_q;     // pair slots of each world (explicit list)

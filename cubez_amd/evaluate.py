@@ -2,6 +2,7 @@
 
     python -m cubez_amd.evaluate surrogate   # fresh seeds 1000-1015, batched worlds
     python -m cubez_amd.evaluate towers      # stacked resting contacts, 16384 worlds
+    python -m cubez_amd.evaluate broadphase  # one large world: all pairs vs sort-and-sweep
 
 surrogate: 16 fresh seeds x 256 batched worlds (4096 worlds), 600 steps at
   1/60.  Held-out accuracy of the network on the exact resolver's contacts of
@@ -13,6 +14,11 @@ surrogate: 16 fresh seeds x 256 batched worlds (4096 worlds), 600 steps at
 towers: 4-16 towers of 4-12 cubes per world, 16384 worlds, 1200 steps at
   1/120: the same paths; fallback rate, penetration left, and the drift of
   every tower's top cube, from its start and against the FP64 exact run.
+broadphase: one world of n cubes (n = 4000, 16000, 40000) spread over
+  x, z in +-2 sqrt(n): the all-pairs narrowphase against the broadphase
+  (cbz_narrowphase_broad) on the same state, wall time per call (mean of 5,
+  host round trips included), the pairs each checks, and that both give the
+  same contact count.
 
 Each prints one JSON object."""
 import argparse
@@ -124,9 +130,39 @@ def eval_towers(model_path=None, seed=0, n_worlds=16384, steps=None, tol=surroga
     return res
 
 
+def eval_broadphase(sizes=(4000, 16000, 40000), settle_steps=60, reps=5):
+    """All pairs against sort-and-sweep on one large world.  The smallest
+    world is first stepped until its cubes land (so it has contacts); the
+    larger ones are checked as generated."""
+    res = {"worlds": []}
+    for n in sizes:
+        sc = scenes.batched(4, 1, n, n)
+        with World(sc) as w:
+            if n <= 4000:
+                w.step(sc.dt, settle_steps, sc.iter_mult)
+            w.integrate(sc.dt)
+            w.narrowphase_broad()  # sizes the broadphase buffers
+            w.narrowphase()
+            w.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                total_all = w.narrowphase()
+            w.synchronize()
+            t_all = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                total_bp, checked = w.narrowphase_broad()
+            w.synchronize()
+            t_bp = (time.perf_counter() - t0) / reps
+        res["worlds"].append({"bodies": n, "all_pairs_ms": 1000 * t_all, "broadphase_ms": 1000 * t_bp,
+                              "pairs_all": n * n, "pairs_checked": int(checked), "contacts": int(total_all),
+                              "same_contact_count": bool(total_all == total_bp)})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("which", choices=("surrogate", "towers"))
+    ap.add_argument("which", choices=("surrogate", "towers", "broadphase"))
     ap.add_argument("--model", default=None)
     ap.add_argument("--worlds", type=int, default=None)
     ap.add_argument("--steps", type=int, default=None)
@@ -138,8 +174,10 @@ def main():
         if a.steps:
             kw["steps"] = a.steps
         out = eval_surrogate(a.model, **kw)
-    else:
+    elif a.which == "towers":
         out = eval_towers(a.model, n_worlds=a.worlds or 16384, steps=a.steps)
+    else:
+        out = eval_broadphase()
     print(json.dumps(out), flush=True)
 
 
