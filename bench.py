@@ -215,11 +215,24 @@ def run_surrogate(World, sc, model, steps, warmup, tol, polish, zero=False):
     fb = w.step_sur(sc.dt, steps, sc.iter_mult, tol, polish, zero)
     w.synchronize()
     wall = time.perf_counter() - t0
+    # host-buffer transfers across the boundary: the scene upload (with the
+    # derived data), then, after the profiled steps, the state read-back
+    t1 = time.perf_counter()
     w.load(sc)
+    w.synchronize()
+    upload_s = time.perf_counter() - t1
     w.set_profiling(True)
     w.step_sur(sc.dt, steps, sc.iter_mult, tol, polish, zero)
     w.set_profiling(False)
     ms, cnt = w.sur_phase_times()
+    t1 = time.perf_counter()
+    for k in ("pos", "quat", "vel", "rot"):
+        w.get(k)
+    download_s = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    for k in ("pos", "quat"):
+        w.get(k)
+    pos_quat_s = time.perf_counter() - t1
     w.close()
     n_res, contacts, fallbacks, vel_iters = (int(x) for x in cnt)
     names = ("position_phase", "features", "network", "per_body_sums", "velocity_phase")
@@ -227,6 +240,8 @@ def run_surrogate(World, sc, model, steps, warmup, tol, polish, zero=False):
         "value": sc.n_bodies * steps / wall,
         "ms_per_step": 1000.0 * wall / steps,
         "fallback_rate": float(fb.sum()) / (steps * sc.n_worlds),
+        "transfer": {"wall_s": wall, "upload_s": upload_s, "download_state_s": download_s,
+                     "download_pos_quat_s": pos_quat_s},
         "resolver_ms_per_step": {k: round(ms[i] / max(n_res, 1), 5) for i, k in enumerate(names)},
         "contacts_per_step": contacts / max(n_res, 1),
         "velocity_iterations_per_step": vel_iters / max(n_res, 1),
@@ -352,6 +367,15 @@ def main():
                       "achieved_tflops": tfl, "peak_tflops": 2500.0, "mfma_frac": tfl / 2500.0,
                       "bound": "mfma", "kernel": "k_mlp_forward (fused 4-layer forward)"}
         out["zero_prediction"] = {k: zero[k] for k in ("value", "ms_per_step", "fallback_rate")}
+        tr = sur["transfer"]
+        out["pcie_inclusive"] = {
+            "job_value": nb * args.steps / (tr["upload_s"] + tr["wall_s"] + tr["download_state_s"]),
+            "job": "scene upload from host buffers + the timed steps + read-back of pos/quat/vel/rot",
+            "per_step_readback_value": nb / (tr["wall_s"] / args.steps + tr["download_pos_quat_s"]),
+            "per_step_readback": "pos and quat read back to the host after every step",
+            "unit": "body-steps/s", "upload_ms": 1000 * tr["upload_s"],
+            "download_state_ms": 1000 * tr["download_state_s"], "download_pos_quat_ms": 1000 * tr["download_pos_quat_s"],
+        }
     else:
         out["contacts_per_step"] = exact32["contacts_per_step"]
         out["contacts_per_s"] = exact32["contacts_per_s"]
