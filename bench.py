@@ -4,23 +4,40 @@
 Workload: 4096 independent worlds, each a ground plane and n ~ U{8..256}
 cubes (the batched-worlds configuration), advanced with the reference's
 step -- Integrate + CalculateDerivedData, the all-pairs narrowphase,
-ResolveContacts(4 x contacts) -- for K steps at dt = 1/60 from the same
-initial state.  A step is one pass of that hot path over all worlds.  The
-metric is body-steps/s (bodies x steps / seconds) with the state resident in
-HBM, for the learned-surrogate resolver with exact fallback (FP32 state,
-tolerance 0.06 m/s of residual closing speed = 1 mm of penetration per
-step), next to the exact FP64 GPU resolver it is compared with.
+ResolveContacts(4 x contacts) -- at dt = 1/60.  A step is one pass of that
+hot path over all worlds.  The metric is body-steps/s (bodies x steps /
+seconds) with the state resident in HBM, for the learned-surrogate resolver
+with exact fallback (FP32 state, tolerance 0.06 m/s of residual closing
+speed = 1 mm of penetration per step), next to the exact FP64 GPU resolver
+it is compared with.
+
+The timed window: with --steps 600 (the default) the whole workload, steps
+0..599 from the seeded initial state.  With fewer steps, a window of the
+settled regime: the exact FP64 step runs to step --window-start (300) in the
+untimed region, its state is checkpointed, and every leg (exact FP64, exact
+FP32, surrogate, zero prediction) restores that same state, runs --warmup
+untimed steps, restores it again and times --steps steps.  The first steps
+from the initial state are free fall with 2 % of the settled contact count,
+so a short window from step 0 would time a regime the workload barely spends
+time in.
 
 Prints one JSON line.  Besides the metric it reports:
   exact_f64, exact_f32   the exact resolver (hipGraph-replayed step)
   zero_prediction        the surrogate's polish started from a zero
                          prediction: the same method without the network
   fallback_rate          worlds re-solved by the plain method / world-steps
-  mlp                    the network's forward GEMMs: FLOP, time, MFMA share
-  roofline               K1 (integrate + derived data, HBM-bound) from its
-                         average launch time measured with HIP events
+  error                  penetration left after the window (narrowphase of
+                         the final state: mean, p99, max) for the surrogate
+                         and both exact paths, and the surrogate's final
+                         positions against each exact path's
+  kernels                HIP-event times per step of every phase
+  roofline               the dominant kernel of the surrogate step, its
+                         algorithmic bytes over its event-timed duration
+  k1, k2, mlp            the rooflines of integrate, the narrowphase and the
+                         network beside it
   cpu_baseline           the FP64 oracle (a C restatement of the Go
-                         reference) on a bounded sample of the same worlds
+                         reference) on a bounded sample of the same worlds,
+                         on all host cores and on one
 """
 import argparse
 import json
@@ -29,33 +46,45 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "body-steps/s, 4096 worlds: surrogate vs exact GPU resolver at ≤1e-3 m error"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector, FMA counted as two
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 SAT_AXIS_FLOP = 75
 PLANE_CHECK_FLOP = 40
 K1_LAUNCHES = 50
+DEFAULT_WINDOW_START = 300
 
-# Algorithmic bytes of K1 per body-step: an integrated body reads 29 reals
-# (pos 3, quat 4, vel 3, rot 3, acc 3, inv_mass 1, iit_body 9, motion 1, the
-# two Pow factors) + its flag byte and writes 50 reals (pos, quat, vel, rot,
-# last_acc, transform 12, iit_world 9, collider transform 12, motion) + the
-# flag byte; a sleeping body reads its flag byte and transform (12) and
+# Algorithmic bytes of K1 per body-step: an integrated body reads 38 reals
+# (pos 3, quat 4, vel 3, rot 3, acc 3, inv_mass 1, iit_body 9, the old
+# iit_world 9 that angularAcceleration = iitWorld * torqueAccum reads, motion
+# 1, the two Pow factors) + its flag byte and writes 50 reals (pos, quat, vel,
+# rot, last_acc, transform 12, iit_world 9, collider transform 12, motion) +
+# the flag byte; a sleeping body reads its flag byte and transform (12) and
 # writes the collider transform (12).  Reals are 8 bytes (FP64) or 4 (FP32).
+K1_READ_REALS, K1_WRITE_REALS = 38, 50
+
+# Algorithmic bytes of the resolver per resolve, for its roofline: every
+# contact record read once (ids 8 B, point 3, normal 3, pen 1) and its
+# resolver internals written once (pen 1, c2w 9, rel 6, cvel 3, ddv 1), and
+# every body of a world with contacts read once (pos 3, quat 4, vel 3, rot 3,
+# inv_mass 1, iit_world 9, last_acc 3, flags 1 B) and written once (pos 3,
+# quat 4, vel 3, rot 3, motion 1, flags 1 B).  The serial iterations re-read
+# what an iteration changed; those re-reads are not algorithmic.
+RES_CONTACT_READ, RES_CONTACT_WRITE = 7, 20
+RES_BODY_READ, RES_BODY_WRITE = 26, 14
 
 
-_SCENE = None
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_integrate_hbm_traffic.md")
-
-
-def traffic_ratio(precision):
+def traffic_ratio(precision, path=os.path.join(ROOT, "profiles", "k_integrate_hbm_traffic.md")):
     """HBM traffic / algorithmic bytes of K1, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE measurement (copy-calibrated), or None."""
     try:
-        for line in open(TRAFFIC_PROFILE):
+        for line in open(path):
             if line.startswith(f"traffic_over_algorithmic_{precision}:"):
                 return float(line.split(":")[1])
     except OSError:
@@ -63,52 +92,29 @@ def traffic_ratio(precision):
     return None
 
 
+# --- CPU baseline ------------------------------------------------------------
+
+_SCENE = None
+
+
 def _cpu_world(args):
     """Oracle run of one world (worker process; the scene is inherited)."""
     from oracle.cbz_oracle import OracleWorld
 
-    w, steps = args
+    w, steps, window_start = args
     sc = _SCENE
     o = OracleWorld(sc, w)
     o.calculate_derived()
     t0 = time.perf_counter()
-    o.step(sc.dt, steps, sc.iter_mult)
-    return o.n * steps, time.perf_counter() - t0, o.counters()
-
-
-def cpu_baseline(scene, steps, budget_s):
-    """Time the oracle on an evenly spaced sample of worlds, all steps, with
-    one process per host core (up to 16)."""
-    import multiprocessing as mp
-
-    from oracle import cbz_oracle
-
-    global _SCENE
-    _SCENE = scene
-    n_worlds = scene.n_worlds
-    cbz_oracle.load()
-    cores = max(1, min(16, os.cpu_count() or 1))
-    # a first world gives the per-body-step cost to size the sample
-    probe = _cpu_world((0, steps))
-    per_bs = probe[1] / max(probe[0], 1)
-    mean_n = 132.0
-    n_sample = int(max(cores, min(n_worlds, budget_s * cores / max(per_bs * mean_n * steps, 1e-9))))
-    stride = max(1, n_worlds // n_sample)
-    worlds = list(range(0, n_worlds, stride))[:n_sample]
-    t0 = time.perf_counter()
-    with mp.get_context("fork").Pool(cores) as pool:
-        res = pool.map(_cpu_world, [(w, steps) for w in worlds])
-    wall = time.perf_counter() - t0
-    bs = sum(r[0] for r in res)
-    counters = {k: sum(r[2][k] for r in res) for k in res[0][2]} if res else {}
-    return {
-        "value": bs / wall,
-        "unit": "body-steps/s",
-        "cores": cores,
-        "kind": "port",
-        "sample": f"{len(worlds)} of {n_worlds} worlds (every {stride}th), all {steps} steps, "
-                  f"{bs} body-steps in {wall:.1f} s wall on {cores} processes; host: {_host_cpu()}",
-    }, counters, bs
+    first = min(window_start, steps)
+    if first:
+        o.step(sc.dt, first, sc.iter_mult)
+    before = o.counters()
+    if steps > first:
+        o.step(sc.dt, steps - first, sc.iter_mult)
+    after = o.counters()
+    # the work counters of the window's steps alone
+    return o.n * steps, time.perf_counter() - t0, {k: after[k] - before[k] for k in after}, o.n * (steps - first)
 
 
 def _host_cpu():
@@ -125,50 +131,133 @@ def _host_cpu():
     return f"nproc {os.cpu_count()}, {model}"
 
 
-def narrowphase_rate(counters, sample_body_steps, n_bodies, kernels):
-    """K2's FP64 VALU rate: the reference's narrowphase work on the same
-    steps, from the oracle's op counters on the CPU sample (scaled to the
-    whole scene by body-steps), over the FP64 narrowphase kernels' time per
-    step.  Work model (per unit, the reference's arithmetic in
-    colliders.go:445-477 tryAxis / penetrationOnAxis, :762-770
-    transformToAxis, :308-367 the half-space check): 75 flops per SAT axis
-    tested (normalise, two projections, the distance), 40 per body-plane check;
-    contact generation is not counted, so the rate is a lower bound."""
+def cpu_baseline(scene, steps, budget_s, window_start=0):
+    """Time the oracle on an evenly spaced sample of worlds, all `steps`
+    steps from the initial state, with one process per host core (up to 16);
+    the single-core rate is the sum of the per-world CPU times.  Also
+    returns the oracle's work counters over steps window_start..steps-1 and
+    the body-steps they cover."""
+    import multiprocessing as mp
+
+    from oracle import cbz_oracle
+
+    global _SCENE
+    _SCENE = scene
+    n_worlds = scene.n_worlds
+    cbz_oracle.load()
+    cores = max(1, min(16, os.cpu_count() or 1))
+    # a first world gives the per-body-step cost to size the sample
+    probe = _cpu_world((0, steps, window_start))
+    per_bs = probe[1] / max(probe[0], 1)
+    mean_n = 132.0
+    n_sample = int(max(cores, min(n_worlds, budget_s * cores / max(per_bs * mean_n * steps, 1e-9))))
+    stride = max(1, n_worlds // n_sample)
+    worlds = list(range(0, n_worlds, stride))[:n_sample]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_world, [(w, steps, window_start) for w in worlds])
+    wall = time.perf_counter() - t0
+    bs = sum(r[0] for r in res)
+    cpu_s = sum(r[1] for r in res)
+    counters = {k: sum(r[2][k] for r in res) for k in res[0][2]} if res else {}
+    window_bs = sum(r[3] for r in res)
+    return {
+        "value": bs / wall,
+        "unit": "body-steps/s",
+        "cores": cores,
+        "value_1core": bs / cpu_s,
+        "kind": "port",
+        "sample": f"{len(worlds)} of {n_worlds} worlds (every {stride}th), steps 0..{steps - 1} from the initial "
+                  f"state, {bs} body-steps in {wall:.1f} s wall on {cores} processes ({cpu_s:.1f} s of CPU time, "
+                  f"which gives value_1core); host: {_host_cpu()}",
+    }, counters, window_bs
+
+
+def narrowphase_rate(counters, sample_body_steps, n_bodies, ms_per_step):
+    """K2's FP64 VALU rate: the reference's narrowphase work per step, from
+    the oracle's op counters on the CPU sample (scaled to the whole scene by
+    body-steps), over the FP64 narrowphase kernels' time per step.  Work
+    model (the reference's arithmetic in colliders.go:445-477 tryAxis /
+    penetrationOnAxis, :762-770 transformToAxis, :308-367 the half-space
+    check): 75 flops per SAT axis tested, 40 per body-plane check; contact
+    generation is not counted, so the rate is a lower bound."""
     scale = n_bodies / max(sample_body_steps, 1)
     per_step = {k: counters[k] * scale for k in ("pairs", "plane_checks", "sat_axes", "sat_skipped", "contacts")}
     flop = SAT_AXIS_FLOP * per_step["sat_axes"] + PLANE_CHECK_FLOP * per_step["plane_checks"]
-    ms = kernels["np_count"] + kernels["np_scans"] + kernels["np_emit"]
-    tf = flop / (ms / 1000.0) / 1e12 if ms > 0 else 0.0
+    tf = flop / (ms_per_step / 1000.0) / 1e12 if ms_per_step > 0 else 0.0
     return {
-        "kernels": "k_np_cull + k_np_eval + scans + k_np_emit_hits (FP64 state, same 600 steps)",
+        "kernels": "k_np_cull + k_np_eval + scans + k_np_emit_hits (FP64 state)",
         "bound": "fp64-valu", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": tf / FP64_PEAK_TFLOPS, "frac_of_unfused_peak": tf / (FP64_PEAK_TFLOPS / 2),
-        "flop_per_step": flop, "ms_per_step": ms, "reference_work_per_step": per_step,
+        "flop_per_step": flop, "ms_per_step": ms_per_step, "reference_work_per_step": per_step,
+        "work_window": "the oracle's work counters on the CPU sample over the timed window's steps, scaled "
+                       "to the scene by body-steps",
         "flop_model": f"{SAT_AXIS_FLOP} per SAT axis tested + {PLANE_CHECK_FLOP} per body-plane check "
                       "(contact generation not counted: a lower bound)",
     }
 
 
-def run_path(World, sc, precision, steps, warmup):
-    """Exact resolver: warm up, then time `steps` steps from the initial
-    state (graph replay), then the same steps again with HIP events between
-    kernels."""
+# --- GPU legs ----------------------------------------------------------------
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _start(w, sc, ck):
+    if ck is None:
+        w.load(sc)
+    else:
+        w.restore(ck)
+    w.synchronize()
+
+
+def residual_penetration(w):
+    """Penetration left in the current state: a narrowphase of it (no
+    integration), every contact's depth: count, mean, p99 and max (m)."""
+    w.narrowphase()
+    p = w.contacts(-1, internals=False)["pen"]
+    if not len(p):
+        return {"contacts": 0}
+    return {"contacts": int(len(p)), "mean_m": float(p.mean()), "p99_m": float(np.percentile(p, 99)),
+            "max_m": float(p.max())}
+
+
+def settle(World, sc, start):
+    """The exact FP64 step from the initial state to step `start`: the
+    checkpoint every leg starts its window from."""
+    if start <= 0:
+        return None, 0.0
+    t0 = time.perf_counter()
+    with World(sc, precision="f64") as w:
+        w.step(sc.dt, start, sc.iter_mult)
+        w.synchronize()
+        ck = w.checkpoint()
+    return ck, time.perf_counter() - t0
+
+
+def run_path(World, sc, ck, precision, steps, warmup):
+    """Exact resolver: warm up from the window's start, restore it, time
+    `steps` steps (graph replay), then the same steps again with HIP events
+    between kernels."""
     w = World(sc, precision=precision)
+    _start(w, sc, ck)
     if warmup > 0:
         w.step(sc.dt, warmup, sc.iter_mult)
-    w.load(sc)
-    w.synchronize()
+        _start(w, sc, ck)
     t0 = time.perf_counter()
     w.step(sc.dt, steps, sc.iter_mult)
     w.synchronize()
     wall = time.perf_counter() - t0
-    w.load(sc)
+    final_pos = w.get("pos")
+    resid = residual_penetration(w)
+    _start(w, sc, ck)
     w.set_profiling(True)
     w.step(sc.dt, steps, sc.iter_mult)
     w.set_profiling(False)
     ms, cnt = w.phase_times()
-    # K1 alone: back-to-back launches on the initial state between two events
-    w.load(sc)
+    # K1 alone: back-to-back launches between two events, on the window's
+    # start state
+    _start(w, sc, ck)
     k1_ms, k1_bodies = w.integrate_timed(sc.dt, K1_LAUNCHES)
     per_cu, n_cu = w.resolver_occupancy()
     w.close()
@@ -181,12 +270,12 @@ def run_path(World, sc, precision, steps, warmup):
         "ms_per_step": 1000.0 * wall / steps,
         "kernels": {k: round(ms[i] / sp, 5) for i, k in enumerate(names)},
         "steps_profiled": sp,
-        "integrated": integrated,
+        "integrated_per_step": integrated / sp,
         "contacts_per_step": contacts / sp,
-        "contacts_per_s": contacts / (ms[5] / 1000.0) if ms[5] > 0 else None,
+        "contacts_per_s": contacts / sp * steps / wall,
         "iters_per_step": iters / sp,
-        # the resolver's latency measures (its phase has no roofline): one
-        # wavefront per world, worlds_per_cu of them resident per CU
+        # the resolver's latency measures: one wavefront per world,
+        # worlds_per_cu of them resident per CU
         "resolver": {
             "iterations_per_world_per_step": iters / sp / sc.n_worlds,
             "worlds_per_cu": per_cu,
@@ -197,49 +286,55 @@ def run_path(World, sc, precision, steps, warmup):
         },
         "k1_ms_per_launch": k1_ms,
         "k1_bodies_per_launch": k1_bodies / K1_LAUNCHES,
+        "residual_penetration": resid,
+        "_final_pos": final_pos,
     }
 
 
-def run_surrogate(World, sc, model, steps, warmup, tol, polish, zero=False):
-    """Surrogate resolver (FP32 state): timed steps from the initial state,
+def run_surrogate(World, sc, ck, model, steps, warmup, tol, polish, zero=False):
+    """Surrogate resolver (FP32 state): timed steps from the window's start,
     then the same steps again with HIP events around its phases."""
     params, norm = model
     w = World(sc, precision="f32")
     w.sur_set_norm(norm)
     w.sur_set_params(params)
+    _start(w, sc, ck)
     if warmup > 0:
         w.step_sur(sc.dt, warmup, sc.iter_mult, tol, polish, zero)
-    w.load(sc)
-    w.synchronize()
+        _start(w, sc, ck)
     t0 = time.perf_counter()
     fb = w.step_sur(sc.dt, steps, sc.iter_mult, tol, polish, zero)
     w.synchronize()
     wall = time.perf_counter() - t0
-    # host-buffer transfers across the boundary: the scene upload (with the
-    # derived data), then, after the profiled steps, the state read-back
-    t1 = time.perf_counter()
-    w.load(sc)
-    w.synchronize()
-    upload_s = time.perf_counter() - t1
-    w.set_profiling(True)
-    w.step_sur(sc.dt, steps, sc.iter_mult, tol, polish, zero)
-    w.set_profiling(False)
-    ms, cnt = w.sur_phase_times()
+    out = {"value": sc.n_bodies * steps / wall, "ms_per_step": 1000.0 * wall / steps,
+           "fallback_rate": float(fb.sum()) / (steps * sc.n_worlds)}
+    if zero:
+        w.close()
+        return out
+    # read-back of the state across the host boundary, after the timed steps
     t1 = time.perf_counter()
     for k in ("pos", "quat", "vel", "rot"):
         w.get(k)
     download_s = time.perf_counter() - t1
     t1 = time.perf_counter()
-    for k in ("pos", "quat"):
-        w.get(k)
+    final_pos = w.get("pos")
+    w.get("quat")
     pos_quat_s = time.perf_counter() - t1
+    resid = residual_penetration(w)
+    # scene upload from host buffers (with the derived data)
+    t1 = time.perf_counter()
+    w.load(sc)
+    w.synchronize()
+    upload_s = time.perf_counter() - t1
+    _start(w, sc, ck)
+    w.set_profiling(True)
+    w.step_sur(sc.dt, steps, sc.iter_mult, tol, polish, zero)
+    w.set_profiling(False)
+    ms, cnt = w.sur_phase_times()
     w.close()
     n_res, contacts, fallbacks, vel_iters = (int(x) for x in cnt)
     names = ("position_phase", "features", "network", "per_body_sums", "velocity_phase")
-    return {
-        "value": sc.n_bodies * steps / wall,
-        "ms_per_step": 1000.0 * wall / steps,
-        "fallback_rate": float(fb.sum()) / (steps * sc.n_worlds),
+    out.update({
         "transfer": {"wall_s": wall, "upload_s": upload_s, "download_state_s": download_s,
                      "download_pos_quat_s": pos_quat_s},
         "resolver_ms_per_step": {k: round(ms[i] / max(n_res, 1), 5) for i, k in enumerate(names)},
@@ -247,7 +342,15 @@ def run_surrogate(World, sc, model, steps, warmup, tol, polish, zero=False):
         "velocity_iterations_per_step": vel_iters / max(n_res, 1),
         "network_ms_total": ms[2],
         "contacts_total": contacts,
-    }
+        "residual_penetration": resid,
+        "_final_pos": final_pos,
+    })
+    return out
+
+
+def deviation(a, b):
+    d = np.linalg.norm(a - b, axis=1)
+    return {"mean_m": float(d.mean()), "p99_m": float(np.percentile(d, 99)), "max_m": float(d.max())}
 
 
 def probe_k1(World, sc, launches=20):
@@ -276,6 +379,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--window-start", type=int, default=None,
+                    help="first step of the timed window (default: 0 for 600 or more steps, else 300)")
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--model", default=None, help="surrogate weights (default: the packaged model)")
@@ -306,33 +411,33 @@ def main():
               "use --exact-only to time the exact resolver alone", file=sys.stderr)
         sys.exit(1)
     tol, polish = surrogate.DEFAULT_TOL, surrogate.DEFAULT_POLISH
+    start = args.window_start if args.window_start is not None else (0 if args.steps >= 600 else DEFAULT_WINDOW_START)
 
-    exact32 = run_path(World, sc, "f32", args.steps, args.warmup)
-    exact64 = run_path(World, sc, "f64", args.steps, args.warmup)
+    _log(f"bench: settling to step {start}")
+    ck, settle_s = settle(World, sc, start)
+    _log("bench: exact FP32")
+    exact32 = run_path(World, sc, ck, "f32", args.steps, args.warmup)
+    _log("bench: exact FP64")
+    exact64 = run_path(World, sc, ck, "f64", args.steps, args.warmup)
     sur = zero = None
     if use_sur:
         model = surrogate.load(model_path)
-        sur = run_surrogate(World, sc, model, args.steps, args.warmup, tol, polish)
-        zero = run_surrogate(World, sc, model, args.steps, args.warmup, tol, polish, zero=True)
+        _log("bench: surrogate")
+        sur = run_surrogate(World, sc, ck, model, args.steps, args.warmup, tol, polish)
+        _log("bench: zero prediction")
+        zero = run_surrogate(World, sc, ck, model, args.steps, args.warmup, tol, polish, zero=True)
 
-    kern = exact32["kernels"]
     rs = 4
-    # K1 roofline (FP32, the kernel the surrogate path launches): algorithmic
-    # bytes per launch / average launch time of K1_LAUNCHES back-to-back
-    # launches on the workload's initial state, timed with HIP events on the
-    # library's stream
-    k1_awake = 29 * rs + 1 + 50 * rs + 1
-    k1_sleep = 1 + 12 * rs + 12 * rs
-    aw = exact32["k1_bodies_per_launch"]
-    k1_bytes = aw * k1_awake + (nb - aw) * k1_sleep
-    achieved = k1_bytes / (exact32["k1_ms_per_launch"] / 1000.0) / 1e9
-    main_path = sur if use_sur else exact32
-    workload = (f"batched worlds: {args.worlds} worlds, plane + n~U{{8..256}} cubes each ({nb} bodies), "
-                f"first {args.steps} steps at dt=1/60 from the seeded initial state, all-pairs narrowphase, "
-                f"maxIterations=4*C, FP32 state with FP64 SAT and contact basis, ")
+    window = (f"steps {start}..{start + args.steps - 1}" + (" (the whole workload)" if start == 0 and args.steps >= 600
+                                                               else f" (a window of the settled regime: the state "
+                                                                    f"of the exact FP64 run at step {start}, loaded "
+                                                                    f"into every leg)"))
+    workload = (f"batched worlds: {args.worlds} worlds, plane + n~U{{8..256}} cubes each ({nb} bodies), {window} at "
+                f"dt=1/60, all-pairs narrowphase, maxIterations=4*C, FP32 state with FP64 SAT and contact basis, ")
     workload += (f"resolver: exact position phase + BF16 MLP surrogate for the velocity phase, polished by "
                  f"the exact worst-first loop to velocityEpsilon+{tol} m/s within {polish}*C iterations, "
                  f"else the world is re-solved by the plain method" if use_sur else "resolver: exact")
+    main_path = sur if use_sur else exact32
     out = {
         "metric": METRIC,
         "value": main_path["value"],
@@ -348,24 +453,24 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": workload, "worlds": args.worlds, "bodies": nb, "seed": args.seed,
+            "window_start": start, "window_steps": args.steps,
             "resolver": "surrogate+fallback" if use_sur else "exact", "tolerance_m_per_s": tol,
             "polish_per_contact": polish, "network": "40-256-256-256-12 SiLU, BF16 MFMA, FP32 accumulate",
         },
+        "settle_s": settle_s,
     }
+    kern32 = exact32["kernels"]
+    # per-phase kernel times of the surrogate step (ms/step): integrate and
+    # narrowphase are the FP32 exact path's kernels on the same window
+    phases = {"integrate": kern32["integrate"], "narrowphase": kern32["np_count"] + kern32["np_scans"] + kern32["np_emit"]}
     if use_sur:
         out["speedup_vs_exact_f64"] = sur["value"] / exact64["value"]
         out["speedup_vs_exact_f32"] = sur["value"] / exact32["value"]
         out["fallback_rate"] = sur["fallback_rate"]
-        out["velocity_iterations_per_step"] = {"surrogate": sur["velocity_iterations_per_step"],
-                                               "zero_prediction": zero["velocity_iterations_per_step"]}
-        out["surrogate_resolver_ms_per_step"] = sur["resolver_ms_per_step"]
+        out["velocity_iterations_per_step"] = {"surrogate": sur["velocity_iterations_per_step"]}
         out["contacts_per_step"] = sur["contacts_per_step"]
         out["contacts_per_s"] = sur["contacts_per_step"] * sur["value"] / nb
-        flop = 2 * sum(i * o for i, o in surrogate.LAYERS)  # 288,768 per contact
-        tfl = flop * sur["contacts_total"] / (sur["network_ms_total"] / 1000.0) / 1e12 if sur["network_ms_total"] else 0
-        out["mlp"] = {"flop_per_contact": flop, "ms_per_step": sur["resolver_ms_per_step"]["network"],
-                      "achieved_tflops": tfl, "peak_tflops": 2500.0, "mfma_frac": tfl / 2500.0,
-                      "bound": "mfma", "kernel": "k_mlp_forward (fused 4-layer forward)"}
+        phases.update(sur["resolver_ms_per_step"])
         out["zero_prediction"] = {k: zero[k] for k in ("value", "ms_per_step", "fallback_rate")}
         tr = sur["transfer"]
         out["pcie_inclusive"] = {
@@ -376,28 +481,84 @@ def main():
             "unit": "body-steps/s", "upload_ms": 1000 * tr["upload_s"],
             "download_state_ms": 1000 * tr["download_state_s"], "download_pos_quat_ms": 1000 * tr["download_pos_quat_s"],
         }
+        out["error"] = {
+            "bar": f"tolerance {tol} m/s of residual closing speed per contact = {tol / 60.0 * 1000:.1f} mm of "
+                   "extra penetration per 1/60 s step; worlds above it are re-solved exactly",
+            "residual_penetration": {"surrogate": sur["residual_penetration"],
+                                     "exact_f32": exact32["residual_penetration"],
+                                     "exact_f64": exact64["residual_penetration"]},
+            "surrogate_minus_exact_f64_mean_penetration_m": sur["residual_penetration"].get("mean_m", 0.0)
+                                                           - exact64["residual_penetration"].get("mean_m", 0.0),
+            "position_deviation_after_window": {
+                "surrogate_vs_exact_f32": deviation(sur["_final_pos"], exact32["_final_pos"]),
+                "surrogate_vs_exact_f64": deviation(sur["_final_pos"], exact64["_final_pos"]),
+                "exact_f32_vs_exact_f64": deviation(exact32["_final_pos"], exact64["_final_pos"]),
+            },
+        }
     else:
         out["contacts_per_step"] = exact32["contacts_per_step"]
         out["contacts_per_s"] = exact32["contacts_per_s"]
-    out["exact_f64"] = {k: exact64[k] for k in ("value", "ms_per_step", "kernels", "iters_per_step", "resolver")}
-    out["exact_f32"] = {k: exact32[k] for k in ("value", "ms_per_step", "kernels", "iters_per_step", "resolver")}
-    out["kernels_ms_per_step"] = kern
-    out["dominant_kernel"] = max(("integrate", "np_count", "np_scans", "np_emit", "resolve"), key=lambda k: kern[k])
+    out["kernels_ms_per_step"] = {k: round(v, 5) for k, v in phases.items()}
+    for k, p in (("exact_f64", exact64), ("exact_f32", exact32)):
+        out[k] = {x: p[x] for x in ("value", "ms_per_step", "kernels", "iters_per_step", "contacts_per_step",
+                                     "resolver")}
+
+    # K1 roofline (FP32, the kernel the surrogate path launches): algorithmic
+    # bytes per launch / average launch time of K1_LAUNCHES back-to-back
+    # launches on the window's start state, timed with HIP events on the
+    # library's stream
+    aw = exact32["k1_bodies_per_launch"]
+    k1_bytes = aw * (K1_READ_REALS * rs + 1 + K1_WRITE_REALS * rs + 1) + (nb - aw) * (1 + 24 * rs)
+    k1_gbs = k1_bytes / (exact32["k1_ms_per_launch"] / 1000.0) / 1e9
     ratio = traffic_ratio("f32")
-    out["roofline"] = {
-        "kernel": "k_integrate (K1: Integrate + CalculateDerivedData)",
-        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": ratio * k1_bytes if ratio else None,
-        "bytes_per_launch": k1_bytes, "ms_per_launch": exact32["k1_ms_per_launch"],
-        "measured_on": f"{K1_LAUNCHES} back-to-back launches on the initial state (every body awake)",
-        "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE of k_integrate<float>, copy-calibrated, "
-                           f"traffic/algorithmic = {ratio} ({os.path.relpath(TRAFFIC_PROFILE, ROOT)})")
-        if ratio else None,
+    out["k1"] = {
+        "kernel": "k_integrate<float> (Integrate + CalculateDerivedData)", "bound": "hbm", "achieved": k1_gbs,
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k1_gbs / HBM_PEAK_GBS,
+        "traffic": ratio * k1_bytes if ratio else None, "bytes_per_launch": k1_bytes,
+        "ms_per_launch": exact32["k1_ms_per_launch"], "awake_bodies": aw,
     }
+    if use_sur:
+        flop = 2 * sum(i * o for i, o in surrogate.LAYERS)  # 288,768 per contact
+        tfl = flop * sur["contacts_total"] / (sur["network_ms_total"] / 1000.0) / 1e12 if sur["network_ms_total"] else 0
+        out["mlp"] = {"kernel": "k_mlp_forward (fused 4-layer forward)", "bound": "mfma", "flop_per_contact": flop,
+                      "ms_per_step": phases["network"], "achieved": tfl, "peak": BF16_PEAK_TFLOPS,
+                      "unit": "TFLOP/s", "frac": tfl / BF16_PEAK_TFLOPS}
+    # the dominant kernel of the headline step and its roofline
+    if use_sur:
+        dom = max(("position_phase", "velocity_phase", "network", "narrowphase", "integrate", "features"),
+                  key=lambda k: phases[k])
+    else:
+        dom = "resolve"
+    C = main_path["contacts_per_step"]
+    res_bytes = C * (RES_CONTACT_READ * rs + 8 + RES_CONTACT_WRITE * rs) + nb * (RES_BODY_READ * rs + 1 +
+                                                                                  RES_BODY_WRITE * rs + 1)
+    if dom in ("position_phase", "velocity_phase", "resolve"):
+        ms = phases[dom] if use_sur else kern32["resolve"]
+        ach = res_bytes / (ms / 1000.0) / 1e9
+        roof = {"kernel": {"position_phase": "k_resolve_f32<plain> (prepareContacts + adjustPositions)",
+                           "velocity_phase": "k_resolve_f32<polish> (adjustVelocities from the network's changes, "
+                                             "with the exact fallback)",
+                           "resolve": "k_resolve_f32<plain>"}[dom],
+                "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": None, "bytes_per_launch": res_bytes, "ms_per_launch": ms,
+                "note": "serial worst-first iterations per world: latency-bound, see exact_*.resolver"}
+    elif dom == "network":
+        roof = dict(out["mlp"])
+        roof["traffic"] = None
+    elif dom == "narrowphase":
+        roof = {"kernel": "narrowphase (k_np_cull + k_np_eval + scans + k_np_emit_hits, FP64 SAT)",
+                "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None, "ms_per_launch": phases[dom]}
+    else:
+        roof = dict(out["k1"])
+    roof["phase"] = dom
+    out["roofline"] = roof
     if not args.no_cpu:
-        out["cpu_baseline"], counters, sample_bs = cpu_baseline(sc, args.steps, args.cpu_budget)
+        _log("bench: CPU baseline")
+        out["cpu_baseline"], counters, sample_bs = cpu_baseline(sc, start + args.steps, args.cpu_budget, start)
         if counters:
-            out["narrowphase"] = narrowphase_rate(counters, sample_bs, nb, exact64["kernels"])
+            k64 = exact64["kernels"]
+            out["k2"] = narrowphase_rate(counters, sample_bs, nb, k64["np_count"] + k64["np_scans"] + k64["np_emit"])
     print(json.dumps(out), flush=True)
 
 

@@ -65,7 +65,7 @@ EXPORTS = (
     "cbz_sur_train", "cbz_sur_grad", "cbz_sur_predict", "cbz_sur_resolve", "cbz_sur_world_mask", "cbz_sur_phase_times", "cbz_step_sur", "cbz_integrate_timed",
     "cbz_set_materials", "cbz_set_plane_materials", "cbz_set_contact_materials",
     "cbz_checkpoint_size", "cbz_checkpoint_save", "cbz_checkpoint_load", "cbz_narrowphase_broad",
-    "cbz_resolver_occupancy",
+    "cbz_resolver_occupancy", "cbz_sur_check_gemms",
 )
 
 
@@ -138,6 +138,7 @@ def load():
         "cbz_sur_train": ([P, i32, i32, ctypes.c_float, ctypes.c_uint64, P], i32),
         "cbz_sur_grad": ([P, i32, P, P, P, P], i32),
         "cbz_sur_predict": ([P, i32, P, P, i32], i32),
+        "cbz_sur_check_gemms": ([i32, P, i32], i32),
         "cbz_sur_resolve": ([P, f64, i64, f64, i64, i32, P], i32),
         "cbz_sur_world_mask": ([P, P], i32),
         "cbz_step_sur": ([P, f64, i32, i64, f64, i64, i32, P, P], i32),

@@ -6,6 +6,8 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "cbz_broad.h"
@@ -191,6 +193,10 @@ struct cbz_context {
   int64_t* d_prof = nullptr;           // [4] device-side step counters (k_prof_accum)
   std::vector<hipEvent_t> pev;         // per-step event pool of profiled cbz_step calls
   void* pos0 = nullptr;  // body positions at the last prepareContacts
+  // Shape check (cbz_sur_check_gemms): when set, the surrogate's GEMMs are
+  // checked against these [begin, end) byte ranges instead of launched, and
+  // its other kernels are skipped.
+  const std::vector<std::pair<uintptr_t, uintptr_t>>* dry_regions = nullptr;
 };
 
 namespace {
@@ -775,27 +781,66 @@ LayerOff layer_offsets() {
 constexpr int kNorm = kFeat + kFeat + kOut + kOut + kOut;
 int64_t round32(int64_t n) { return (n + 31) / 32 * 32; }
 
+// The training batch buffers for Mpad rows, each with its element count:
+// the allocator and the shape check (cbz_sur_check_gemms) share this list.
+template <typename F>
+void sur_batch_buffers(cbz_context::Sur& u, int64_t Mpad, F&& f) {
+  f(u.idx, Mpad);
+  f(u.bX, Mpad * kFeatPad);
+  f(u.bT, Mpad * kOut);
+  f(u.At[0], Mpad * kFeatPad);
+  for (int l = 0; l < kLayers; ++l) {
+    const int64_t w = Mpad * layer_out_pad(l);
+    f(u.Z[l], w);
+    f(u.dZ[l], w);
+    f(u.dZt[l], w);
+    if (l < kLayers - 1) {
+      f(u.Hb[l], w);
+      f(u.At[l + 1], w);
+    }
+  }
+  f(u.Yout, Mpad * kOut);
+  f(u.dA, Mpad * kHidden);
+  f(u.row_loss, Mpad);
+}
+
+// The model's device buffers (master weights, optimiser state,
+// normalisation, packed BF16 weights) with their element counts.
+template <typename F>
+void sur_model_buffers(cbz_context::Sur& u, F&& f) {
+  const int64_t total = layer_offsets().total;
+  f(u.params, total);
+  f(u.grad, total);
+  f(u.adam_m, total);
+  f(u.adam_v, total);
+  f(u.norm, int64_t(kNorm));
+  for (int l = 0; l < kLayers; ++l) {
+    f(u.Wnk[l], int64_t(layer_out_pad(l)) * layer_in_pad(l));
+    f(u.Wkn[l], int64_t(layer_out_pad(l)) * layer_in_pad(l));
+  }
+}
+
+// The inference activations for Mpad rows.
+template <typename F>
+void sur_infer_buffers(cbz_context::Sur& u, int64_t Mpad, F&& f) {
+  f(u.X, Mpad * kFeatPad);
+  f(u.H[0], Mpad * kHidden);
+  f(u.H[1], Mpad * kHidden);
+  f(u.Yp, Mpad * kOut);
+}
+
+
 int sur_alloc_model(cbz_ctx* c) {
   auto& u = c->sur;
   if (u.params) return CBZ_OK;
-  const LayerOff o = layer_offsets();
-  int rc;
-  if ((rc = dalloc(c, &u.params, (size_t)o.total))) return rc;
-  if ((rc = dalloc(c, &u.grad, (size_t)o.total))) return rc;
-  if ((rc = dalloc(c, &u.adam_m, (size_t)o.total))) return rc;
-  if ((rc = dalloc(c, &u.adam_v, (size_t)o.total))) return rc;
-  if ((rc = dalloc(c, &u.norm, (size_t)kNorm))) return rc;
-  HIPCHK(c, hipMemset(u.adam_m, 0, o.total * sizeof(float)));
-  HIPCHK(c, hipMemset(u.adam_v, 0, o.total * sizeof(float)));
-  HIPCHK(c, hipMemset(u.grad, 0, o.total * sizeof(float)));
-  for (int l = 0; l < kLayers; ++l) {
-    const size_t sz = (size_t)layer_out_pad(l) * layer_in_pad(l);
-    if ((rc = dalloc(c, &u.Wnk[l], sz))) return rc;
-    if ((rc = dalloc(c, &u.Wkn[l], sz))) return rc;
-    HIPCHK(c, hipMemset(u.Wnk[l], 0, sz * sizeof(__bf16)));
-    HIPCHK(c, hipMemset(u.Wkn[l], 0, sz * sizeof(__bf16)));
-  }
-  return CBZ_OK;
+  int rc = CBZ_OK;
+  sur_model_buffers(u, [&](auto*& p, int64_t n) {
+    if (rc == CBZ_OK) rc = dalloc(c, &p, (size_t)n);
+    // zeroed: the optimiser state, and the padding of the packed weights
+    if (rc == CBZ_OK && hipMemset(p, 0, (size_t)n * sizeof(*p)) != hipSuccess)
+      rc = fail(c, CBZ_ERR_HIP, "sur: hipMemset failed");
+  });
+  return rc;
 }
 
 void sur_free_model(cbz_ctx* c) {
@@ -832,6 +877,45 @@ int sur_pack(cbz_ctx* c) {
   for (int l = 0; l < kLayers; ++l)
     HIPCHK(c, launch_pack_weights(u.params + o.w[l], layer_out(l), layer_in(l), u.Wnk[l], layer_in_pad(l), u.Wkn[l],
                                   layer_out_pad(l), c->stream));
+  return CBZ_OK;
+}
+
+// The bytes [p, p + bytes) lie inside one region of the shape check.
+bool dry_inside(const cbz_ctx* c, const void* p, int64_t bytes) {
+  const uintptr_t a = (uintptr_t)p;
+  for (const auto& r : *c->dry_regions)
+    if (a >= r.first && a < r.second) return bytes >= 0 && a + (uintptr_t)bytes <= r.second;
+  return false;
+}
+
+// One GEMM of the surrogate: launched, or in the shape check compared with
+// the extents of the buffers it reads and writes (k_gemm reads rows < M of A
+// and rows < N of Bt, K elements each; writes Z/Y where row < m_valid and
+// col < n_valid, H/Ht where row < M and col < N).
+int run_gemm(cbz_ctx* c, const GemmArgs& g, const char* what) {
+  if (!c->dry_regions) {
+    HIPCHK(c, launch_gemm(g, c->stream));
+    return CBZ_OK;
+  }
+  auto bad = [&](const char* op) {
+    return fail(c, CBZ_ERR_STATE, std::string("GEMM ") + what + ": operand " + op + " exceeds its buffer (M " +
+                                      std::to_string(g.M) + ", N " + std::to_string(g.N) + ", K " +
+                                      std::to_string(g.K) + ")");
+  };
+  if (g.M <= 0 || g.N <= 0) return CBZ_OK;
+  if (g.K <= 0 || g.K % 16 || g.lda < g.K || g.ldb < g.K) return bad("K");
+  const int64_t b2 = 2, b4 = 4;
+  if (!dry_inside(c, g.A, ((g.M - 1) * g.lda + g.K) * b2)) return bad("A");
+  if (!dry_inside(c, g.Bt, ((g.N - 1) * g.ldb + g.K) * b2)) return bad("Bt");
+  const int64_t mv = std::min(g.M, g.m_valid), nv = std::min(g.N, g.n_valid);
+  if (g.bias && nv > 0 && !dry_inside(c, g.bias, nv * b4)) return bad("bias");
+  if (mv > 0 && nv > 0) {
+    if (g.Z && !dry_inside(c, g.Z, ((mv - 1) * g.ldz + nv) * b4)) return bad("Z");
+    if (g.Y && !dry_inside(c, g.Y, ((mv - 1) * g.ldy + nv) * b4)) return bad("Y");
+    if (g.y_scale && (!dry_inside(c, g.y_scale, nv * b4) || !dry_inside(c, g.y_shift, nv * b4))) return bad("y_scale");
+  }
+  if (g.H && !dry_inside(c, g.H, ((g.M - 1) * g.ldh + g.N) * b2)) return bad("H");
+  if (g.Ht && !dry_inside(c, g.Ht, ((g.N - 1) * g.ldht + g.M) * b2)) return bad("Ht");
   return CBZ_OK;
 }
 
@@ -876,7 +960,8 @@ int sur_forward(cbz_ctx* c, const __bf16* X, int64_t M, int64_t Mpad, bool train
       g.y_scale = u.norm + 2 * kFeat + kOut;  // out_std
       g.y_shift = u.norm + 2 * kFeat;         // out_mean
     }
-    HIPCHK(c, launch_gemm(g, c->stream));
+    int rc = run_gemm(c, g, train ? "forward (training)" : "forward");
+    if (rc) return rc;
   }
   return CBZ_OK;
 }
@@ -904,8 +989,11 @@ int sur_infer(cbz_ctx* c, const __bf16* X, int64_t M) {
 int sur_backward(cbz_ctx* c, int64_t M, int64_t Mpad) {
   auto& u = c->sur;
   const LayerOff o = layer_offsets();
-  HIPCHK(c, launch_mse(u.Yout, u.bT, (int)Mpad, (int)M, u.row_loss, u.dZ[kLayers - 1], u.dZt[kLayers - 1],
-                       (int)Mpad, c->stream));
+  const bool dry = c->dry_regions != nullptr;
+  int rc;
+  if (!dry)
+    HIPCHK(c, launch_mse(u.Yout, u.bT, (int)Mpad, (int)M, u.row_loss, u.dZ[kLayers - 1], u.dZt[kLayers - 1],
+                         (int)Mpad, c->stream));
   for (int l = kLayers - 1; l >= 0; --l) {
     const int N = layer_out(l), K = layer_in(l), Np = layer_out_pad(l), Kp = layer_in_pad(l);
     // dW = dZ^T * A_in : [Np x Mpad] * [Mpad x Kp]
@@ -921,8 +1009,8 @@ int sur_backward(cbz_ctx* c, int64_t M, int64_t Mpad) {
     g.ldy = K;
     g.m_valid = N;
     g.n_valid = K;
-    HIPCHK(c, launch_gemm(g, c->stream));
-    HIPCHK(c, launch_colsum(u.dZ[l], (int)Mpad, N, Np, u.grad + o.b[l], c->stream));
+    if ((rc = run_gemm(c, g, "weight gradient"))) return rc;
+    if (!dry) HIPCHK(c, launch_colsum(u.dZ[l], (int)Mpad, N, Np, u.grad + o.b[l], c->stream));
     if (l > 0) {
       // dA_in = dZ * W : [Mpad x Np] * [Np x Kp], B given as Wkn [Kp x Np]
       GemmArgs h{};
@@ -937,8 +1025,9 @@ int sur_backward(cbz_ctx* c, int64_t M, int64_t Mpad) {
       h.ldy = Kp;
       h.m_valid = (int)M;
       h.n_valid = Kp;
-      HIPCHK(c, launch_gemm(h, c->stream));
-      HIPCHK(c, launch_silu_bwd(u.dA, u.Z[l - 1], (int)Mpad, (int)M, Kp, Kp, u.dZ[l - 1], u.dZt[l - 1], (int)Mpad,
+      if ((rc = run_gemm(c, h, "input gradient"))) return rc;
+      if (!dry)
+        HIPCHK(c, launch_silu_bwd(u.dA, u.Z[l - 1], (int)Mpad, (int)M, Kp, Kp, u.dZ[l - 1], u.dZt[l - 1], (int)Mpad,
                                 c->stream));
     }
   }
@@ -949,24 +1038,11 @@ int sur_ensure_batch(cbz_ctx* c, int rows) {
   auto& u = c->sur;
   const int Mpad = (int)round32(rows);
   if (Mpad <= u.batch_cap) return CBZ_OK;
-  int rc;
-  if ((rc = dalloc(c, &u.idx, (size_t)Mpad))) return rc;
-  if ((rc = dalloc(c, &u.bX, (size_t)Mpad * kFeatPad))) return rc;
-  if ((rc = dalloc(c, &u.bT, (size_t)Mpad * kOut))) return rc;
-  if ((rc = dalloc(c, &u.At[0], (size_t)Mpad * kFeatPad))) return rc;
-  for (int l = 0; l < kLayers; ++l) {
-    const size_t w = (size_t)Mpad * layer_out_pad(l);
-    if ((rc = dalloc(c, &u.Z[l], w))) return rc;
-    if ((rc = dalloc(c, &u.dZ[l], w))) return rc;
-    if ((rc = dalloc(c, &u.dZt[l], w))) return rc;
-    if (l < kLayers - 1) {
-      if ((rc = dalloc(c, &u.Hb[l], w))) return rc;
-      if ((rc = dalloc(c, &u.At[l + 1], w))) return rc;
-    }
-  }
-  if ((rc = dalloc(c, &u.Yout, (size_t)Mpad * kOut))) return rc;
-  if ((rc = dalloc(c, &u.dA, (size_t)Mpad * kHidden))) return rc;
-  if ((rc = dalloc(c, &u.row_loss, (size_t)Mpad))) return rc;
+  int rc = CBZ_OK;
+  sur_batch_buffers(u, Mpad, [&](auto*& p, int64_t n) {
+    if (rc == CBZ_OK) rc = dalloc(c, &p, (size_t)n);
+  });
+  if (rc) return rc;
   u.batch_cap = Mpad;
   return CBZ_OK;
 }
@@ -976,11 +1052,11 @@ int sur_ensure_inference(cbz_ctx* c, int64_t rows) {
   int64_t Mpad = round32(std::max<int64_t>(rows, 32));
   if (Mpad <= u.inf_cap && u.mask) return CBZ_OK;
   Mpad = std::max(Mpad, u.inf_cap + u.inf_cap / 4);  // grow geometrically
-  int rc;
-  if ((rc = dalloc(c, &u.X, (size_t)Mpad * kFeatPad))) return rc;
-  if ((rc = dalloc(c, &u.H[0], (size_t)Mpad * kHidden))) return rc;
-  if ((rc = dalloc(c, &u.H[1], (size_t)Mpad * kHidden))) return rc;
-  if ((rc = dalloc(c, &u.Yp, (size_t)Mpad * kOut))) return rc;
+  int rc = CBZ_OK;
+  sur_infer_buffers(u, Mpad, [&](auto*& p, int64_t n) {
+    if (rc == CBZ_OK) rc = dalloc(c, &p, (size_t)n);
+  });
+  if (rc) return rc;
   if ((rc = dalloc(c, &u.deg, (size_t)std::max<int64_t>(c->nb, 1)))) return rc;
   if ((rc = dalloc(c, &u.mask, (size_t)std::max(c->nw, 1)))) return rc;
   if ((rc = dalloc(c, &u.nfall, 1))) return rc;
@@ -1497,13 +1573,20 @@ int cbz_contact_counts(cbz_ctx* c, int64_t* counts) {
 int cbz_get_contacts(cbz_ctx* c, int32_t world, cbz_contact_view* out) {
   int rc = need_scene(c);
   if (rc) return rc;
-  if (!out || world < 0 || world >= c->nw) return fail(c, CBZ_ERR_ARG, "get_contacts: bad arguments");
+  if (!out || world < -1 || world >= c->nw) return fail(c, CBZ_ERR_ARG, "get_contacts: bad arguments");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  int32_t n = 0;
-  int64_t off = 0;
-  HIPCHK(c, hipMemcpy(&n, c->world_cnt + world, sizeof n, hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(&off, c->world_coff + world, sizeof off, hipMemcpyDeviceToHost));
+  int64_t n = 0, off = 0;
+  if (world < 0) {  // every world: the lists are packed in world order from slot 0
+    if (!c->have_contacts) return CBZ_OK;
+    HIPCHK(c, hipMemcpy(&n, c->total, sizeof n, hipMemcpyDeviceToHost));
+    n = std::min(n, c->cap);
+  } else {
+    int32_t nw = 0;
+    HIPCHK(c, hipMemcpy(&nw, c->world_cnt + world, sizeof nw, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&off, c->world_coff + world, sizeof off, hipMemcpyDeviceToHost));
+    n = nw;
+  }
   if (n == 0) return CBZ_OK;
   if (off + n > c->cap) return fail(c, CBZ_ERR_CAPACITY, "get_contacts: world's contacts exceed the capacity");
   const size_t rs = real_size(c);
@@ -1531,7 +1614,7 @@ int cbz_get_contacts(cbz_ctx* c, int32_t world, cbz_contact_view* out) {
     if (mat) {
       if ((rc = download_real(c, base + (size_t)mats[m].plane0 * S * rs, S, off, n, 1, mats[m].dst))) return rc;
     } else {
-      for (int32_t k = 0; k < n; ++k) mats[m].dst[k] = consts[m];
+      for (int64_t k = 0; k < n; ++k) mats[m].dst[k] = consts[m];
     }
   }
   return CBZ_OK;
@@ -2082,6 +2165,36 @@ int cbz_sur_train(cbz_ctx* c, int32_t steps, int32_t batch, float lr, uint64_t s
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (losses && steps > 0) HIPCHK(c, hipMemcpy(losses, u.losses, steps * sizeof(float), hipMemcpyDeviceToHost));
   return CBZ_OK;
+}
+
+int cbz_sur_check_gemms(int32_t rows, char* msg, int32_t msg_len) {
+  if (rows <= 0) return CBZ_ERR_ARG;
+  // A context with no device: every buffer the training step and the
+  // inference pass use gets a disjoint fake address range of exactly the
+  // size its allocator gives it, then both passes run with their GEMMs
+  // checked instead of launched.
+  cbz_context fake;
+  std::vector<std::pair<uintptr_t, uintptr_t>> regions;
+  uintptr_t next = uintptr_t(1) << 44;
+  auto place = [&](auto*& p, int64_t n) {
+    using T = std::remove_reference_t<decltype(*p)>;
+    const uintptr_t bytes = uintptr_t(std::max<int64_t>(n, 1)) * sizeof(T);
+    p = (decltype(p))next;
+    regions.push_back({next, next + bytes});
+    next += ((bytes + (uintptr_t(1) << 20)) >> 20 << 20) + (uintptr_t(1) << 20);
+  };
+  const int64_t Mpad = round32(rows);
+  sur_model_buffers(fake.sur, place);
+  sur_batch_buffers(fake.sur, Mpad, place);
+  sur_infer_buffers(fake.sur, Mpad, place);
+  fake.dry_regions = &regions;
+  int rc = sur_forward(&fake, fake.sur.bX, rows, Mpad, true);
+  if (!rc) rc = sur_backward(&fake, rows, Mpad);
+  if (!rc) rc = sur_forward(&fake, fake.sur.bX, rows, Mpad, false);
+  if (msg && msg_len > 0) {
+    std::snprintf(msg, (size_t)msg_len, "%s", rc ? fake.err.c_str() : "");
+  }
+  return rc;
 }
 
 // rows of raw features (and raw labels) -> the staging batch

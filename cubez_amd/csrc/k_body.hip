@@ -98,10 +98,10 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies<R> b, R dt, R bias) {
     V3<R> vel = ld3(b.vel, S, i);
     V3<R> rot = ld3(b.rot, S, i);
     // angularAcceleration = iitWorld * torqueAccum (:223).  Each component
-    // is a signed zero for a finite tensor, and adding a signed zero leaves
-    // every Rotation component unchanged except a zero one, so the old
-    // tensor is read only when some component of Rotation is zero.
-    if (rot.x == R(0) || rot.y == R(0) || rot.z == R(0)) {
+    // is a signed zero for a finite tensor (which changes only a zero
+    // Rotation component's sign) and NaN for a tensor with an infinite or
+    // NaN entry, so the product is always evaluated.
+    {
       R IWo[9];
       ldk<R, 9>(b.iit_world, S, i, IWo);
       V3<R> aa = m3v(IWo, V3<R>{R(0), R(0), R(0)});

@@ -170,7 +170,9 @@ __global__ void k_fill_f32(float* p, int64_t n, float v) {
 // fragments as 16-byte vectors.  Fragment layout of v_mfma_f32_32x32x16_bf16
 // for lane l (r = l % 32, h = l / 32): A[r][8h .. 8h+7], B[8h .. 8h+7][r],
 // and accumulator j holds D[8 (j/4) + 4h + j%4][r].  Each wave computes one
-// 32x32 tile; a block holds WM x WN waves.  M, N are multiples of 32, K of 16.
+// 32x32 tile; a block holds WM x WN waves.  K is a multiple of 16.  A tile
+// that extends past row M of A or row N of Bt reads zeros there and writes
+// nothing there, so an operand needs only M (N) rows.
 template <int WM, int WN>
 __global__ void __launch_bounds__(64 * WM * WN) k_gemm(GemmArgs g) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -178,19 +180,24 @@ __global__ void __launch_bounds__(64 * WM * WN) k_gemm(GemmArgs g) {
   const int n0 = (blockIdx.y * WN + wave / WM) * 32;
   if (m0 >= g.M || n0 >= g.N) return;
   const int r = lane & 31, h = lane >> 5;
-  const __bf16* ap = g.A + int64_t(m0 + r) * g.lda + 8 * h;
-  const __bf16* bp = g.Bt + int64_t(n0 + r) * g.ldb + 8 * h;
+  const bool a_in = m0 + r < g.M, b_in = n0 + r < g.N;
+  const __bf16* ap = g.A + int64_t(a_in ? m0 + r : 0) * g.lda + 8 * h;
+  const __bf16* bp = g.Bt + int64_t(b_in ? n0 + r : 0) * g.ldb + 8 * h;
   f32x16 acc = {};
+  const bf16x8 zero = {};
   for (int k = 0; k < g.K; k += 16) {
-    const bf16x8 a = *(const bf16x8*)(ap + k);
-    const bf16x8 bb = *(const bf16x8*)(bp + k);
+    const bf16x8 a = a_in ? *(const bf16x8*)(ap + k) : zero;
+    const bf16x8 bb = b_in ? *(const bf16x8*)(bp + k) : zero;
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
   }
   const int col = n0 + r;
-  const float bias = g.bias ? g.bias[col] : 0.f;
+  if (col >= g.N) return;
+  // bias has n_valid entries (padding columns produce zeros)
+  const float bias = g.bias && col < g.n_valid ? g.bias[col] : 0.f;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const int row = m0 + 8 * (j >> 2) + 4 * h + (j & 3);
+    if (row >= g.M) continue;
     const float z = acc[j] + bias;
     const float y = g.act ? silu(z) : z;
     if (row < g.m_valid && col < g.n_valid) {
@@ -474,6 +481,8 @@ __global__ void __launch_bounds__(256) k_sur_scatter(SurApply<R> a) {
 // ---------------------------------------------------------------------------
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  // the fragment loads step K by 16 and read K contiguous elements per row
+  if (g.K <= 0 || g.K % 16 != 0 || g.lda < g.K || g.ldb < g.K) return hipErrorInvalidValue;
   if (g.N >= 128) {
     dim3 grid((g.M + 63) / 64, (g.N + 63) / 64);
     k_gemm<2, 2><<<grid, 256, 0, st>>>(g);

@@ -120,7 +120,9 @@ class World:
         return c
 
     def contacts(self, world=0, internals=True):
-        n = int(self.contact_counts()[world])
+        """Contact records of one world, or of every world (world=-1)."""
+        counts = self.contact_counts()
+        n = int(counts.sum()) if world < 0 else int(counts[world])
         out = {
             "body0": np.empty(n, np.int32), "body1": np.empty(n, np.int32),
             "point": np.empty((n, 3)), "normal": np.empty((n, 3)), "pen": np.empty(n),

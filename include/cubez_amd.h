@@ -160,7 +160,8 @@ int cbz_narrowphase_pairs(cbz_ctx *ctx, int64_t n_pairs, const int32_t *one, con
 int cbz_narrowphase_broad(cbz_ctx *ctx, int64_t *total, int64_t *pairs_checked);
 /* contact count of every world after the last narrowphase ([n_worlds]) */
 int cbz_contact_counts(cbz_ctx *ctx, int64_t *counts);
-/* all contacts of one world, in emission order */
+/* all contacts of one world, in emission order; world = -1: the contacts of
+ * every world, concatenated in world order (the total of cbz_narrowphase) */
 int cbz_get_contacts(cbz_ctx *ctx, int32_t world, cbz_contact_view *out);
 /* Replace the contact lists (counts[n_worlds], then the records of all
  * worlds concatenated): contacts built by the caller, as when a Go caller
@@ -312,6 +313,12 @@ int cbz_sur_grad(cbz_ctx *ctx, int32_t rows, const float *features, const float 
  * give the same bits) */
 #define CBZ_SUR_PREDICT_LAYERED 1
 int cbz_sur_predict(cbz_ctx *ctx, int32_t rows, const float *features, float *out, int32_t flags);
+/* Host-only consistency check of the surrogate's matrix products (needs no
+ * GPU): every GEMM of one training step and of one layered inference pass
+ * over `rows` rows is compared with the sizes the library allocates for its
+ * operands; returns CBZ_OK, or CBZ_ERR_STATE with the first GEMM whose reads
+ * or writes would leave a buffer described in msg[msg_len]. */
+int cbz_sur_check_gemms(int32_t rows, char *msg, int32_t msg_len);
 /* ResolveContacts with the surrogate.  The position phase is the exact
  * one.  The network's per-contact velocity changes are summed per body and
  * applied; then, per world, the reference's worst-first velocity loop runs

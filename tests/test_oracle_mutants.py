@@ -35,7 +35,40 @@ MUTANTS = {
     "frictionless_angular_operands": ("dvw = v3_cross(&dvw, &c->relativeContactPosition[0]);",
                                       "dvw = v3_cross(&c->relativeContactPosition[0], &dvw);"),
     "friction_cone_coefficient": ("ic.v[2] *= c->Friction * ic.v[0];", "ic.v[2] *= ic.v[0];"),
+    # box-box edge-edge contact (colliders.go:642-709)
+    "edge_edge_normal_flip": ("if (v3_dot(&axis, &toCenter) > 0) v3_mul_with(&axis, -1.0);",
+                              "if (v3_dot(&axis, &toCenter) < 0) v3_mul_with(&axis, -1.0);"),
+    "edge_point_one_sign": ("if (v3_dot(&a, &axis) > 0) ptOnOneEdge.v[i] = -ptOnOneEdge.v[i];",
+                            "if (v3_dot(&a, &axis) < 0) ptOnOneEdge.v[i] = -ptOnOneEdge.v[i];"),
+    "edge_point_two_sign": ("if (v3_dot(&a, &axis) < 0) ptOnTwoEdge.v[i] = -ptOnTwoEdge.v[i];",
+                            "if (v3_dot(&a, &axis) > 0) ptOnTwoEdge.v[i] = -ptOnTwoEdge.v[i];"),
+    # the SAT's near-parallel threshold (colliders.go:460-462), either side
+    "sat_parallel_threshold_high": ("if (v3_square_magnitude(&axis) < Epsilon) {",
+                                    "if (v3_square_magnitude(&axis) < 1e-3) {"),
+    "sat_parallel_threshold_low": ("if (v3_square_magnitude(&axis) < Epsilon) {",
+                                   "if (v3_square_magnitude(&axis) < 1e-8) {"),
+    # applyPositionChange (contact.go:286-386)
+    "angular_limit": ("const Real angularLimit = 0.2;", "const Real angularLimit = 0.02;"),
+    "angular_change_scale": ("v3_mul_with(&angularChange[i], angularMove[i] / angularInertia[i]);",
+                             "v3_mul_with(&angularChange[i], angularMove[i]);"),
+    "rederive_awake_bodies": ("if (body->IsAwake == 0) rb_calculate_derived(body);",
+                              "rb_calculate_derived(body);"),
+    # calculateDesiredDeltaVelocity: body one's acceleration term (contact.go:98-101)
+    "velocity_from_acc_body_one_sign": ("        velocityFromAcc -= v3_dot(&tempVelocity, &c->ContactNormal);",
+                                        "        velocityFromAcc += v3_dot(&tempVelocity, &c->ContactNormal);"),
+    # matchAwakeState (contact.go:195-200)
+    "wake_the_wrong_body": ("            rb_set_awake(c->Bodies[1], 1);\n        else\n            rb_set_awake(c->Bodies[0], 1);",
+                            "            rb_set_awake(c->Bodies[0], 1);\n        else\n            rb_set_awake(c->Bodies[1], 1);"),
 }
+# Not planted: inverting useOne (colliders.go:687-690).  useOne only selects
+# the point that contactPoint returns when the two edges are parallel or
+# their closest points fall outside an edge (:538-563).  When an edge-edge
+# axis is the minimal SAT axis, it is the Minkowski-difference face nearest
+# the origin, whose foot point lies inside that face, i.e. the closest points
+# lie within both edges; and an edge pair parallel enough for the first
+# fallback has its axis skipped by tryAxis.  Random and constructed searches
+# over 70,000 overlapping pairs (rods, plates, edge-on-face) reached neither
+# fallback, so no input distinguishes the mutant.
 
 
 @pytest.mark.parametrize("name", sorted(MUTANTS))
@@ -51,6 +84,7 @@ def test_oracle_checks_catch_planted_mistake(name, tmp_path):
     r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider", "-m", "not gpu",
                         os.path.join(ROOT, "tests", "test_oracle_math.py"),
                         os.path.join(ROOT, "tests", "test_oracle_physics.py"),
+                        os.path.join(ROOT, "tests", "test_oracle_pins.py"),
                         os.path.join(ROOT, "tests", "test_oracle_materials.py")],
                        cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
     assert r.returncode != 0, f"mutant {name} passed every oracle check:\n{r.stdout[-2000:]}"
