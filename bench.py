@@ -255,6 +255,7 @@ def run_path(World, sc, ck, precision, steps, warmup):
     w.step(sc.dt, steps, sc.iter_mult)
     w.set_profiling(False)
     ms, cnt = w.phase_times()
+    crit = critical_path(w)
     # K1 alone: back-to-back launches between two events, on the window's
     # start state
     _start(w, sc, ck)
@@ -284,6 +285,7 @@ def run_path(World, sc, ck, precision, steps, warmup):
             "ns_per_iteration_aggregate": resolve_ns / (iters / sp) if iters else None,
             "ns_per_iteration_per_world": resolve_ns * resident / (iters / sp) if iters else None,
         },
+        "critical_path_last_step": crit,
         "k1_ms_per_launch": k1_ms,
         "k1_bodies_per_launch": k1_bodies / K1_LAUNCHES,
         "residual_penetration": resid,
@@ -331,6 +333,7 @@ def run_surrogate(World, sc, ck, model, steps, warmup, tol, polish, zero=False):
     w.step_sur(sc.dt, steps, sc.iter_mult, tol, polish, zero)
     w.set_profiling(False)
     ms, cnt = w.sur_phase_times()
+    crit = critical_path(w)
     w.close()
     n_res, contacts, fallbacks, vel_iters = (int(x) for x in cnt)
     names = ("position_phase", "features", "network", "per_body_sums", "velocity_phase")
@@ -343,8 +346,31 @@ def run_surrogate(World, sc, ck, model, steps, warmup, tol, polish, zero=False):
         "network_ms_total": ms[2],
         "contacts_total": contacts,
         "residual_penetration": resid,
+        "critical_path_last_step": crit,
         "_final_pos": final_pos,
     })
+    return out
+
+
+def critical_path(w):
+    """The last step's resolver launches, from the per-world workgroup
+    timestamps: each launch's span (first start to last end) and its
+    longest world (duration, contacts, iterations), in microseconds."""
+    t = w.resolver_world_times()
+    counts = w.contact_counts()
+    iters = w.resolver_iterations()
+    out = {}
+    for name, k in (("position_launch", 0), ("velocity_launch", 2)):
+        st, en = t[:, k], t[:, k + 1]
+        ok = st >= 0
+        if not ok.any():
+            continue
+        dur = np.where(ok, en - st, -1.0)
+        wmax = int(np.argmax(dur))
+        out[name] = {"span_us": float(en[ok].max() - st[ok].min()), "longest_world_us": float(dur[wmax]),
+                     "longest_world": wmax, "its_contacts": int(counts[wmax]),
+                     "its_iterations": [int(x) for x in iters[wmax]],
+                     "median_world_us": float(np.median(dur[ok]))}
     return out
 
 
@@ -468,6 +494,9 @@ def main():
         out["speedup_vs_exact_f32"] = sur["value"] / exact32["value"]
         out["fallback_rate"] = sur["fallback_rate"]
         out["velocity_iterations_per_step"] = {"surrogate": sur["velocity_iterations_per_step"]}
+        out["critical_path_last_step"] = {"surrogate": sur["critical_path_last_step"],
+                                          "exact_f64": exact64["critical_path_last_step"],
+                                          "exact_f32": exact32["critical_path_last_step"]}
         out["contacts_per_step"] = sur["contacts_per_step"]
         out["contacts_per_s"] = sur["contacts_per_step"] * sur["value"] / nb
         phases.update(sur["resolver_ms_per_step"])

@@ -32,6 +32,8 @@ const FieldInfo kFields[CBZ_NUM_FIELDS] = {
 };
 
 constexpr uint8_t kUserFlags = CBZ_FLAG_AWAKE | CBZ_FLAG_CAN_SLEEP | CBZ_FLAG_STATIC;
+// real planes per contact slot (see cbz_context::c_real)
+constexpr int kContactPlanes = 40;
 
 }  // namespace
 
@@ -120,12 +122,15 @@ struct cbz_context {
   int64_t cap = 0;
   void* c_body0 = nullptr;
   void* c_body1 = nullptr;
-  void* c_real = nullptr;  // point 3, normal 3, pen 1, c2w 9, rel 6, cvel 3, ddv 1, friction 1, restitution 1 (28 planes)
+  // point 3, normal 3, pen 1, c2w 9, rel 6, cvel 3, ddv 1, friction 1,
+  // restitution 1, then the resolver's impulse data 12 (40 planes)
+  void* c_real = nullptr;
   int32_t* world_cnt = nullptr;
   int64_t* world_coff = nullptr;
   int64_t* total = nullptr;
   int32_t* overflow = nullptr;  // [0] this narrowphase, [1] sticky since the host cleared it
   int64_t* world_iters = nullptr;
+  int64_t* world_time = nullptr;  // [4 * nw] resolver workgroup start/end ticks (position, velocity launch)
   int32_t* order = nullptr;
   bool have_contacts = false;
 
@@ -145,7 +150,8 @@ struct cbz_context {
     float* dsY = nullptr;
     int64_t ds_rows = 0, ds_cap = 0;
     // inference buffers
-    int64_t inf_cap = 0;
+    int64_t inf_cap = 0, lay_cap = 0;
+  int64_t generation = 0;  // bumped whenever the inference buffers are reallocated
     __bf16* X = nullptr;
     __bf16* H[2] = {};
     float* Yp = nullptr;
@@ -176,6 +182,16 @@ struct cbz_context {
   hipGraphExec_t gexec = nullptr;
   double g_dt = 0;
   int64_t g_mult = -1;
+  // captured surrogate step (cbz_step_sur without counts or profiling),
+  // keyed by its arguments and the inference buffers' generation; each
+  // replay appends the step's fallback count to fb_hist at fb_ctr
+  hipGraphExec_t sgexec = nullptr;
+  double sg_dt = 0, sg_tol = 0;
+  int64_t sg_mult = -1, sg_polish = -1, sg_gen = -1;
+  int32_t sg_flags = -1;
+  int32_t* fb_hist = nullptr;
+  int32_t* fb_ctr = nullptr;
+  int64_t fb_cap = 0;
 
   // profiling
   bool prof = false;
@@ -287,6 +303,9 @@ void drop_graph(cbz_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   c->gexec = nullptr;
   c->g_mult = -1;
+  if (c->sgexec) (void)hipGraphExecDestroy(c->sgexec);
+  c->sgexec = nullptr;
+  c->sg_mult = -1;
 }
 
 void free_sur_buffers(cbz_ctx* c) {
@@ -302,11 +321,16 @@ void free_sur_buffers(cbz_ctx* c) {
   dfree(u.vel0);
   dfree(u.cvel0);
   u.inf_cap = 0;
+  u.lay_cap = 0;
+  u.generation += 1;
 }
 
 void free_scene(cbz_ctx* c) {
   drop_graph(c);
   free_sur_buffers(c);
+  dfree(c->fb_hist);
+  dfree(c->fb_ctr);
+  c->fb_cap = 0;
   for (auto& f : c->field) dfree(f);
   dfree(c->pow_lin);
   dfree(c->pow_ang);
@@ -360,6 +384,7 @@ void free_scene(cbz_ctx* c) {
   dfree(c->total);
   dfree(c->overflow);
   dfree(c->world_iters);
+  dfree(c->world_time);
   dfree(c->order);
   dfree(c->d_awake);
   dfree(c->d_prof);
@@ -413,6 +438,7 @@ Contacts<R> contacts_view(cbz_ctx* c) {
   const bool mat = (c->mat_flags & CBZ_MATERIALS) != 0;
   v.fric = mat ? base + 26 * S : nullptr;
   v.rest = mat ? base + 27 * S : nullptr;
+  v.imp = base + 28 * S;
   return v;
 }
 
@@ -570,6 +596,7 @@ int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode 
   L.order = c->order;
   L.body_off = c->d_body_off;
   L.world_iters = c->world_iters;
+  L.world_time = c->world_time;
   L.n_worlds = c->nw;
   L.dt = (R)dt;
   L.iter_mult = mult;
@@ -820,13 +847,19 @@ void sur_model_buffers(cbz_context::Sur& u, F&& f) {
   }
 }
 
-// The inference activations for Mpad rows.
+// The fused inference's features and predictions for Mpad rows.
 template <typename F>
 void sur_infer_buffers(cbz_context::Sur& u, int64_t Mpad, F&& f) {
   f(u.X, Mpad * kFeatPad);
+  f(u.Yp, Mpad * kOut);
+}
+
+// The per-layer inference's activations for Mpad rows (cbz_sur_predict
+// with CBZ_SUR_PREDICT_LAYERED only).
+template <typename F>
+void sur_layered_buffers(cbz_context::Sur& u, int64_t Mpad, F&& f) {
   f(u.H[0], Mpad * kHidden);
   f(u.H[1], Mpad * kHidden);
-  f(u.Yp, Mpad * kOut);
 }
 
 
@@ -967,12 +1000,14 @@ int sur_forward(cbz_ctx* c, const __bf16* X, int64_t M, int64_t Mpad, bool train
 }
 
 // Inference forward: all four layers in one fused launch (k_mlp_forward),
-// rows of X [M][kFeatPad] -> denormalised predictions in u.Yp.
-int sur_infer(cbz_ctx* c, const __bf16* X, int64_t M) {
+// rows of X [M][kFeatPad] -> denormalised predictions in u.Yp; with M_dev,
+// the rows are the device count clamped to M.
+int sur_infer(cbz_ctx* c, const __bf16* X, int64_t M, const int64_t* M_dev = nullptr) {
   auto& u = c->sur;
   const LayerOff o = layer_offsets();
   MlpArgs f{};
-  f.M = (int)M;
+  f.M = M;
+  f.M_dev = M_dev;
   f.X = X;
   for (int l = 0; l < kLayers; ++l) {
     f.W[l] = u.Wnk[l];
@@ -1065,16 +1100,33 @@ int sur_ensure_inference(cbz_ctx* c, int64_t rows) {
   if ((rc = dalloc_bytes(c, &u.vel0, 7 * nb * real_size(c)))) return rc;
   if ((rc = dalloc_bytes(c, &u.cvel0, 3 * (size_t)std::max<int64_t>(c->cap, 1) * real_size(c)))) return rc;
   u.inf_cap = Mpad;
+  u.generation += 1;  // buffers moved: a captured surrogate step is stale
+  return CBZ_OK;
+}
+
+int sur_ensure_layered(cbz_ctx* c, int64_t rows) {
+  auto& u = c->sur;
+  const int64_t Mpad = round32(std::max<int64_t>(rows, 32));
+  if (Mpad <= u.lay_cap) return CBZ_OK;
+  int rc = CBZ_OK;
+  sur_layered_buffers(u, Mpad, [&](auto*& p, int64_t n) {
+    if (rc == CBZ_OK) rc = dalloc(c, &p, (size_t)n);
+  });
+  if (rc) return rc;
+  u.lay_cap = Mpad;
   return CBZ_OK;
 }
 
 // Profiling of one surrogate resolve (events ev[8..12] recorded, ev[13]
 // recorded here): accumulates the phase times, the contact count, the
 // fallbacks and the velocity-phase iterations.
-int sur_profile(cbz_ctx* c, int64_t contacts) {
+int sur_profile(cbz_ctx* c) {
   hipEvent_t* ev = c->ev + 8;
   HIPCHK(c, hipEventRecord(ev[5], c->stream));
   HIPCHK(c, hipEventSynchronize(ev[5]));
+  int64_t contacts = 0;
+  HIPCHK(c, hipMemcpy(&contacts, c->total, sizeof contacts, hipMemcpyDeviceToHost));
+  contacts = std::min(contacts, c->cap);
   for (int k = 0; k < 5; ++k) {
     float t = 0;
     HIPCHK(c, hipEventElapsedTime(&t, ev[k], ev[k + 1]));
@@ -1109,7 +1161,6 @@ int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int64_t polish_
   int rc = do_resolve<R>(c, dt, mult, 0, RM_POS);
   if (rc) return rc;
   if (c->prof) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-  int64_t tot = 0;
   if (tol < 0 || (flags & CBZ_SUR_ZERO_PREDICTION)) {
     // every world plain (tol < 0), or polished from a zero prediction
     if (c->prof) {
@@ -1122,17 +1173,16 @@ int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int64_t polish_
     rc = do_resolve<R>(c, dt, mult, 0, RM_VEL | RM_POLISH, u.mask, nullptr, tol, polish_mult);
     if (rc) return rc;
     if (nfall_dst) HIPCHK(c, launch_count_u8(u.mask, c->nw, nfall_dst, c->stream));
-    return c->prof ? sur_profile(c, tot) : CBZ_OK;
+    return c->prof ? sur_profile(c) : CBZ_OK;
   }
-  // the GEMM grid follows the contact count, read once per resolve
-  HIPCHK(c, hipMemcpyAsync(&tot, c->total, sizeof tot, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  tot = std::min(tot, c->cap);
-  if ((rc = sur_ensure_inference(c, tot))) return rc;
-  HIPCHK(c, launch_features<R>(bodies_view<R>(c), contacts_view<R>(c), tot, u.deg, u.norm, u.norm + kFeat, u.X,
-                               nullptr, c->stream));
+  // The feature and network grids cover the contact capacity and read the
+  // contact count on the device: no host round trip, so the step can be
+  // captured in a graph.
+  if ((rc = sur_ensure_inference(c, c->cap))) return rc;
+  HIPCHK(c, launch_features<R>(bodies_view<R>(c), contacts_view<R>(c), c->cap, c->total, u.deg, u.norm,
+                               u.norm + kFeat, u.X, nullptr, c->stream));
   if (c->prof) HIPCHK(c, hipEventRecord(ev[2], c->stream));
-  if ((rc = sur_infer(c, u.X, tot))) return rc;
+  if ((rc = sur_infer(c, u.X, c->cap, c->total))) return rc;
   if (c->prof) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   SurApply<R> a;
   a.b = bodies_view<R>(c);
@@ -1150,7 +1200,7 @@ int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int64_t polish_
   rc = do_resolve<R>(c, dt, mult, 0, RM_VEL | RM_POLISH, u.mask, nullptr, tol, polish_mult);
   if (rc) return rc;
   if (nfall_dst) HIPCHK(c, launch_count_u8(u.mask, c->nw, nfall_dst, c->stream));
-  return c->prof ? sur_profile(c, tot) : CBZ_OK;
+  return c->prof ? sur_profile(c) : CBZ_OK;
 }
 
 }  // namespace
@@ -1304,7 +1354,7 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   c->cap = c->cap_cfg > 0 ? c->cap_cfg : 16 * nb + 4096;
   if ((rc = dalloc_bytes(c, &c->c_body0, sizeof(int32_t) * (size_t)c->cap))) return rc;
   if ((rc = dalloc_bytes(c, &c->c_body1, sizeof(int32_t) * (size_t)c->cap))) return rc;
-  if ((rc = dalloc_bytes(c, &c->c_real, rs * 28 * (size_t)c->cap))) return rc;
+  if ((rc = dalloc_bytes(c, &c->c_real, rs * kContactPlanes * (size_t)c->cap))) return rc;
   if ((rc = dalloc(c, &c->hits, (size_t)c->cap))) return rc;
   if ((rc = dalloc(c, &c->n_hits, 1))) return rc;
   // candidate pairs of the cull: 4 per contact slot plus the scene's ordered
@@ -1322,6 +1372,7 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc(c, &c->total, 1))) return rc;
   if ((rc = dalloc(c, &c->overflow, 2))) return rc;
   if ((rc = dalloc(c, &c->world_iters, 2 * (size_t)n_worlds))) return rc;
+  if ((rc = dalloc(c, &c->world_time, 4 * (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->order, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->d_awake, kAwakeSlots))) return rc;
   if ((rc = dalloc(c, &c->d_prof, 4))) return rc;
@@ -1329,6 +1380,7 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   HIPCHK(c, hipMemset(c->world_cnt, 0, n_worlds * sizeof(int32_t)));
   HIPCHK(c, hipMemset(c->world_coff, 0, n_worlds * sizeof(int64_t)));
   HIPCHK(c, hipMemset(c->world_iters, 0, 2 * n_worlds * sizeof(int64_t)));
+  HIPCHK(c, hipMemset(c->world_time, 0xFF, 4 * n_worlds * sizeof(int64_t)));  // -1: no launch yet
   HIPCHK(c, hipMemset(c->overflow, 0, 2 * sizeof(int32_t)));
   HIPCHK(c, hipMemset(c->total, 0, sizeof(int64_t)));
   std::vector<int32_t> ord;
@@ -1746,6 +1798,24 @@ int cbz_resolver_occupancy(cbz_ctx* c, int32_t* worlds_per_cu, int32_t* n_cu) {
   return CBZ_OK;
 }
 
+int cbz_resolver_world_times(cbz_ctx* c, double* us) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (!us) return fail(c, CBZ_ERR_ARG, "resolver_world_times: null");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<int64_t> t(4 * (size_t)c->nw);
+  if (c->nw > 0) HIPCHK(c, hipMemcpy(t.data(), c->world_time, t.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+  int khz = 0;
+  HIPCHK(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+  if (khz <= 0) return fail(c, CBZ_ERR_HIP, "resolver_world_times: no wall-clock rate");
+  int64_t t0 = INT64_MAX;
+  for (int64_t v : t)
+    if (v >= 0) t0 = std::min(t0, v);
+  for (size_t k = 0; k < t.size(); ++k) us[k] = t[k] >= 0 ? double(t[k] - t0) * 1000.0 / khz : -1.0;
+  return CBZ_OK;
+}
+
 int cbz_resolver_iterations(cbz_ctx* c, int64_t* iters) {
   int rc = need_scene(c);
   if (rc) return rc;
@@ -2058,10 +2128,10 @@ int cbz_sur_collect(cbz_ctx* c, double dt, int64_t iter_per_contact, int64_t* ad
   rc = f64 ? do_resolve<double>(c, dt, iter_per_contact, 0, RM_POS) : do_resolve<float>(c, dt, iter_per_contact, 0, RM_POS);
   if (rc) return rc;
   if (f64)
-    HIPCHK(c, launch_features<double>(bodies_view<double>(c), contacts_view<double>(c), tot, u.deg, nullptr, nullptr,
+    HIPCHK(c, launch_features<double>(bodies_view<double>(c), contacts_view<double>(c), tot, nullptr, u.deg, nullptr, nullptr,
                                       nullptr, X, c->stream));
   else
-    HIPCHK(c, launch_features<float>(bodies_view<float>(c), contacts_view<float>(c), tot, u.deg, nullptr, nullptr,
+    HIPCHK(c, launch_features<float>(bodies_view<float>(c), contacts_view<float>(c), tot, nullptr, u.deg, nullptr, nullptr,
                                      nullptr, X, c->stream));
   HIPCHK(c, launch_fill_f32(Y, tot * kOut, 0.f, c->stream));
   rc = f64 ? do_resolve<double>(c, dt, iter_per_contact, 0, RM_VEL | RM_LABEL, nullptr, Y)
@@ -2187,6 +2257,7 @@ int cbz_sur_check_gemms(int32_t rows, char* msg, int32_t msg_len) {
   sur_model_buffers(fake.sur, place);
   sur_batch_buffers(fake.sur, Mpad, place);
   sur_infer_buffers(fake.sur, Mpad, place);
+  sur_layered_buffers(fake.sur, Mpad, place);
   fake.dry_regions = &regions;
   int rc = sur_forward(&fake, fake.sur.bX, rows, Mpad, true);
   if (!rc) rc = sur_backward(&fake, rows, Mpad);
@@ -2251,9 +2322,10 @@ int cbz_sur_predict(cbz_ctx* c, int32_t rows, const float* X, float* Y, int32_t 
   auto& u = c->sur;
   if ((rc = sur_ensure_inference(c, rows))) return rc;
   const int64_t Mpad = round32(rows);
-  if (flags & CBZ_SUR_PREDICT_LAYERED)
+  if (flags & CBZ_SUR_PREDICT_LAYERED) {
+    if ((rc = sur_ensure_layered(c, rows))) return rc;
     rc = sur_forward(c, u.bX, rows, Mpad, false);
-  else
+  } else
     rc = sur_infer(c, u.bX, rows);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2271,7 +2343,7 @@ int cbz_sur_resolve(cbz_ctx* c, double dt, int64_t iter_per_contact, double tol,
   if (!c->have_contacts) return fail(c, CBZ_ERR_STATE, "sur_resolve: no contacts (run a narrowphase first)");
   if (!c->sur.have_model) return fail(c, CBZ_ERR_STATE, "sur_resolve: no model");
   HIPCHK(c, hipSetDevice(c->device));
-  if ((rc = sur_ensure_inference(c, 32))) return rc;
+  if ((rc = sur_ensure_inference(c, c->cap))) return rc;
   rc = c->precision == CBZ_F64
            ? sur_resolve<double>(c, dt, iter_per_contact, tol, polish_per_contact, flags, c->sur.nfall)
            : sur_resolve<float>(c, dt, iter_per_contact, tol, polish_per_contact, flags, c->sur.nfall);
@@ -2303,7 +2375,66 @@ int cbz_step_sur(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contac
     return fail(c, CBZ_ERR_ARG, "step_sur: bad arguments");
   if (!c->sur.have_model) return fail(c, CBZ_ERR_STATE, "step_sur: no model");
   HIPCHK(c, hipSetDevice(c->device));
-  if ((rc = sur_ensure_inference(c, 32))) return rc;
+  if ((rc = sur_ensure_inference(c, c->cap))) return rc;
+  const bool f64 = c->precision == CBZ_F64;
+  if (!counts && !c->prof && n_steps > 0) {
+    // Replay a captured graph of one step, as cbz_step does: the host-side
+    // inputs are refreshed first, and every buffer the step uses exists.
+    rc = refresh_pow(c, dt);
+    if (rc == CBZ_OK) rc = refresh_brad(c);
+    if (rc) return rc;
+    if (n_steps > c->fb_cap) {
+      const int64_t cap = std::max<int64_t>(n_steps, 2 * c->fb_cap);
+      if ((rc = dalloc(c, &c->fb_hist, (size_t)cap))) return rc;
+      if (!c->fb_ctr && (rc = dalloc(c, &c->fb_ctr, 1))) return rc;
+      c->fb_cap = cap;
+      if (c->sgexec) (void)hipGraphExecDestroy(c->sgexec);
+      c->sgexec = nullptr;
+    }
+    if (!c->sgexec || c->sg_dt != dt || c->sg_mult != iter_per_contact || c->sg_tol != tol ||
+        c->sg_polish != polish_per_contact || c->sg_flags != flags || c->sg_gen != c->sur.generation) {
+      if (c->sgexec) (void)hipGraphExecDestroy(c->sgexec);
+      c->sgexec = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int crc = f64 ? do_integrate<double>(c, dt) : do_integrate<float>(c, dt);
+      if (crc == CBZ_OK) crc = f64 ? do_narrowphase<double>(c, -1) : do_narrowphase<float>(c, -1);
+      if (crc == CBZ_OK)
+        crc = f64 ? sur_resolve<double>(c, dt, iter_per_contact, tol, polish_per_contact, flags, c->sur.nfall)
+                  : sur_resolve<float>(c, dt, iter_per_contact, tol, polish_per_contact, flags, c->sur.nfall);
+      if (crc == CBZ_OK && launch_record_count(c->sur.nfall, c->fb_hist, c->fb_ctr, c->fb_cap, c->stream) != hipSuccess)
+        crc = fail(c, CBZ_ERR_HIP, "step_sur: record launch");
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (crc) {
+        if (g) (void)hipGraphDestroy(g);
+        return crc;
+      }
+      if (e != hipSuccess) return fail(c, CBZ_ERR_HIP, std::string("step_sur: capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&c->sgexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        c->sgexec = nullptr;
+        return fail(c, CBZ_ERR_HIP, std::string("step_sur: instantiate: ") + hipGetErrorString(e));
+      }
+      c->sg_dt = dt;
+      c->sg_mult = iter_per_contact;
+      c->sg_tol = tol;
+      c->sg_polish = polish_per_contact;
+      c->sg_flags = flags;
+      c->sg_gen = c->sur.generation;
+    }
+    HIPCHK(c, launch_fill_i32(c->overflow, 2, 0, c->stream));
+    HIPCHK(c, launch_fill_i32(c->fb_ctr, 1, 0, c->stream));
+    for (int32_t s = 0; s < n_steps; ++s) HIPCHK(c, hipGraphLaunch(c->sgexec, c->stream));
+    c->have_contacts = true;
+    if ((rc = check_overflow(c))) return rc;
+    if (fallbacks) {
+      std::vector<int32_t> f((size_t)n_steps);
+      HIPCHK(c, hipMemcpy(f.data(), c->fb_hist, f.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < f.size(); ++k) fallbacks[k] = f[k];
+    }
+    return CBZ_OK;
+  }
   int32_t* hist = nullptr;
   int32_t* fb = nullptr;
   if (n_steps > 0) {
@@ -2311,7 +2442,6 @@ int cbz_step_sur(cbz_ctx* c, double dt, int32_t n_steps, int64_t iter_per_contac
     HIPCHK(c, hipMalloc((void**)&fb, (size_t)n_steps * sizeof(int32_t)));
   }
   HIPCHK(c, launch_fill_i32(c->overflow, 2, 0, c->stream));
-  const bool f64 = c->precision == CBZ_F64;
   for (int32_t s = 0; s < n_steps && rc == CBZ_OK; ++s) {
     rc = f64 ? do_integrate<double>(c, dt) : do_integrate<float>(c, dt);
     if (rc) break;

@@ -57,6 +57,7 @@ struct ResolveLaunch {
   const int32_t* order;
   const int32_t* body_off;
   int64_t* world_iters;
+  int64_t* world_time = nullptr;         // [4][n_worlds] wall-clock ticks (see the resolver's RS)
   int32_t n_worlds;
   R dt;
   int64_t iter_mult;

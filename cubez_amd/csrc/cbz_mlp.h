@@ -44,8 +44,14 @@ struct GemmArgs {
 };
 
 // fused inference forward: all four layers per block of rows
+// Grid sizes of the grid-stride kernels of the surrogate step: whatever the
+// contact count, one launch of at most this many workgroups covers it.
+constexpr int kGridStrideBlocks = 8192;  // 256-thread workgroups (features)
+constexpr int kMlpGridBlocks = 1024;     // fused-forward workgroups (2 per CU fit its LDS)
+
 struct MlpArgs {
-  int M;                          // rows
+  int64_t M;                      // rows (the capacity when M_dev is set)
+  const int64_t* M_dev = nullptr; // rows counted on the device (clamped to M), or null
   const __bf16* X;                // [M][kFeatPad] normalised features
   const __bf16* W[kLayers];       // per layer [Npad][Kpad] row-major (the k_gemm Bt operand)
   const float* bias[kLayers];
@@ -71,8 +77,9 @@ struct SurApply {
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t st);
 hipError_t launch_mlp_forward(const MlpArgs& f, hipStream_t st);
 template <typename R>
-hipError_t launch_features(const Bodies<R>& b, const Contacts<R>& c, int64_t n, int32_t* deg, const float* mean,
-                           const float* inv_std, __bf16* X, float* raw, hipStream_t st);
+hipError_t launch_features(const Bodies<R>& b, const Contacts<R>& c, int64_t n_cap, const int64_t* n_dev,
+                           int32_t* deg, const float* mean, const float* inv_std, __bf16* X, float* raw,
+                           hipStream_t st);
 template <typename R>
 hipError_t launch_sur_scatter(const SurApply<R>& a, int n_worlds, hipStream_t st);
 hipError_t launch_gather_batch(const float* Xraw, const float* Yraw, const int64_t* idx, int rows, int valid_rows,
@@ -84,6 +91,7 @@ hipError_t launch_sample_idx(int64_t* idx, int n, uint64_t seed, uint64_t step, 
 hipError_t launch_col_stats(const float* A, int64_t rows, int cols, float* mean, float* stdv, hipStream_t st);
 hipError_t launch_count_u8(const uint8_t* m, int n, int32_t* out, hipStream_t st);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t st);
+hipError_t launch_record_count(const int32_t* value, int32_t* hist, int32_t* ctr, int64_t cap, hipStream_t st);
 hipError_t launch_mse(const float* Y, const float* T, int rows, int valid_rows, float* row_loss, __bf16* dY,
                       __bf16* dYt, int ldt, hipStream_t st);
 hipError_t launch_colsum(const __bf16* A, int rows, int cols, int ld, float* out, hipStream_t st);

@@ -77,6 +77,10 @@ struct Contacts {
   // kRestitution, as in the reference.
   R* fric;         // [cap]
   R* rest;         // [cap]
+  // resolver scratch: per contact the impulse data of the velocity phase
+  // (impulseMatrix 9, deltaVelocity entries 0, 3, 6), written when the
+  // phase starts
+  R* imp;          // [12][cap]
 };
 
 // Sources of the per-contact materials the narrowphase writes: friction and

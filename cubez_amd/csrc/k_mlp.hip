@@ -14,6 +14,7 @@
 //   k_silu_bwd, k_mse, k_colsum, k_adam   backward pass and optimiser
 //   k_sur_scatter per world: sum the predicted changes per body (the
 //                 resolver then polishes or re-solves the world)
+#include <algorithm>
 #include <climits>
 
 #include "cbz_devmath.h"
@@ -44,21 +45,32 @@ __device__ __forceinline__ float silu_grad(float z) {
 // ---------------------------------------------------------------------------
 // features
 // ---------------------------------------------------------------------------
-__global__ void k_degree(const int32_t* body0, const int32_t* body1, int64_t n, int32_t* deg) {
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  atomicAdd(&deg[body0[g]], 1);
-  if (body1[g] >= 0) atomicAdd(&deg[body1[g]], 1);
+// Rows handled by a grid-stride kernel: n_cap, or the device count n_dev
+// (read when the kernel runs) clamped to n_cap.  The grids of the surrogate
+// step are sized from capacities, so the step needs no host round trip and
+// can be captured in a graph.
+__device__ __forceinline__ int64_t rows_of(int64_t n_cap, const int64_t* n_dev) {
+  if (!n_dev) return n_cap;
+  const int64_t n = *n_dev;
+  return n < n_cap ? n : n_cap;
+}
+
+__global__ void k_degree(const int32_t* body0, const int32_t* body1, int64_t n_cap, const int64_t* n_dev,
+                         int32_t* deg) {
+  const int64_t n = rows_of(n_cap, n_dev);
+  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < n; g += int64_t(gridDim.x) * blockDim.x) {
+    atomicAdd(&deg[body0[g]], 1);
+    if (body1[g] >= 0) atomicAdd(&deg[body1[g]], 1);
+  }
 }
 
 // Feature layout (kFeat = 40): normal 3, rel0 3, rel1 3, v0-v1 3, w0-w1 3,
 // invMass0, invMass1, iitWorld0 9, iitWorld1 9, penetration,
 // desiredDeltaVelocity, contactVelocity.x, 1/deg0, 1/deg1.
 template <typename R>
-__global__ void k_features(Bodies<R> b, Contacts<R> c, int64_t n, const int32_t* deg, const float* mean,
-                           const float* inv_std, __bf16* X, float* raw) {
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= n) return;
+__device__ __forceinline__ void contact_features(const Bodies<R>& b, const Contacts<R>& c, int64_t g,
+                                                 const int32_t* deg, const float* mean, const float* inv_std,
+                                                 __bf16* X, float* raw) {
   const int64_t S = b.stride, CS = c.cap;
   const int b0 = c.body0[g], b1 = c.body1[g];
   float f[kFeat];
@@ -83,6 +95,14 @@ __global__ void k_features(Bodies<R> b, Contacts<R> c, int64_t n, const int32_t*
     for (int i = 0; i < kFeat; ++i) x[i] = __bf16((f[i] - mean[i]) * inv_std[i]);
     for (int i = kFeat; i < kFeatPad; ++i) x[i] = __bf16(0.f);
   }
+}
+
+template <typename R>
+__global__ void k_features(Bodies<R> b, Contacts<R> c, int64_t n_cap, const int64_t* n_dev, const int32_t* deg,
+                           const float* mean, const float* inv_std, __bf16* X, float* raw) {
+  const int64_t n = rows_of(n_cap, n_dev);
+  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < n; g += int64_t(gridDim.x) * blockDim.x)
+    contact_features(b, c, g, deg, mean, inv_std, X, raw);
 }
 
 // Batch sampling on the device: row i of training step `step` is
@@ -155,6 +175,15 @@ __global__ void k_count_u8(const uint8_t* m, int n, int32_t* out) {
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = part[0];
+}
+
+// Appends *value to hist at *ctr (while below cap) and advances *ctr: one
+// entry per replay of a captured step.
+__global__ void k_record_count(const int32_t* value, int32_t* hist, int32_t* ctr, int64_t cap) {
+  if (threadIdx.x != 0) return;
+  const int32_t i = *ctr;
+  if (i < cap) hist[i] = *value;
+  *ctr = i + 1;
 }
 
 __global__ void k_fill_f32(float* p, int64_t n, float v) {
@@ -306,11 +335,10 @@ constexpr int kColTiles = kHidden / 32 / kFuseWaves;  // 32-column tiles per wav
 
 __device__ __forceinline__ int fswz(int row, int k) { return row * kHidden + ((((k >> 3) ^ (row & 31))) << 3) + (k & 7); }
 
-__global__ void __launch_bounds__(64 * kFuseWaves) __attribute__((amdgpu_waves_per_eu(2, 2))) k_mlp_forward(MlpArgs f) {
-  __shared__ __align__(16) __bf16 H[kFuseRows * kHidden];
+// One block of kFuseRows rows starting at m0 (of M rows), all four layers.
+__device__ __forceinline__ void mlp_block(const MlpArgs& f, int64_t M, int64_t m0, __bf16* H) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 31, h = lane >> 5;
-  const int64_t m0 = int64_t(blockIdx.x) * kFuseRows;
   // The weights are the MFMA's first operand and the rows its second, so
   // acc[t][u][j] is feature 32 (wave kColTiles + u) + 8(j/4) + 4h + j%4 of tile row
   // 32t + r: a lane holds runs of 4 consecutive features of one row, stored
@@ -348,7 +376,7 @@ __global__ void __launch_bounds__(64 * kFuseWaves) __attribute__((amdgpu_waves_p
 #pragma unroll
     for (int t = 0; t < kRowTiles; ++t) {
       const int64_t row = m0 + 32 * t + r;
-      a[t] = row < f.M ? *(const bf16x8*)(f.X + row * kFeatPad + k + 8 * h) : bf16x8{};
+      a[t] = row < M ? *(const bf16x8*)(f.X + row * kFeatPad + k + 8 * h) : bf16x8{};
     }
 #pragma unroll
     for (int u = 0; u < kColTiles; ++u) b[u] = *(const bf16x8*)(f.W[0] + (wave * 32 * kColTiles + 32 * u + r) * kFeatPad + k + 8 * h);
@@ -394,7 +422,7 @@ __global__ void __launch_bounds__(64 * kFuseWaves) __attribute__((amdgpu_waves_p
       y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, y, 0, 0, 0);
     }
     const int64_t row = m0 + 32 * wave + r;
-    if (row < f.M) {
+    if (row < M) {
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const int col = 8 * q + 4 * h;
@@ -411,9 +439,19 @@ __global__ void __launch_bounds__(64 * kFuseWaves) __attribute__((amdgpu_waves_p
   }
 }
 
+__global__ void __launch_bounds__(64 * kFuseWaves) __attribute__((amdgpu_waves_per_eu(2, 2))) k_mlp_forward(MlpArgs f) {
+  __shared__ __align__(16) __bf16 H[kFuseRows * kHidden];
+  const int64_t M = rows_of(f.M, f.M_dev);
+  // grid-stride over blocks of kFuseRows rows
+  for (int64_t m0 = int64_t(blockIdx.x) * kFuseRows; m0 < M; m0 += int64_t(gridDim.x) * kFuseRows) {
+    mlp_block(f, M, m0, H);
+    __syncthreads();  // every wave is done with the tile before the next block's rows
+  }
+}
+
 hipError_t launch_mlp_forward(const MlpArgs& f, hipStream_t st) {
   if (f.M <= 0) return hipSuccess;
-  const int64_t blocks = (int64_t(f.M) + kFuseRows - 1) / kFuseRows;
+  const int64_t blocks = std::min<int64_t>((f.M + kFuseRows - 1) / kFuseRows, kMlpGridBlocks);
   k_mlp_forward<<<int(blocks), 64 * kFuseWaves, 0, st>>>(f);
   return hipGetLastError();
 }
@@ -494,14 +532,15 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t st) {
 }
 
 template <typename R>
-hipError_t launch_features(const Bodies<R>& b, const Contacts<R>& c, int64_t n, int32_t* deg, const float* mean,
-                           const float* inv_std, __bf16* X, float* raw, hipStream_t st) {
-  if (n <= 0) return hipSuccess;
+hipError_t launch_features(const Bodies<R>& b, const Contacts<R>& c, int64_t n_cap, const int64_t* n_dev,
+                           int32_t* deg, const float* mean, const float* inv_std, __bf16* X, float* raw,
+                           hipStream_t st) {
+  if (n_cap <= 0) return hipSuccess;
   hipError_t e = launch_fill_i32(deg, b.n, 0, st);
   if (e != hipSuccess) return e;
-  const int blocks = int((n + 255) / 256);
-  k_degree<<<blocks, 256, 0, st>>>(c.body0, c.body1, n, deg);
-  k_features<R><<<blocks, 256, 0, st>>>(b, c, n, deg, mean, inv_std, X, raw);
+  const int blocks = int(std::min<int64_t>((n_cap + 255) / 256, kGridStrideBlocks));
+  k_degree<<<blocks, 256, 0, st>>>(c.body0, c.body1, n_cap, n_dev, deg);
+  k_features<R><<<blocks, 256, 0, st>>>(b, c, n_cap, n_dev, deg, mean, inv_std, X, raw);
   return hipGetLastError();
 }
 
@@ -530,6 +569,10 @@ hipError_t launch_col_stats(const float* A, int64_t rows, int cols, float* mean,
 }
 hipError_t launch_count_u8(const uint8_t* m, int n, int32_t* out, hipStream_t st) {
   k_count_u8<<<1, 256, 0, st>>>(m, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_record_count(const int32_t* value, int32_t* hist, int32_t* ctr, int64_t cap, hipStream_t st) {
+  k_record_count<<<1, 64, 0, st>>>(value, hist, ctr, cap);
   return hipGetLastError();
 }
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t st) {
@@ -567,9 +610,11 @@ hipError_t launch_pack_weights(const float* W, int N, int K, __bf16* Wnk, int kp
   return hipGetLastError();
 }
 
-template hipError_t launch_features<double>(const Bodies<double>&, const Contacts<double>&, int64_t, int32_t*,
+template hipError_t launch_features<double>(const Bodies<double>&, const Contacts<double>&, int64_t, const int64_t*,
+                                            int32_t*,
                                             const float*, const float*, __bf16*, float*, hipStream_t);
-template hipError_t launch_features<float>(const Bodies<float>&, const Contacts<float>&, int64_t, int32_t*,
+template hipError_t launch_features<float>(const Bodies<float>&, const Contacts<float>&, int64_t, const int64_t*,
+                                           int32_t*,
                                            const float*, const float*, __bf16*, float*, hipStream_t);
 template hipError_t launch_sur_scatter<double>(const SurApply<double>&, int, hipStream_t);
 template hipError_t launch_sur_scatter<float>(const SurApply<float>&, int, hipStream_t);

@@ -1,0 +1,7 @@
+// k_resolve_f64_polish.hip — the FP64 resolver kernel, kind KIND_POLISH (cbz_resolve.h).
+#define CBZ_RESOLVE_INSTANTIATE
+#include "cbz_resolve.h"
+
+namespace cbz {
+CBZ_RESOLVE_KIND(, double, KIND_POLISH)
+}  // namespace cbz

@@ -191,6 +191,14 @@ class World:
         self._chk(self._lib.cbz_resolver_occupancy(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def resolver_world_times(self):
+        """[n_worlds, 4] microseconds: start and end of each world's
+        workgroup in the last position-only and velocity launches (-1:
+        none)."""
+        t = np.empty((self.n_worlds, 4))
+        self._chk(self._lib.cbz_resolver_world_times(self._h, _ptr(t)))
+        return t
+
     def resolver_iterations(self):
         it = np.empty(2 * self.n_worlds, np.int64)
         self._chk(self._lib.cbz_resolver_iterations(self._h, _ptr(it)))

@@ -232,6 +232,15 @@ int cbz_resolve(cbz_ctx *ctx, double dt, int64_t iter_per_contact, int64_t iter_
 /* iterations the last resolve used per world: [2*n_worlds] = positions,
  * velocities (the loop counters of contact.go:235 and :392) */
 int cbz_resolver_iterations(cbz_ctx *ctx, int64_t *iters);
+/* When each world's resolver workgroup ran in the last resolve(s):
+ * [n_worlds][4] microseconds -- start and end of the launch that ran the
+ * position phase alone, then start and end of the launch that ran the
+ * velocity phase (cbz_resolve runs both in one launch, recorded in the
+ * second pair) -- from the GPU's constant-rate clock, relative to the
+ * earliest recorded start; -1 where no such launch ran since cbz_set_scene.
+ * A world's end minus its start is the latency of its serial chain; no
+ * counterpart in the reference (a measurement of this implementation). */
+int cbz_resolver_world_times(cbz_ctx *ctx, double *us);
 /* Worlds the resolver keeps resident per CU (the plain kernel of the
  * context's precision: one wavefront per world, bounded by its registers
  * and LDS) and the device's CU count; the bench derives the time per

@@ -107,3 +107,35 @@ def test_gpu_reproduces_ballistic_and_batched_fixtures(golden):
         assert np.array_equal(counts, golden["batched_counts"])
         for wi in range(sc.n_worlds):
             assert state_hash(w.get, sc.world_slice(wi)) == golden["batched_hash"][wi], wi
+
+
+# --- configurations 1, 2 and 4 at their sizes ---------------------------------
+
+def test_oracle_reproduces_config_fixtures_in_part(golden):
+    """Slices of the config fixtures, recomputed by the oracle: config1's
+    first 60 steps, config2's smallest world over all 600 steps and its
+    largest over 60, and four tower worlds over 240 steps."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(os.path.dirname(GOLDEN), "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    counts, hashes, _ = _oracle_run(scenes.ballistic(0, 512, 512), 0, (1, 60))
+    assert np.array_equal(counts, golden["config1_counts"][:60])
+    assert [hashes[1], hashes[60]] == list(golden["config1_hash"][:2])
+    sc = scenes.batched(0, 4096)
+    ws = list(golden["config2_worlds"])
+    assert ws == mg.config2_worlds(sc)
+    n = np.diff(sc.body_off)
+    small, large = ws.index(int(np.argmin(n))), ws.index(int(np.argmax(n)))
+    counts, hashes, _ = _oracle_run(sc, ws[small], (1, 60, 600))
+    assert np.array_equal(counts, golden["config2_counts"][:, small])
+    assert [hashes[k] for k in (1, 60, 600)] == list(golden["config2_hash"][small])
+    counts, hashes, _ = _oracle_run(sc, ws[large], (1, 60))
+    assert np.array_equal(counts, golden["config2_counts"][:60, large])
+    assert [hashes[1], hashes[60]] == list(golden["config2_hash"][large][:2])
+    sc = mg.config4_scene()
+    for w in (0, 77, 128, 255):
+        counts, hashes, _ = _oracle_run(sc, w, (120, 240))
+        assert np.array_equal(counts, golden["config4_counts"][:, w])
+        assert [hashes[120], hashes[240]] == list(golden["config4_hash"][w])

@@ -317,3 +317,29 @@ def test_training_gains_over_the_method_without_the_network():
     it_zero, fb_zero = velocity_iterations(True)
     print("velocity iterations: network", it_net, "zero prediction", it_zero, "fallbacks", fb_net, fb_zero)
     assert it_net < it_zero
+
+
+def test_captured_surrogate_step_equals_eager_steps():
+    """cbz_step_sur without counts replays one captured graph per step (no
+    host round trip: the feature and network grids read the contact count on
+    the device); with counts it launches eagerly.  Replays in chunks of 3, 1
+    and 4 steps, with eager work and synchronisations between them, give the
+    eager path's state bit for bit and its per-step fallback counts."""
+    params, norm = surrogate.load()
+    sc = scenes.batched(5, 48)
+    with World(sc, precision="f32") as g, World(sc, precision="f32") as e:
+        for w in (g, e):
+            w.sur_set_norm(norm)
+            w.sur_set_params(params)
+            w.step(sc.dt, 200, sc.iter_mult)  # into the contact-rich regime
+        for chunk in (3, 1, 4):
+            fg = g.step_sur(sc.dt, chunk, sc.iter_mult, surrogate.DEFAULT_TOL, surrogate.DEFAULT_POLISH)
+            fe, counts = e.step_sur(sc.dt, chunk, sc.iter_mult, surrogate.DEFAULT_TOL, surrogate.DEFAULT_POLISH,
+                                    counts=True)
+            assert np.array_equal(fg, fe)
+            assert np.array_equal(g.contact_counts(), counts[-1])
+            for k in ("pos", "quat", "vel", "rot", "motion", "flags"):
+                assert np.array_equal(g.get(k), e.get(k)), (chunk, k)
+            g.narrowphase()  # eager work on the graph's context between replays
+            e.narrowphase()
+            g.synchronize()
