@@ -66,6 +66,7 @@ EXPORTS = (
     "cbz_set_materials", "cbz_set_plane_materials", "cbz_set_contact_materials",
     "cbz_checkpoint_size", "cbz_checkpoint_save", "cbz_checkpoint_load", "cbz_narrowphase_broad",
     "cbz_resolver_occupancy", "cbz_sur_check_gemms", "cbz_resolver_world_times",
+    "cbz_sur_predict_device",
 )
 
 
@@ -140,6 +141,7 @@ def load():
         "cbz_sur_predict": ([P, i32, P, P, i32], i32),
         "cbz_sur_check_gemms": ([i32, P, i32], i32),
         "cbz_resolver_world_times": ([P, P], i32),
+        "cbz_sur_predict_device": ([P, i32, P, P], i32),
         "cbz_sur_resolve": ([P, f64, i64, f64, i64, i32, P], i32),
         "cbz_sur_world_mask": ([P, P], i32),
         "cbz_step_sur": ([P, f64, i32, i64, f64, i64, i32, P, P], i32),

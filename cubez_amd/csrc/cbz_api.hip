@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -35,6 +36,36 @@ constexpr uint8_t kUserFlags = CBZ_FLAG_AWAKE | CBZ_FLAG_CAN_SLEEP | CBZ_FLAG_ST
 // real planes per contact slot (see cbz_context::c_real)
 constexpr int kContactPlanes = 40;
 
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// The allocation-free inference entry point (cbz_sur_predict_device) keeps
+// the normalised BF16 rows it feeds the network in this file-scope device
+// array: one copy per GPU, at an address each device resolves once (in
+// cbz_create) with hipGetSymbolAddress.  Host code uses that address, never
+// the array's name.  Every call on a GPU shares its copy.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocRows = 65536;
+static __device__ __align__(256) __bf16 scratch[kNoAllocRows * kFeatPad];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // per device ordinal; null until resolved
+
+// The device's copy of `scratch`, resolved on first use with that device
+// current (hipSetDevice done by the caller).
+hipError_t scratch_address(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
 }  // namespace
 
 struct cbz_context {
@@ -209,6 +240,7 @@ struct cbz_context {
   int64_t* d_prof = nullptr;           // [4] device-side step counters (k_prof_accum)
   std::vector<hipEvent_t> pev;         // per-step event pool of profiled cbz_step calls
   void* pos0 = nullptr;  // body positions at the last prepareContacts
+  void* scratch = nullptr;  // this device's copy of the file-scope scratch array
   // Shape check (cbz_sur_check_gemms): when set, the surrogate's GEMMs are
   // checked against these [begin, end) byte ranges instead of launched, and
   // its other kernels are skipped.
@@ -1236,6 +1268,7 @@ int cbz_create(const cbz_config* cfg, cbz_ctx** out) {
   bool ok = hipSetDevice(c->device) == hipSuccess &&
             hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   for (auto& e : c->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  ok = ok && scratch_address(c->device, &c->scratch) == hipSuccess;
   if (!ok) {
     cbz_destroy(c);
     return CBZ_ERR_HIP;
@@ -2330,6 +2363,37 @@ int cbz_sur_predict(cbz_ctx* c, int32_t rows, const float* X, float* Y, int32_t 
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(Y, u.Yp, (size_t)rows * kOut * sizeof(float), hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+int cbz_sur_predict_device(cbz_ctx* c, int32_t rows, const float* features, float* out) {
+  if (!c || rows < 0 || (rows > 0 && (!features || !out)))
+    return fail(c, CBZ_ERR_ARG, "sur_predict_device: bad arguments");
+  if (rows > kNoAllocRows)
+    return fail(c, CBZ_ERR_ARG, "sur_predict_device: " + std::to_string(rows) + " rows; at most " +
+                                    std::to_string(kNoAllocRows) + " per call");
+  if (!c->sur.have_model) return fail(c, CBZ_ERR_STATE, "sur_predict_device: no model");
+  if (!c->scratch) return fail(c, CBZ_ERR_STATE, "sur_predict_device: no scratch address");
+  if (rows == 0) return CBZ_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  auto& u = c->sur;
+  const float* nm = u.norm;
+  __bf16* X = (__bf16*)c->scratch;  // every row the network reads is written here first
+  HIPCHK(c, launch_gather_batch(features, nullptr, nullptr, rows, rows, nm, nm + kFeat, nm + 2 * kFeat,
+                                nm + 2 * kFeat + 2 * kOut, X, nullptr, nullptr, c->stream));
+  const LayerOff o = layer_offsets();
+  MlpArgs f{};
+  f.M = rows;
+  f.X = X;
+  for (int l = 0; l < kLayers; ++l) {
+    f.W[l] = u.Wnk[l];
+    f.bias[l] = u.params + o.b[l];
+  }
+  f.y_scale = u.norm + 2 * kFeat + kOut;  // out_std
+  f.y_shift = u.norm + 2 * kFeat;         // out_mean
+  f.Y = out;
+  HIPCHK(c, launch_mlp_forward(f, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return CBZ_OK;
 }
 

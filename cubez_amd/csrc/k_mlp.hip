@@ -118,21 +118,23 @@ __global__ void k_sample_idx(int64_t* idx, int n, uint64_t seed, uint64_t step, 
 }
 
 // raw features / labels of a sampled batch -> normalised BF16 inputs and
-// FP32 targets (training).
+// FP32 targets (training).  idx null takes rows 0..valid_rows-1 in order;
+// Xt (the transposed copy) and T (the targets) may be null.
 __global__ void k_gather_batch(const float* Xraw, const float* Yraw, const int64_t* idx, int rows, int valid_rows,
                                const float* in_mean, const float* in_inv_std, const float* out_mean,
                                const float* out_inv_std, __bf16* X, __bf16* Xt, float* T) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= rows) return;
   const bool v = i < valid_rows;
-  const int64_t s = v ? idx[i] : 0;
+  const int64_t s = v ? (idx ? idx[i] : int64_t(i)) : 0;  // idx null: row i itself
   for (int k = 0; k < kFeatPad; ++k) {
     const __bf16 x = __bf16(v && k < kFeat ? (Xraw[s * kFeat + k] - in_mean[k]) * in_inv_std[k] : 0.f);
     X[int64_t(i) * kFeatPad + k] = x;
-    Xt[int64_t(k) * rows + i] = x;
+    if (Xt) Xt[int64_t(k) * rows + i] = x;
   }
-  for (int k = 0; k < kOut; ++k)
-    T[int64_t(i) * kOut + k] = v && Yraw ? (Yraw[s * kOut + k] - out_mean[k]) * out_inv_std[k] : 0.f;
+  if (T)
+    for (int k = 0; k < kOut; ++k)
+      T[int64_t(i) * kOut + k] = v && Yraw ? (Yraw[s * kOut + k] - out_mean[k]) * out_inv_std[k] : 0.f;
 }
 
 // column mean and standard deviation of a row-major FP32 table (one block

@@ -322,6 +322,19 @@ int cbz_sur_grad(cbz_ctx *ctx, int32_t rows, const float *features, const float 
  * give the same bits) */
 #define CBZ_SUR_PREDICT_LAYERED 1
 int cbz_sur_predict(cbz_ctx *ctx, int32_t rows, const float *features, float *out, int32_t flags);
+/* cbz_sur_predict (fused inference) on device-resident rows, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  features: device pointer to
+ * [rows][40] raw FP32 features; out: device pointer to [rows][12] FP32
+ * predictions (denormalised), both caller-owned.  rows <= 65536
+ * (CBZ_ERR_ARG beyond).  The normalised BF16 rows go to a fixed 6 MB
+ * device array the library holds per GPU for the process's lifetime:
+ * calls that overlap on one GPU (several threads or contexts) race on it.
+ * Ordered on the context's stream (the caller's writes of `features` must
+ * be complete); returns after `out` is written.  The results are those of
+ * cbz_sur_predict, bit for bit. */
+int cbz_sur_predict_device(cbz_ctx *ctx, int32_t rows, const float *features, float *out);
 /* Host-only consistency check of the surrogate's matrix products (needs no
  * GPU): every GEMM of one training step and of one layered inference pass
  * over `rows` rows is compared with the sizes the library allocates for its
