@@ -203,12 +203,31 @@ def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+RESOLVER_SPLIT = "auto"  # --resolver-split: "auto", "default" or "min,max"
+TUNE_STEPS = 4
+
+
 def _start(w, sc, ck):
     if ck is None:
         w.load(sc)
     else:
         w.restore(ck)
+    if RESOLVER_SPLIT not in ("auto", "default"):
+        w.set_resolver_split(*(int(x) for x in RESOLVER_SPLIT.split(",")))
     w.synchronize()
+
+
+def _tune(w, sc, precision, surrogate=False, tol=0.06, polish=1):
+    """The resolver split of this leg: measured from the window's start
+    state by the autotuner (the state is restored), or as given."""
+    if RESOLVER_SPLIT != "auto":
+        return {"split": list(w.resolver_split()), "tuned": False}
+    from cubez_amd import autotune
+    r = autotune.tune(w, sc.dt, sc.iter_mult, precision, surrogate, steps=TUNE_STEPS, tol=tol, polish=polish,
+                      use_cache=False)
+    ms = r["ms_per_step"]
+    return {"split": list(r["split"]), "tuned": True, "tune_steps": TUNE_STEPS, "ms_per_step": ms,
+            "gain_vs_one_wave_only": ms["0,0"] / min(ms.values()) if "0,0" in ms else None}
 
 
 def residual_penetration(w):
@@ -240,6 +259,8 @@ def run_path(World, sc, ck, precision, steps, warmup):
     `steps` steps (graph replay), then the same steps again with HIP events
     between kernels."""
     w = World(sc, precision=precision)
+    _start(w, sc, ck)
+    split = _tune(w, sc, precision)
     _start(w, sc, ck)
     if warmup > 0:
         w.step(sc.dt, warmup, sc.iter_mult)
@@ -286,6 +307,7 @@ def run_path(World, sc, ck, precision, steps, warmup):
             "ns_per_iteration_per_world": resolve_ns * resident / (iters / sp) if iters else None,
         },
         "critical_path_last_step": crit,
+        "resolver_split": split,
         "k1_ms_per_launch": k1_ms,
         "k1_bodies_per_launch": k1_bodies / K1_LAUNCHES,
         "residual_penetration": resid,
@@ -300,6 +322,8 @@ def run_surrogate(World, sc, ck, model, steps, warmup, tol, polish, zero=False):
     w = World(sc, precision="f32")
     w.sur_set_norm(norm)
     w.sur_set_params(params)
+    _start(w, sc, ck)
+    split = _tune(w, sc, "f32", surrogate=True, tol=tol, polish=polish) if not zero else None
     _start(w, sc, ck)
     if warmup > 0:
         w.step_sur(sc.dt, warmup, sc.iter_mult, tol, polish, zero)
@@ -347,6 +371,7 @@ def run_surrogate(World, sc, ck, model, steps, warmup, tol, polish, zero=False):
         "contacts_total": contacts,
         "residual_penetration": resid,
         "critical_path_last_step": crit,
+        "resolver_split": split,
         "_final_pos": final_pos,
     })
     return out
@@ -413,6 +438,10 @@ def main():
     ap.add_argument("--exact-only", action="store_true", help="time the exact resolver only")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle wall time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--resolver-split", default="auto",
+                    help="the resolver's split between its kernels: 'auto' (the autotuner, per leg, from the "
+                         "window's start state), 'default' (the library's), or 'min,max' (the max largest worlds "
+                         "of at least min contacts take the multi-wave kernel; 0,0: none)")
     ap.add_argument("--probe-k1", action="store_true",
                     help="for hardware-counter runs: 20 launches of K1 (FP32 and FP64) on the workload, then 20 "
                          "copies of a 256 MiB float32 buffer as the calibration kernel; prints no JSON")
@@ -425,6 +454,8 @@ def main():
 
     from cubez_amd import World, scenes, surrogate
 
+    global RESOLVER_SPLIT
+    RESOLVER_SPLIT = args.resolver_split
     sc = scenes.batched(args.seed, args.worlds)
     if args.probe_k1:
         probe_k1(World, sc)
@@ -497,6 +528,11 @@ def main():
         out["critical_path_last_step"] = {"surrogate": sur["critical_path_last_step"],
                                           "exact_f64": exact64["critical_path_last_step"],
                                           "exact_f32": exact32["critical_path_last_step"]}
+        out["resolver_split"] = {"surrogate": sur["resolver_split"], "exact_f64": exact64["resolver_split"],
+                                 "exact_f32": exact32["resolver_split"],
+                                 "what": "(min_contacts, max_worlds): the max_worlds largest worlds of at least "
+                                         "min_contacts contacts resolve on the 4-wavefront kernel (0,0: none); "
+                                         "picked per leg by the autotuner from the window's start state"}
         out["contacts_per_step"] = sur["contacts_per_step"]
         out["contacts_per_s"] = sur["contacts_per_step"] * sur["value"] / nb
         phases.update(sur["resolver_ms_per_step"])

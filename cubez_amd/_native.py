@@ -65,7 +65,8 @@ EXPORTS = (
     "cbz_sur_train", "cbz_sur_grad", "cbz_sur_predict", "cbz_sur_resolve", "cbz_sur_world_mask", "cbz_sur_phase_times", "cbz_step_sur", "cbz_integrate_timed",
     "cbz_set_materials", "cbz_set_plane_materials", "cbz_set_contact_materials",
     "cbz_checkpoint_size", "cbz_checkpoint_save", "cbz_checkpoint_load", "cbz_narrowphase_broad",
-    "cbz_resolver_occupancy", "cbz_sur_check_gemms", "cbz_resolver_world_times",
+    "cbz_resolver_occupancy", "cbz_sur_check_gemms", "cbz_resolver_world_times", "cbz_set_resolver_split", "cbz_get_resolver_split",
+    "cbz_autotune_resolver",
     "cbz_sur_predict_device",
 )
 
@@ -141,6 +142,9 @@ def load():
         "cbz_sur_predict": ([P, i32, P, P, i32], i32),
         "cbz_sur_check_gemms": ([i32, P, i32], i32),
         "cbz_resolver_world_times": ([P, P], i32),
+        "cbz_set_resolver_split": ([P, i32, i32], i32),
+        "cbz_get_resolver_split": ([P, P, P], i32),
+        "cbz_autotune_resolver": ([P, f64, i64, i32, i32, f64, i64, i32, P, P, P, P], i32),
         "cbz_sur_predict_device": ([P, i32, P, P], i32),
         "cbz_sur_resolve": ([P, f64, i64, f64, i64, i32, P], i32),
         "cbz_sur_world_mask": ([P, P], i32),

@@ -35,6 +35,10 @@ const FieldInfo kFields[CBZ_NUM_FIELDS] = {
 constexpr uint8_t kUserFlags = CBZ_FLAG_AWAKE | CBZ_FLAG_CAN_SLEEP | CBZ_FLAG_STATIC;
 // real planes per contact slot (see cbz_context::c_real)
 constexpr int kContactPlanes = 40;
+// The resolver's multi-wave kernel takes the 32 largest worlds of more than
+// 256 contacts.
+constexpr int32_t kDefaultMwMin = 257;
+constexpr int32_t kDefaultMwMax = 32;
 
 }  // namespace
 
@@ -74,6 +78,13 @@ struct cbz_context {
   int64_t cap_cfg = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // the resolver's multi-wave kernel: the mw_max largest worlds of at
+  // least mw_min contacts run on stream `side` beside the one-wave kernel
+  hipStream_t side = nullptr;
+  hipEvent_t mw_fork = nullptr, mw_sel = nullptr, mw_join = nullptr;
+  int32_t mw_min = kDefaultMwMin;
+  int32_t mw_max = kDefaultMwMax;
+  int32_t n_cu = 256;
 
   // scene
   int32_t nw = 0;
@@ -162,6 +173,9 @@ struct cbz_context {
   int32_t* overflow = nullptr;  // [0] this narrowphase, [1] sticky since the host cleared it
   int64_t* world_iters = nullptr;
   int64_t* world_time = nullptr;  // [4 * nw] resolver workgroup start/end ticks (position, velocity launch)
+  int32_t* mw_list = nullptr;     // [nw] worlds of the multi-wave kernel (resolver launch)
+  int32_t* mw_count = nullptr;    // [2] list length, work-queue head
+  uint8_t* mw_flag = nullptr;     // [nw] listed
   int32_t* order = nullptr;
   bool have_contacts = false;
 
@@ -417,6 +431,9 @@ void free_scene(cbz_ctx* c) {
   dfree(c->overflow);
   dfree(c->world_iters);
   dfree(c->world_time);
+  dfree(c->mw_list);
+  dfree(c->mw_count);
+  dfree(c->mw_flag);
   dfree(c->order);
   dfree(c->d_awake);
   dfree(c->d_prof);
@@ -629,6 +646,16 @@ int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode 
   L.body_off = c->d_body_off;
   L.world_iters = c->world_iters;
   L.world_time = c->world_time;
+  L.mw_min = c->mw_min;
+  L.mw_max = c->mw_max;
+  L.mw_list = c->mw_list;
+  L.mw_count = c->mw_count;
+  L.mw_flag = c->mw_flag;
+  L.side = c->side;
+  L.fork = c->mw_fork;
+  L.sel = c->mw_sel;
+  L.join = c->mw_join;
+  L.n_cu = c->n_cu;
   L.n_worlds = c->nw;
   L.dt = (R)dt;
   L.iter_mult = mult;
@@ -1268,6 +1295,11 @@ int cbz_create(const cbz_config* cfg, cbz_ctx** out) {
   bool ok = hipSetDevice(c->device) == hipSuccess &&
             hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   for (auto& e : c->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) == hipSuccess &&
+       hipEventCreateWithFlags(&c->mw_fork, hipEventDisableTiming) == hipSuccess &&
+       hipEventCreateWithFlags(&c->mw_sel, hipEventDisableTiming) == hipSuccess &&
+       hipEventCreateWithFlags(&c->mw_join, hipEventDisableTiming) == hipSuccess &&
+       hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess;
   ok = ok && scratch_address(c->device, &c->scratch) == hipSuccess;
   if (!ok) {
     cbz_destroy(c);
@@ -1287,6 +1319,11 @@ int cbz_destroy(cbz_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->pev)
     if (e) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamSynchronize(c->side);
+  if (c->mw_fork) (void)hipEventDestroy(c->mw_fork);
+  if (c->mw_sel) (void)hipEventDestroy(c->mw_sel);
+  if (c->mw_join) (void)hipEventDestroy(c->mw_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CBZ_OK;
@@ -1407,6 +1444,9 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc(c, &c->world_iters, 2 * (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->world_time, 4 * (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->order, (size_t)n_worlds))) return rc;
+  if ((rc = dalloc(c, &c->mw_list, (size_t)n_worlds))) return rc;
+  if ((rc = dalloc(c, &c->mw_count, 2))) return rc;
+  if ((rc = dalloc(c, &c->mw_flag, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->d_awake, kAwakeSlots))) return rc;
   if ((rc = dalloc(c, &c->d_prof, 4))) return rc;
   if ((rc = dalloc_bytes(c, &c->pos0, rs * 3 * (size_t)std::max<int64_t>(nb, 1)))) return rc;
@@ -1829,6 +1869,96 @@ int cbz_resolver_occupancy(cbz_ctx* c, int32_t* worlds_per_cu, int32_t* n_cu) {
   if (worlds_per_cu) *worlds_per_cu = w;
   if (n_cu) *n_cu = cu;
   return CBZ_OK;
+}
+
+int cbz_set_resolver_split(cbz_ctx* c, int32_t min_contacts, int32_t max_worlds) {
+  if (!c) return CBZ_ERR_ARG;
+  if (min_contacts < 0 || max_worlds < 0)
+    return fail(c, CBZ_ERR_ARG, "set_resolver_split: min_contacts and max_worlds must be >= 0");
+  if (min_contacts != c->mw_min || max_worlds != c->mw_max) {
+    drop_graph(c);  // captured launches carry the split
+    c->mw_min = min_contacts;
+    c->mw_max = max_worlds;
+  }
+  return CBZ_OK;
+}
+
+int cbz_get_resolver_split(cbz_ctx* c, int32_t* min_contacts, int32_t* max_worlds) {
+  if (!c) return CBZ_ERR_ARG;
+  if (min_contacts) *min_contacts = c->mw_min;
+  if (max_worlds) *max_worlds = c->mw_max;
+  return CBZ_OK;
+}
+
+int cbz_autotune_resolver(cbz_ctx* c, double dt, int64_t iter_per_contact, int32_t n_steps, int32_t surrogate,
+                          double tol, int64_t polish_per_contact, int32_t n_cand, const int32_t* cand_min,
+                          const int32_t* cand_max, double* ms_per_step, int32_t* best) {
+  int rc = need_scene(c);
+  if (rc) return rc;
+  if (n_steps <= 0 || n_cand <= 0 || !cand_min || !cand_max)
+    return fail(c, CBZ_ERR_ARG, "autotune_resolver: bad arguments");
+  if (c->prof) return fail(c, CBZ_ERR_STATE, "autotune_resolver: profiling is on");
+  for (int32_t i = 0; i < n_cand; ++i)
+    if (cand_min[i] < 0 || cand_max[i] < 0) return fail(c, CBZ_ERR_ARG, "autotune_resolver: negative split");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // the body state on the device, restored before every timed run
+  std::vector<size_t> off(CBZ_NUM_FIELDS + 1, 0), len(CBZ_NUM_FIELDS, 0);
+  for (int f = 0; f < CBZ_NUM_FIELDS; ++f) {
+    const FieldInfo& fi = kFields[f];
+    const size_t el = fi.kind == 0 ? real_size(c) : fi.kind == 1 ? sizeof(int32_t) : sizeof(uint8_t);
+    len[f] = fi.kind == 3 ? 0 : el * fi.width * (size_t)c->nb;
+    off[f + 1] = off[f] + ((len[f] + 7) & ~size_t(7));
+  }
+  void* snap = nullptr;
+  if ((rc = dalloc_bytes(c, &snap, std::max<size_t>(off[CBZ_NUM_FIELDS], 8)))) return rc;
+  auto copy = [&](bool save) -> hipError_t {
+    for (int f = 0; f < CBZ_NUM_FIELDS; ++f) {
+      const size_t n = len[f];
+      if (!n || !c->field[f]) continue;
+      char* d = (char*)snap + off[f];
+      const hipError_t e = save ? hipMemcpyAsync(d, c->field[f], n, hipMemcpyDeviceToDevice, c->stream)
+                                : hipMemcpyAsync(c->field[f], d, n, hipMemcpyDeviceToDevice, c->stream);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  };
+  const int32_t old_min = c->mw_min, old_max = c->mw_max;
+  std::vector<int64_t> fb((size_t)n_steps + 1);
+  auto run = [&](int32_t steps) {
+    return surrogate ? cbz_step_sur(c, dt, steps, iter_per_contact, tol, polish_per_contact, 0, nullptr, fb.data())
+                     : cbz_step(c, dt, steps, iter_per_contact, nullptr);
+  };
+  int32_t pick = -1;
+  double pick_ms = 0.0;
+  hipError_t e = copy(true);
+  for (int32_t i = 0; e == hipSuccess && !rc && i < n_cand; ++i) {
+    if ((rc = cbz_set_resolver_split(c, cand_min[i], cand_max[i]))) break;
+    // one untimed step captures the step for this split
+    if ((e = copy(false)) != hipSuccess || (rc = run(1))) break;
+    if ((e = copy(false)) != hipSuccess || (e = hipEventRecord(c->ev[0], c->stream)) != hipSuccess) break;
+    if ((rc = run(n_steps))) break;
+    if ((e = hipEventRecord(c->ev[1], c->stream)) != hipSuccess || (e = hipEventSynchronize(c->ev[1])) != hipSuccess)
+      break;
+    float ms = 0.f;
+    if ((e = hipEventElapsedTime(&ms, c->ev[0], c->ev[1])) != hipSuccess) break;
+    const double per = double(ms) / n_steps;
+    if (ms_per_step) ms_per_step[i] = per;
+    if (pick < 0 || per < pick_ms) {
+      pick = i;
+      pick_ms = per;
+    }
+  }
+  if (e == hipSuccess) e = copy(false);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dfree(snap);
+  if (!rc && e != hipSuccess) rc = fail(c, CBZ_ERR_HIP, std::string("autotune_resolver: ") + hipGetErrorString(e));
+  if (rc) {
+    cbz_set_resolver_split(c, old_min, old_max);
+    return rc;
+  }
+  if (best) *best = pick;
+  return cbz_set_resolver_split(c, cand_min[pick], cand_max[pick]);
 }
 
 int cbz_resolver_world_times(cbz_ctx* c, double* us) {

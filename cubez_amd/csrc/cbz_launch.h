@@ -58,6 +58,18 @@ struct ResolveLaunch {
   const int32_t* body_off;
   int64_t* world_iters;
   int64_t* world_time = nullptr;         // [4][n_worlds] wall-clock ticks (see the resolver's RS)
+  // The multi-wave kernel for the mw_max largest worlds of at least mw_min
+  // contacts (either 0: off), on stream `side`, forked from and joined back
+  // into the launch's stream by the events; mw_list [n_worlds], mw_count [2]
+  // and mw_flag [n_worlds] are its work list.
+  int32_t mw_min = 0;
+  int32_t mw_max = 0;
+  int32_t* mw_list = nullptr;
+  int32_t* mw_count = nullptr;
+  uint8_t* mw_flag = nullptr;
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, sel = nullptr, join = nullptr;
+  int32_t n_cu = 256;
   int32_t n_worlds;
   R dt;
   int64_t iter_mult;

@@ -55,6 +55,12 @@ struct RS {
   // [4 * n_worlds] (or null): wall-clock ticks when the world's workgroup
   // started and ended, for a launch without (0, 1) and with (2, 3) RM_VEL
   int64_t* world_time;
+  // worlds taken by the multi-wave kernel (resolve_mw), chosen by
+  // k_mw_select: mw_list[0 .. mw_count[0]), flagged in mw_flag; mw_count[1]
+  // is the kernel's work-queue head (null mw_list: none)
+  const int32_t* mw_list;
+  int32_t* mw_count;
+  const uint8_t* mw_flag;
   int32_t n_worlds;
   R dt;
   int64_t iter_mult;
@@ -328,22 +334,22 @@ __device__ __forceinline__ void skew(const V3<R>& v, R* m) {
 // contact never adds its second body's terms, and a half-space contact
 // dereferences nil there (a Go panic; here the absent body adds nothing).
 // two_body (RM_FRICTIONLESS2) adds the second body's terms when it exists.
+// im: the two bodies' inverse masses (im[1] unused without a second body).
 template <typename R>
-__device__ __forceinline__ R frictionless_delta_velocity(const Bodies<R>& b, const int kb[2], const R* iit0,
-                                                         const R* iit1, const V3<R> rel[2], const R* c2w,
-                                                         bool two_body) {
+__device__ __forceinline__ R frictionless_delta_velocity(const R im[2], bool has1, const R* iit0, const R* iit1,
+                                                         const V3<R> rel[2], const R* c2w, bool two_body) {
   const V3<R> n{c2w[0], c2w[1], c2w[2]};
   V3<R> dvw = cross(rel[0], n);
   dvw = m3v(iit0, dvw);
   dvw = cross(dvw, rel[0]);
   R deltaVelocity = dot(dvw, n);
-  deltaVelocity += b.inv_mass[kb[0]];
-  if (two_body && kb[1] >= 0) {
+  deltaVelocity += im[0];
+  if (two_body && has1) {
     dvw = cross(rel[1], n);
     dvw = m3v(iit1, dvw);
     dvw = cross(dvw, rel[1]);
     deltaVelocity += dot(dvw, n);
-    deltaVelocity += b.inv_mass[kb[1]];
+    deltaVelocity += im[1];
   }
   return deltaVelocity;
 }
@@ -357,16 +363,16 @@ __device__ __forceinline__ R frictionless_delta_velocity(const Bodies<R>& b, con
 // phase for every contact.  It keeps the inverse (9) and deltaVelocity's
 // entries 0, 3, 6 (the only ones the friction cone reads).
 template <typename R>
-__device__ __forceinline__ void friction_matrices(const Bodies<R>& b, const int kb[2], const R* iit0, const R* iit1,
+__device__ __forceinline__ void friction_matrices(const R invm[2], bool has1, const R* iit0, const R* iit1,
                                                   const V3<R> rel[2], const R* c2w, R* im, R* dvcol) {
-  R inverseMass = b.inv_mass[kb[0]];
+  R inverseMass = invm[0];
   R sk[9], t[9], dvw[9];
   skew(rel[0], sk);
   m3mul(sk, iit0, t);
   m3mul(t, sk, dvw);
 #pragma unroll
   for (int k = 0; k < 9; ++k) dvw[k] *= R(-1);
-  if (kb[1] >= 0) {
+  if (has1) {
     R d2[9];
     skew(rel[1], sk);
     m3mul(sk, iit1, t);
@@ -375,7 +381,7 @@ __device__ __forceinline__ void friction_matrices(const Bodies<R>& b, const int 
     for (int k = 0; k < 9; ++k) d2[k] *= R(-1);
 #pragma unroll
     for (int k = 0; k < 9; ++k) dvw[k] += d2[k];
-    inverseMass += b.inv_mass[kb[1]];
+    inverseMass += invm[1];
   }
   R ct[9] = {c2w[0], c2w[3], c2w[6], c2w[1], c2w[4], c2w[7], c2w[2], c2w[5], c2w[8]};
   R dv[9];
@@ -416,6 +422,19 @@ __device__ __forceinline__ V3<R> friction_impulse_from(const R* im, const R* dvc
 // calculateFrictionlessImpulse's deltaVelocity in entry 0.
 constexpr int kImpPlanes = 12;
 
+// From the bodies' world inverse inertia and inverse masses as given.
+template <typename R, int KIND>
+__device__ __forceinline__ void impulse_data_from(int32_t mode, const R* iit0, const R* iit1, const R invm[2], bool has1,
+                                                  const V3<R> rel[2], const R* c2w, R friction, R* out) {
+  if (KIND == KIND_MAT && friction == R(0)) {
+    out[0] = frictionless_delta_velocity(invm, has1, iit0, iit1, rel, c2w, (mode & RM_FRICTIONLESS2) != 0);
+#pragma unroll
+    for (int k = 1; k < kImpPlanes; ++k) out[k] = R(0);
+  } else {
+    friction_matrices(invm, has1, iit0, iit1, rel, c2w, out, out + 9);
+  }
+}
+
 template <typename R, int KIND>
 __device__ __forceinline__ void impulse_data(const RS<R>& a, const int kb[2], const V3<R> rel[2], const R* c2w,
                                              R friction, R* out) {
@@ -426,13 +445,8 @@ __device__ __forceinline__ void impulse_data(const RS<R>& a, const int kb[2], co
 #pragma unroll
   for (int k = 0; k < 9; ++k) iit1[k] = R(0);
   if (kb[1] >= 0) ld9(b.iit_world, S, kb[1], iit1);
-  if (KIND == KIND_MAT && friction == R(0)) {
-    out[0] = frictionless_delta_velocity(b, kb, iit0, iit1, rel, c2w, (a.mode & RM_FRICTIONLESS2) != 0);
-#pragma unroll
-    for (int k = 1; k < kImpPlanes; ++k) out[k] = R(0);
-  } else {
-    friction_matrices(b, kb, iit0, iit1, rel, c2w, out, out + 9);
-  }
+  const R invm[2] = {b.inv_mass[kb[0]], kb[1] >= 0 ? b.inv_mass[kb[1]] : R(0)};
+  impulse_data_from<R, KIND>(a.mode, iit0, iit1, invm, kb[1] >= 0, rel, c2w, friction, out);
 }
 
 // applyVelocityChange, contact.go:448-494, for a contact of the given
@@ -814,17 +828,37 @@ struct RegLds {
   static constexpr size_t bytes = flags + kRegBodies;
 };
 
+// Both bodies' world inverse inertia and inverse mass (zeros for an absent
+// second body).
+template <typename R>
+__device__ __forceinline__ void body_inertia(const Bodies<R>& b, const int kb[2], R (&iw)[2][9], R im[2]) {
+  const int64_t S = b.stride;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) iw[i][k] = R(0);
+    im[i] = R(0);
+  }
+  ld9(b.iit_world, S, kb[0], iw[0]);
+  im[0] = b.inv_mass[kb[0]];
+  if (kb[1] >= 0) {
+    ld9(b.iit_world, S, kb[1], iw[1]);
+    im[1] = b.inv_mass[kb[1]];
+  }
+}
+
 // applyPositionChange (contact.go:286-386) with the bodies' position and
 // orientation in LDS (s_st: pos planes 0-2, quat planes 3-6, world-local
 // index kl).  Returns whether a sleeping body's derived data were rewritten
 // (in HBM).
+// The bodies' world inverse inertia iw and inverse masses im are loaded by
+// the caller (with the iteration's other loads, before any store).
 template <typename R>
 __device__ __forceinline__ bool apply_position_change_lds(const RS<R>& a, const int kb[2], const int kl[2],
                                                           const bool aw[2], const V3<R>& n, const V3<R> rel[2],
-                                                          R penetration, V3<R> lin[2], V3<R> ang[2], R* s_st,
-                                                          int lane) {
+                                                          R penetration, const R (&iw)[2][9], const R im[2],
+                                                          V3<R> lin[2], V3<R> ang[2], R* s_st, int lane) {
   const Bodies<R>& b = a.b;
-  const int64_t S = b.stride;
   constexpr int KB = kRegBodies;
   const R angularLimit = R(0.2);
   R angI[2] = {R(0), R(0)}, linI[2] = {R(0), R(0)};
@@ -835,13 +869,11 @@ __device__ __forceinline__ bool apply_position_change_lds(const RS<R>& a, const 
     lin[i] = V3<R>{R(0), R(0), R(0)};
     ang[i] = V3<R>{R(0), R(0), R(0)};
     if (kb[i] < 0) continue;
-    R iw[9];
-    ld9(b.iit_world, S, kb[i], iw);
     V3<R> aiw = cross(rel[i], n);
-    aiw = m3v(iw, aiw);
+    aiw = m3v(iw[i], aiw);
     aiw = cross(aiw, rel[i]);
     angI[i] = dot(aiw, n);
-    linI[i] = b.inv_mass[kb[i]];
+    linI[i] = im[i];
     total += linI[i] + angI[i];
   }
 #pragma unroll
@@ -867,9 +899,7 @@ __device__ __forceinline__ bool apply_position_change_lds(const RS<R>& a, const 
       ang[i] = V3<R>{R(0), R(0), R(0)};
     } else {
       V3<R> tad = cross(rel[i], n);
-      R iw[9];
-      ld9(b.iit_world, S, bi, iw);
-      ang[i] = mul(m3v(iw, tad), angM / angI[i]);
+      ang[i] = mul(m3v(iw[i], tad), angM / angI[i]);
     }
     lin[i] = mul(n, linM);
     const int l = kl[i];
@@ -896,12 +926,10 @@ __device__ __forceinline__ bool apply_position_change_lds(const RS<R>& a, const 
 // with the bodies' velocity and rotation in LDS (s_st: vel planes 0-2, rot
 // planes 3-5, world-local index kl).
 template <typename R, int KIND>
-__device__ __forceinline__ void apply_velocity_change_lds(const RS<R>& a, const int kb[2], const int kl[2],
-                                                          const V3<R> rel[2], const R* c2w, R ddv, const V3<R>& cvel,
-                                                          const R* imp, V3<R> vc[2], V3<R> rc[2], R* s_st, int lane,
-                                                          R friction) {
-  const Bodies<R>& b = a.b;
-  const int64_t S = b.stride;
+__device__ __forceinline__ void apply_velocity_change_lds(const int kb[2], const int kl[2], const V3<R> rel[2],
+                                                          const R* c2w, R ddv, const V3<R>& cvel, const R* imp,
+                                                          const R (&iw)[2][9], const R im[2], V3<R> vc[2],
+                                                          V3<R> rc[2], R* s_st, int lane, R friction) {
   R* s_rot = s_st + 3 * kRegBodies;
   V3<R> ic;
   if (KIND == KIND_MAT && friction == R(0))
@@ -910,21 +938,18 @@ __device__ __forceinline__ void apply_velocity_change_lds(const RS<R>& a, const 
     ic = friction_impulse_from(imp, imp + 9, ddv, cvel, friction);
   const V3<R> impulse = m3v(c2w, ic);
   const V3<R> zero{R(0), R(0), R(0)};
-  R iw[9];
-  ld9(b.iit_world, S, kb[0], iw);
   V3<R> torque = cross(rel[0], impulse);
-  rc[0] = m3v(iw, torque);
-  vc[0] = add_scaled(zero, impulse, b.inv_mass[kb[0]]);
+  rc[0] = m3v(iw[0], torque);
+  vc[0] = add_scaled(zero, impulse, im[0]);
   rc[1] = zero;
   vc[1] = zero;
   const V3<R> v0 = add(lds3(s_st, kl[0]), vc[0]);
   const V3<R> w0 = add(lds3(s_rot, kl[0]), rc[0]);
   V3<R> v1 = zero, w1 = zero;
   if (kb[1] >= 0) {
-    ld9(b.iit_world, S, kb[1], iw);
     torque = cross(impulse, rel[1]);
-    rc[1] = m3v(iw, torque);
-    vc[1] = add_scaled(zero, impulse, -b.inv_mass[kb[1]]);
+    rc[1] = m3v(iw[1], torque);
+    vc[1] = add_scaled(zero, impulse, -im[1]);
     v1 = add(lds3(s_st, kl[1]), vc[1]);
     w1 = add(lds3(s_rot, kl[1]), rc[1]);
   }
@@ -1085,6 +1110,9 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
       rel[0] = sub(p, lds3(s_pos0, kl[0]));
       rel[1] = kl[1] >= 0 ? sub(p, lds3(s_pos0, kl[1])) : zero;
     }
+    // both bodies' inertia, loaded before this iteration's first store
+    R iw[2][9], im[2];
+    body_inertia(b, kb, iw, im);
     // matchAwakeState, contact.go:185-202
     bool aw[2] = {(s_flags[kl[0]] & F_AWAKE) != 0, kl[1] >= 0 && (s_flags[kl[1]] & F_AWAKE) != 0};
     if (kl[1] >= 0 && (aw[0] || aw[1]) && !(aw[0] && aw[1])) {
@@ -1096,7 +1124,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
       aw[0] = aw[1] = true;
     }
     V3<R> lin[2], ang[2];
-    const bool rederived = apply_position_change_lds(a, kb, kl, aw, n, rel, best, lin, ang, s_st, lane);
+    const bool rederived = apply_position_change_lds(a, kb, kl, aw, n, rel, best, iw, im, lin, ang, s_st, lane);
     const int ln = opaque_lane(lane);
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
@@ -1154,7 +1182,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
   // VGPRs where they fit, else in the contact array, where each slot is
   // read and written only by the lane that owns it (the selected one reaches
   // the other lanes by readlane).
-  constexpr bool CV_REG = SLOTS * sizeof(R) <= 32;
+  constexpr bool CV_REG = SLOTS * sizeof(R) <= 64;
   R* s_rot = s_st + 3 * KB;
   V3<R> cvr[CV_REG ? SLOTS : 1];
   // slot s of this lane, contact g = base + 64 s + lane
@@ -1382,6 +1410,10 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
       }
 #pragma unroll
       for (int k = 0; k < kImpPlanes; ++k) imp[k] = c.imp[k * CS + base + bi];
+      // both bodies' inertia, loaded with the impulse data before this
+      // iteration's first store
+      R iw[2][9], im[2];
+      body_inertia(b, kb, iw, im);
       // matchAwakeState, contact.go:185-202
       const bool aw0 = (s_flags[kl[0]] & F_AWAKE) != 0;
       const bool aw1 = kl[1] >= 0 && (s_flags[kl[1]] & F_AWAKE) != 0;
@@ -1393,7 +1425,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
         }
       }
       V3<R> vc[2], rc[2];
-      apply_velocity_change_lds<R, KIND>(a, kb, kl, rel, m, best, cvk, imp, vc, rc, s_st, lane,
+      apply_velocity_change_lds<R, KIND>(kb, kl, rel, m, best, cvk, imp, iw, im, vc, rc, s_st, lane,
                                          friction_of<R, KIND>(c, base + bi));
       if constexpr (KIND == KIND_LABEL)
         if (lane == 0) add_label(a, base + bi, vc, rc);
@@ -1462,6 +1494,766 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
   }
 }
 
+// ----------------------------------------------------------------------
+// The largest worlds: one workgroup of kMwWaves wavefronts per world.
+//
+// In a world of many hundred contacts, an iteration of the one-wave kernel
+// scans up to 32 contacts per lane and reads the selected contact's impulse
+// data, the bodies' inertia and the contact velocities from HBM: a chain
+// of dependent memory round trips.  Here thread t (of kMwThreads, one wave
+// per SIMD) holds contacts j = t + kMwThreads * s (s < SL, SL = 4 or 8):
+// their selection keys, body ids, basis scales, contact velocities and
+// impulse data in registers; their normals and points are in LDS by
+// contact, beside the bodies' state, world inverse inertia, inverse mass
+// and flags.  An iteration:
+//   1. each wave takes its best contact (larger key; equal keys -> lower
+//      index) by DPP, and the lane that holds it offers its register data
+//      in LDS;                                              barrier A
+//   2. wave 0 alone takes the best of the offers, computes the correction
+//      and stores the bodies' new state (lane 0; wave 0 is the only reader
+//      of that state), then publishes the two bodies' changes; barrier C
+//   3. every thread updates its own contacts that share one of the bodies.
+// The arithmetic, its order and the selection are those of resolve_reg, so
+// both give the same bits.
+constexpr int kMwWaves = 4;
+constexpr int kMwThreads = 64 * kMwWaves;
+constexpr int kMwMaxSlots = 8;
+constexpr int kMwMaxContacts = kMwThreads * kMwMaxSlots;  // 2048
+// per-wave offer: contact velocity 3, impulse data 12, friction 1
+constexpr int kMwPay = 16;
+// wave 0's result: state, packed body ids, woken body
+enum : int { MW_APPLY = 0, MW_DONE = 1, MW_BUDGET = 2 };
+
+// A world the multi-wave kernel can take (the selection then keeps the
+// largest of these).
+__device__ __forceinline__ bool mw_fits(int C, int n, int mw_min) {
+  return mw_min > 0 && C >= mw_min && C <= kMwMaxContacts && n <= kRegBodies;
+}
+
+constexpr size_t up8(size_t x) { return (x + 7) & ~size_t(7); }
+
+template <typename R>
+struct MwLds {
+  static constexpr size_t pos0 = 0;                                   // R[3][kRegBodies]
+  static constexpr size_t lad = pos0 + 3 * kRegBodies * sizeof(R);    // R[3][kRegBodies]
+  static constexpr size_t st = lad + 3 * kRegBodies * sizeof(R);      // R[7][kRegBodies]
+  static constexpr size_t iit = st + 7 * kRegBodies * sizeof(R);      // R[9][kRegBodies]
+  static constexpr size_t im = iit + 9 * kRegBodies * sizeof(R);      // R[kRegBodies]
+  static constexpr size_t cn = im + kRegBodies * sizeof(R);           // R[3][kMwMaxContacts] normals
+  static constexpr size_t cp = cn + 3 * kMwMaxContacts * sizeof(R);   // R[3][kMwMaxContacts] points
+  static constexpr size_t pbs = up8(cp + 3 * kMwMaxContacts * sizeof(R));  // double[kMwWaves] offered basis scale
+  static constexpr size_t sum = pbs + kMwWaves * sizeof(double);      // double[kMwWaves]
+  static constexpr size_t pay = sum + kMwWaves * sizeof(double);      // R[kMwWaves][kMwPay]
+  static constexpr size_t res = up8(pay + kMwWaves * kMwPay * sizeof(R));  // R[12] wave 0's result
+  static constexpr size_t pkey = up8(res + 12 * sizeof(R));           // R[kMwWaves]
+  static constexpr size_t pidx = up8(pkey + kMwWaves * sizeof(R));    // int32[kMwWaves]
+  static constexpr size_t pkid = pidx + kMwWaves * sizeof(int32_t);   // uint32[kMwWaves]
+  static constexpr size_t resi = pkid + kMwWaves * sizeof(uint32_t);  // int32[4] state, ids, woken body
+  static constexpr size_t flags = resi + 4 * sizeof(int32_t);         // uint8[kRegBodies]
+  static constexpr size_t bytes = flags + kRegBodies;
+};
+
+// Sum over the workgroup in a fixed order (each wave's butterfly, then the
+// waves in index order); every thread gets the result.
+__device__ __forceinline__ double mw_sum(double v, double* s_sum, int t) {
+  v = wave_sum_f64(v);
+  if ((t & 63) == 0) s_sum[t >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int k = 0; k < kMwWaves; ++k) r += s_sum[k];
+  __syncthreads();
+  return r;
+}
+
+// The best of the waves' offers (larger key; equal keys -> lower index);
+// gw = -1 when no wave offered a contact.
+template <typename R>
+__device__ __forceinline__ int mw_best(const R* s_pkey, const int32_t* s_pidx, R& key, int& idx) {
+  int gw = -1;
+#pragma unroll
+  for (int v = 0; v < kMwWaves; ++v) {
+    const int i = s_pidx[v];
+    if (i == INT_MAX) continue;
+    const R k = s_pkey[v];
+    if (k > key || (k == key && i < idx)) {
+      key = k;
+      idx = i;
+      gw = v;
+    }
+  }
+  return gw;
+}
+
+// applyPositionChange (contact.go:286-386) as apply_position_change_lds,
+// from the LDS tables, storing nothing: the bodies' new position and
+// orientation come back in np/nq.  Bit i of the result is set when body i
+// sleeps (its derived data are rewritten from np/nq, whose orientation has
+// then been normalised a second time, as CalculateDerivedData does).
+template <typename R>
+__device__ __forceinline__ int position_change_mw(const int kl[2], const bool aw[2], const V3<R>& n,
+                                                  const V3<R> rel[2], R penetration, V3<R> lin[2], V3<R> ang[2],
+                                                  V3<R> np[2], Q4<R> nq[2], const R* s_st, const R* s_iit,
+                                                  const R* s_im) {
+  constexpr int KB = kRegBodies;
+  const R angularLimit = R(0.2);
+  R angI[2] = {R(0), R(0)}, linI[2] = {R(0), R(0)};
+  R total = R(0);
+  int sleeping = 0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    lin[i] = V3<R>{R(0), R(0), R(0)};
+    ang[i] = V3<R>{R(0), R(0), R(0)};
+    np[i] = V3<R>{R(0), R(0), R(0)};
+    nq[i] = Q4<R>{R(1), R(0), R(0), R(0)};
+    if (kl[i] < 0) continue;
+    R iw[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) iw[k] = s_iit[k * KB + kl[i]];
+    V3<R> aiw = cross(rel[i], n);
+    aiw = m3v(iw, aiw);
+    aiw = cross(aiw, rel[i]);
+    angI[i] = dot(aiw, n);
+    linI[i] = s_im[kl[i]];
+    total += linI[i] + angI[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int l = kl[i];
+    if (l < 0) continue;
+    R sign = R(1);
+    if (i != 0) sign = R(-1);
+    R angM = sign * penetration * (angI[i] / total);
+    R linM = sign * penetration * (linI[i] / total);
+    V3<R> projection = add_scaled(rel[i], n, -dot(rel[i], n));
+    R maxMagnitude = angularLimit * mag(projection);
+    if (angM < -maxMagnitude) {
+      R totalMove = angM + linM;
+      angM = -maxMagnitude;
+      linM = totalMove - angM;
+    } else if (angM > maxMagnitude) {
+      R totalMove = angM + linM;
+      angM = maxMagnitude;
+      linM = totalMove - angM;
+    }
+    if (angM == R(0)) {
+      ang[i] = V3<R>{R(0), R(0), R(0)};
+    } else {
+      V3<R> tad = cross(rel[i], n);
+      R iw[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) iw[k] = s_iit[k * KB + l];
+      ang[i] = mul(m3v(iw, tad), angM / angI[i]);
+    }
+    lin[i] = mul(n, linM);
+    np[i] = add_scaled(lds3(s_st, l), n, linM);
+    Q4<R> q{s_st[3 * KB + l], s_st[4 * KB + l], s_st[5 * KB + l], s_st[6 * KB + l]};
+    q = q_add_scaled_vector(q, ang[i], R(1));
+    q = q_normalize(q);
+    if (!aw[i]) {
+      q = q_normalize(q);
+      sleeping |= 1 << i;
+    }
+    nq[i] = q;
+  }
+  return sleeping;
+}
+
+template <typename R, int KIND, int SL>
+__device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w, int n, char* lds, int t) {
+  using L = MwLds<R>;
+  constexpr int KB = kRegBodies, T = kMwThreads, KC = kMwMaxContacts;
+  constexpr bool MAT = KIND == KIND_MAT;
+  R* s_pos0 = (R*)(lds + L::pos0);
+  R* s_lad = (R*)(lds + L::lad);
+  R* s_st = (R*)(lds + L::st);
+  R* s_rot = s_st + 3 * KB;
+  R* s_iit = (R*)(lds + L::iit);
+  R* s_im = (R*)(lds + L::im);
+  R* s_cn = (R*)(lds + L::cn);
+  R* s_cp = (R*)(lds + L::cp);
+  double* s_pbs = (double*)(lds + L::pbs);
+  double* s_sum = (double*)(lds + L::sum);
+  R* s_pay = (R*)(lds + L::pay);
+  R* s_res = (R*)(lds + L::res);
+  R* s_pkey = (R*)(lds + L::pkey);
+  int32_t* s_pidx = (int32_t*)(lds + L::pidx);
+  uint32_t* s_pkid = (uint32_t*)(lds + L::pkid);
+  int32_t* s_resi = (int32_t*)(lds + L::resi);
+  uint8_t* s_flags = (uint8_t*)(lds + L::flags);
+  auto cnrm = [&](int j) { return V3<R>{s_cn[j], s_cn[KC + j], s_cn[2 * KC + j]}; };
+  auto cpt = [&](int j) { return V3<R>{s_cp[j], s_cp[KC + j], s_cp[2 * KC + j]}; };
+  // wave 0's result: four vectors (the two bodies' linear and angular
+  // changes)
+  auto res_store = [&](int k, const V3<R>& v) {
+    s_res[3 * k] = v.x;
+    s_res[3 * k + 1] = v.y;
+    s_res[3 * k + 2] = v.z;
+  };
+  auto res_load = [&](int k) { return V3<R>{s_res[3 * k], s_res[3 * k + 1], s_res[3 * k + 2]}; };
+  const int lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const bool do_pos = (a.mode & RM_POS) != 0;
+  const bool do_vel = (a.mode & RM_VEL) != 0 && (KIND == KIND_POLISH || !a.world_mask || a.world_mask[w]);
+  if (a.world_iters && t == 0) {
+    if (do_pos) a.world_iters[2 * w] = 0;
+    if (a.mode & RM_VEL) a.world_iters[2 * w + 1] = 0;
+  }
+  // ResolveContacts' guard, contact.go:210-212
+  if (a.overflow[0] != 0 || C <= 0 || a.dt <= R(0)) return;
+  if (!do_pos && !do_vel) return;
+
+  const Bodies<R>& b = a.b;
+  const Contacts<R>& c = a.c;
+  const int64_t S = b.stride, CS = c.cap;
+  const int64_t base = a.world_coff[w];
+  const int64_t maxIter = a.iter_fixed > 0 ? a.iter_fixed : a.iter_mult * int64_t(C);
+  const V3<R> zero{R(0), R(0), R(0)};
+
+  for (int i = t; i < n; i += T) {
+    const int g = b0w + i;
+    const V3<R> p = ld3(do_pos ? b.pos : a.pos0, S, g);
+    if (do_pos) st3(a.pos0, S, g, p);
+    lds3_store(s_pos0, i, p);
+    lds3_store(s_lad, i, mul(ld3(b.last_acc, S, g), a.dt));
+    s_flags[i] = b.flags[g];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s_iit[k * KB + i] = b.iit_world[k * S + g];
+    s_im[i] = b.inv_mass[g];
+  }
+  __syncthreads();
+
+  // prepareContacts / calculateInternals, contact.go:225-229, :59-85, as in
+  // resolve_reg
+  if (do_pos) {
+    for (int j = t; j < C; j += T) {
+      const int64_t g = base + j;
+      int c0 = c.body0[g], c1 = c.body1[g];
+      V3<R> nn = ld3(c.normal, CS, g);
+      if (c0 < 0) {  // contact.go:61-65
+        nn = mul(nn, R(-1));
+        c0 = c1;
+        c1 = -1;
+        c.body0[g] = c0;
+        c.body1[g] = c1;
+        st3(c.normal, CS, g, nn);
+      }
+      const int l0 = c0 - b0w, l1 = c1 >= 0 ? c1 - b0w : -1;
+      R m[9];
+      basis_from_scale(nn, basis_scale(nn), m);
+      const V3<R> p = ld3(c.point, CS, g);
+      const V3<R> r0 = sub(p, lds3(s_pos0, l0));
+      V3<R> cv = local_velocity_lds(b, c0, lds3(s_lad, l0), r0, m);
+      V3<R> r1 = zero;
+      if (c1 >= 0) {
+        r1 = sub(p, lds3(s_pos0, l1));
+        cv = sub(cv, local_velocity_lds(b, c1, lds3(s_lad, l1), r1, m));
+      }
+      const R ddv = desired_dv_lds(s_flags, s_lad, l0, l1, nn, cv.x, restitution_of<R, KIND>(c, g));
+#pragma unroll
+      for (int k = 0; k < 9; ++k) c.c2w[k * CS + g] = m[k];
+      st3(c.rel, CS, g, r0);
+      st3(c.rel + 3 * CS, CS, g, r1);
+      st3(c.cvel, CS, g, cv);
+      c.ddv[g] = ddv;
+    }
+  }
+  // this thread's contacts into registers and LDS (it wrote what it reads
+  // here)
+  R key[SL];
+  uint32_t ids[SL];
+  double bsc[SL];
+  R fr[MAT ? SL : 1], rs[MAT ? SL : 1];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const int j = T * s + t;
+    key[s] = R(0);
+    ids[s] = 0xFFFFFFFFu;
+    bsc[s] = 0.0;
+    if constexpr (MAT) fr[s] = rs[s] = R(0);
+    if (j < C) {
+      const int64_t g = base + j;
+      const int c0 = c.body0[g], c1 = c.body1[g];
+      ids[s] = uint32_t(c0 - b0w) | (uint32_t(c1 >= 0 ? c1 - b0w : int(kNoBody)) << 16);
+      if (do_pos) key[s] = c.pen[g];
+      const V3<R> nn = ld3(c.normal, CS, g), p = ld3(c.point, CS, g);
+      s_cn[j] = nn.x; s_cn[KC + j] = nn.y; s_cn[2 * KC + j] = nn.z;
+      s_cp[j] = p.x; s_cp[KC + j] = p.y; s_cp[2 * KC + j] = p.z;
+      bsc[s] = basis_scale(nn);
+      if constexpr (MAT) {
+        fr[s] = c.fric[g];
+        rs[s] = c.rest[g];
+      }
+    }
+  }
+  if (do_pos) {
+    for (int i = t; i < n; i += T) {
+      const int g = b0w + i;
+      lds3_store(s_st, i, ld3(b.pos, S, g));
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s_st[(3 + k) * KB + i] = b.quat[k * S + g];
+    }
+  }
+  __syncthreads();
+
+  // adjustPositions, contact.go:233-283.  A slot past the world's contacts
+  // keeps key 0 and ids 0xFFFFFFFF, so the loops need no bounds test.
+  int64_t used = 0;
+  while (do_pos && used < maxIter) {
+    opaque_ids(ids);
+    R best = R(kPositionEpsilon);
+    int bi = INT_MAX;
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      if (key[s] > best) {
+        best = key[s];
+        bi = T * s + t;
+      }
+    }
+    wave_argmax_dpp(best, bi);
+    if (bi == INT_MAX) {
+      if (lane == 0) s_pidx[wv] = INT_MAX;
+    } else if ((bi & (T - 1)) == t) {
+      const int ss = unsigned(bi) / T;
+      s_pkey[wv] = best;
+      s_pidx[wv] = bi;
+#pragma unroll
+      for (int s = 0; s < SL; ++s)
+        if (s == ss) s_pkid[wv] = ids[s];
+    }
+    __syncthreads();  // A
+    if (wv == 0) {
+      R gk = R(kPositionEpsilon);
+      int gi = INT_MAX;
+      const int gw = mw_best(s_pkey, s_pidx, gk, gi);
+      if (gw < 0) {
+        if (lane == 0) s_resi[0] = MW_DONE;
+      } else {
+        const uint32_t kid = s_pkid[gw];
+        const int kl[2] = {id_lo(kid), id_hi(kid)};
+        const V3<R> nn = cnrm(gi), p = cpt(gi);
+        V3<R> rel[2];
+        rel[0] = sub(p, lds3(s_pos0, kl[0]));
+        rel[1] = kl[1] >= 0 ? sub(p, lds3(s_pos0, kl[1])) : zero;
+        // matchAwakeState, contact.go:185-202
+        bool aw[2] = {(s_flags[kl[0]] & F_AWAKE) != 0, kl[1] >= 0 && (s_flags[kl[1]] & F_AWAKE) != 0};
+        int woke = -1;
+        if (kl[1] >= 0 && (aw[0] || aw[1]) && !(aw[0] && aw[1])) {
+          woke = aw[0] ? kl[1] : kl[0];
+          aw[0] = aw[1] = true;
+        }
+        V3<R> lin[2], ang[2], np[2];
+        Q4<R> nq[2];
+        const int sleeping = position_change_mw(kl, aw, nn, rel, gk, lin, ang, np, nq, s_st, s_iit, s_im);
+        if (lane == 0) {
+          if (woke >= 0) {  // SetAwake(true), rigidbody.go:183-186
+            s_flags[woke] = uint8_t(s_flags[woke] | F_AWAKE);
+            b.motion[b0w + woke] = R(kSleepEpsilon * 2.0);
+          }
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            const int l = kl[i];
+            if (l < 0) continue;
+            lds3_store(s_st, l, np[i]);
+            s_st[3 * KB + l] = nq[i].w;
+            s_st[4 * KB + l] = nq[i].x;
+            s_st[5 * KB + l] = nq[i].y;
+            s_st[6 * KB + l] = nq[i].z;
+            if (sleeping & (1 << i)) {  // CalculateDerivedData, rigidbody.go:268-272
+              const int g = b0w + l;
+              R Tm[12], IB[9], IW[9];
+              set_as_transform(np[i], nq[i], Tm);
+              ld9(b.iit_body, S, g, IB);
+              transform_inertia(IB, Tm, IW);
+#pragma unroll
+              for (int k = 0; k < 12; ++k) b.transform[k * S + g] = Tm[k];
+#pragma unroll
+              for (int k = 0; k < 9; ++k) {
+                b.iit_world[k * S + g] = IW[k];
+                s_iit[k * KB + l] = IW[k];
+              }
+            }
+          }
+          res_store(0, lin[0]);
+          res_store(1, lin[1]);
+          res_store(2, ang[0]);
+          res_store(3, ang[1]);
+          s_resi[1] = int32_t(kid);
+          s_resi[0] = MW_APPLY;
+        }
+      }
+    }
+    __syncthreads();  // C
+    if (s_resi[0] != MW_APPLY) break;
+    {
+      const uint32_t kid = uint32_t(s_resi[1]);
+      const int kl[2] = {id_lo(kid), id_hi(kid)};
+      const V3<R> lin[2] = {res_load(0), res_load(1)};
+      const V3<R> ang[2] = {res_load(2), res_load(3)};
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+        bool any = false;
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+          for (int d = 0; d < 2; ++d) any |= (jb[bb] >= 0 && jb[bb] == kl[d]);
+        if (!any) continue;
+        R pen = key[s];
+        const V3<R> ns = cnrm(T * s + t), ps = cpt(T * s + t);
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+          if (jb[bb] < 0) continue;
+          const V3<R> rj = sub(ps, lds3(s_pos0, jb[bb]));
+#pragma unroll
+          for (int d = 0; d < 2; ++d) {
+            if (jb[bb] == kl[d]) {
+              V3<R> dp = add(cross(ang[d], rj), lin[d]);
+              R sign = R(1);
+              if (bb == 0) sign = R(-1);
+              pen += dot(dp, ns) * sign;
+            }
+          }
+        }
+        key[s] = pen;
+      }
+    }
+    ++used;
+  }
+  const int64_t pos_iters = used;
+  __syncthreads();
+  if (do_pos) {
+    for (int i = t; i < n; i += T) {
+      const int g = b0w + i;
+      st3(b.pos, S, g, lds3(s_st, i));
+#pragma unroll
+      for (int k = 0; k < 4; ++k) b.quat[k * S + g] = s_st[(3 + k) * KB + i];
+    }
+  }
+
+  // adjustVelocities, contact.go:390-445
+  V3<R> cv[SL];
+  R imp[SL][kImpPlanes];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const int j = T * s + t;
+    cv[s] = zero;
+#pragma unroll
+    for (int k = 0; k < kImpPlanes; ++k) imp[s][k] = R(0);
+    if (do_pos && j < C) c.pen[base + j] = key[s];
+    if (do_vel && j < C) {
+      key[s] = c.ddv[base + j];  // selection keys as prepare stored them
+      cv[s] = ld3(c.cvel, CS, base + j);
+      // the impulse data, fixed for the whole phase (as impulse_data from
+      // the contact arrays prepare stored: same values)
+      const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+      R mj[9], i0[9], i1[9];
+      const V3<R> ps = cpt(j);
+      basis_from_scale(cnrm(j), bsc[s], mj);
+      const V3<R> rj[2] = {sub(ps, lds3(s_pos0, jb[0])), jb[1] >= 0 ? sub(ps, lds3(s_pos0, jb[1])) : zero};
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        i0[k] = s_iit[k * KB + jb[0]];
+        i1[k] = jb[1] >= 0 ? s_iit[k * KB + jb[1]] : R(0);
+      }
+      const R invm[2] = {s_im[jb[0]], jb[1] >= 0 ? s_im[jb[1]] : R(0)};
+      R fj = R(kFriction);
+      if constexpr (MAT) fj = fr[s];
+      impulse_data_from<R, KIND>(a.mode, i0, i1, invm, jb[1] >= 0, rj, mj, fj, imp[s]);
+    }
+  }
+  // the position write-back read the LDS state that the velocity phase
+  // reuses
+  __syncthreads();
+  used = 0;
+  if (do_vel) {
+    for (int i = t; i < n; i += T) {
+      const int g = b0w + i;
+      lds3_store(s_st, i, ld3(b.vel, S, g));
+      lds3_store(s_rot, i, ld3(b.rot, S, g));
+    }
+    bool polishing = KIND == KIND_POLISH && a.world_mask[w] == 0;
+    double ke_before = 0.0, world_mass = 0.0;
+    if constexpr (KIND == KIND_POLISH) {
+      if (polishing) {
+        // take the predicted changes (as resolve_reg)
+        const float* dl = a.delta;
+        double ke = 0.0, mass = 0.0;
+        for (int i = t; i < n; i += T) {
+          const int g = b0w + i;
+          const V3<R> v = lds3(s_st, i), r = lds3(s_rot, i);
+          R iw[9];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) iw[k] = s_iit[k * KB + i];
+          double mi;
+          ke += kinetic_energy(v, r, s_im[i], iw, mi);
+          mass += mi;
+          a.vel0[6 * S + g] = b.motion[g];
+          const V3<R> dv{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
+          const V3<R> dw{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
+          if (dv.x != R(0) || dv.y != R(0) || dv.z != R(0) || dw.x != R(0) || dw.y != R(0) || dw.z != R(0)) {
+            lds3_store(s_st, i, add(v, dv));
+            lds3_store(s_rot, i, add(r, dw));
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          if (T * s + t >= C) continue;
+          const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+          R mj[9];
+          basis_from_scale(cnrm(T * s + t), bsc[s], mj);
+          const V3<R> ps = cpt(T * s + t);
+          V3<R> cvj = cv[s];
+          bool moved = false;
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            if (jb[bb] < 0) continue;
+            const int g = b0w + jb[bb];
+            const V3<R> dv{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
+            const V3<R> dw{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
+            if (dv.x == R(0) && dv.y == R(0) && dv.z == R(0) && dw.x == R(0) && dw.y == R(0) && dw.z == R(0)) continue;
+            moved = true;
+            const V3<R> rj = sub(ps, lds3(s_pos0, jb[bb]));
+            R sign = R(1);
+            if (bb == 1) sign = R(-1);
+            cvj = add(cvj, mul(m3tv(mj, add(cross(dw, rj), dv)), sign));
+          }
+          if (moved) {
+            cv[s] = cvj;
+            key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], V3<R>{mj[0], mj[1], mj[2]}, cvj.x,
+                                    R(kRestitution));
+          }
+        }
+        ke_before = mw_sum(ke, s_sum, t);
+        world_mass = mw_sum(mass, s_sum, t);
+      }
+    }
+    __syncthreads();
+    R eps = polishing ? R(kVelocityEpsilon) + a.polish_tol : R(kVelocityEpsilon);
+    int64_t budget = polishing ? a.polish_mult * int64_t(C) : maxIter;
+    int64_t it = 0;
+    while (true) {
+      opaque_ids(ids);
+      R best = eps;
+      int bi = INT_MAX;
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        if (key[s] > best) {
+          best = key[s];
+          bi = T * s + t;
+        }
+      }
+      wave_argmax_dpp(best, bi);
+      if (bi == INT_MAX) {
+        if (lane == 0) s_pidx[wv] = INT_MAX;
+      } else if ((bi & (T - 1)) == t) {
+        const int ss = unsigned(bi) / T;
+        R* pay = s_pay + wv * kMwPay;
+        s_pkey[wv] = best;
+        s_pidx[wv] = bi;
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          if (s != ss) continue;
+          s_pkid[wv] = ids[s];
+          s_pbs[wv] = bsc[s];
+          pay[0] = cv[s].x; pay[1] = cv[s].y; pay[2] = cv[s].z;
+#pragma unroll
+          for (int k = 0; k < kImpPlanes; ++k) pay[3 + k] = imp[s][k];
+          if constexpr (MAT) pay[15] = fr[s];
+        }
+      }
+      __syncthreads();  // A
+      if (wv == 0) {
+        R gk = eps;
+        int gi = INT_MAX;
+        const int gw = mw_best(s_pkey, s_pidx, gk, gi);
+        if (gw < 0 || it >= budget) {
+          if (lane == 0) s_resi[0] = gw < 0 ? MW_DONE : MW_BUDGET;
+        } else {
+          const uint32_t kid = s_pkid[gw];
+          const R* pay = s_pay + gw * kMwPay;
+          const int kl[2] = {id_lo(kid), id_hi(kid)};
+          const V3<R> nn = cnrm(gi), p = cpt(gi);
+          const V3<R> cvk{pay[0], pay[1], pay[2]};
+          R m[9], ik[kImpPlanes];
+          basis_from_scale(nn, s_pbs[gw], m);
+#pragma unroll
+          for (int k = 0; k < kImpPlanes; ++k) ik[k] = pay[3 + k];
+          R fk = R(kFriction);
+          if constexpr (MAT) fk = pay[15];
+          V3<R> rel[2];
+          rel[0] = sub(p, lds3(s_pos0, kl[0]));
+          rel[1] = kl[1] >= 0 ? sub(p, lds3(s_pos0, kl[1])) : zero;
+          // matchAwakeState, contact.go:185-202
+          const bool aw0 = (s_flags[kl[0]] & F_AWAKE) != 0;
+          const bool aw1 = kl[1] >= 0 && (s_flags[kl[1]] & F_AWAKE) != 0;
+          int woke = -1;
+          if (kl[1] >= 0 && (aw0 || aw1) && !(aw0 && aw1)) woke = aw0 ? kl[1] : kl[0];
+          // applyVelocityChange, contact.go:448-494 (apply_velocity_change_lds)
+          V3<R> ic;
+          if (MAT && fk == R(0))
+            ic = V3<R>{gk / ik[0], R(0), R(0)};
+          else
+            ic = friction_impulse_from(ik, ik + 9, gk, cvk, fk);
+          const V3<R> impulse = m3v(m, ic);
+          V3<R> vc[2], rc[2];
+          R iw[9];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) iw[k] = s_iit[k * KB + kl[0]];
+          V3<R> torque = cross(rel[0], impulse);
+          rc[0] = m3v(iw, torque);
+          vc[0] = add_scaled(zero, impulse, s_im[kl[0]]);
+          rc[1] = zero;
+          vc[1] = zero;
+          if (kl[1] >= 0) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) iw[k] = s_iit[k * KB + kl[1]];
+            torque = cross(impulse, rel[1]);
+            rc[1] = m3v(iw, torque);
+            vc[1] = add_scaled(zero, impulse, -s_im[kl[1]]);
+          }
+          if (lane == 0) {
+            if (woke >= 0) {  // SetAwake(true), rigidbody.go:183-186
+              s_flags[woke] = uint8_t(s_flags[woke] | F_AWAKE);
+              b.motion[b0w + woke] = R(kSleepEpsilon * 2.0);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+              if (kl[i] < 0) continue;
+              lds3_store(s_st, kl[i], add(lds3(s_st, kl[i]), vc[i]));
+              lds3_store(s_rot, kl[i], add(lds3(s_rot, kl[i]), rc[i]));
+            }
+            if constexpr (KIND == KIND_LABEL) add_label(a, base + gi, vc, rc);
+            res_store(0, vc[0]);
+            res_store(1, vc[1]);
+            res_store(2, rc[0]);
+            res_store(3, rc[1]);
+            s_resi[1] = int32_t(kid);
+            s_resi[0] = MW_APPLY;
+          }
+        }
+      }
+      __syncthreads();  // C
+      const int state = s_resi[0];
+      if (state != MW_APPLY) {
+        bool reject = false;  // the polish did not give an acceptable result
+        if (state == MW_DONE) {
+          if constexpr (KIND == KIND_POLISH) {
+            // the acceptance test of resolve_reg
+            if (polishing) {
+              double ke = 0.0, mass = 0.0;
+              for (int i = t; i < n; i += T) {
+                R iw[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) iw[k] = s_iit[k * KB + i];
+                ke += kinetic_energy(lds3(s_st, i), lds3(s_rot, i), s_im[i], iw, mass);
+              }
+              const double ke_after = mw_sum(ke, s_sum, t);
+              const double tol = double(a.polish_tol);
+              reject = !(ke_after <= ke_before * (1.0 + kEnergyRel) + 0.5 * tol * tol * world_mass);
+            }
+          }
+          if (!reject) break;
+        } else {  // the iteration budget is used
+          if (KIND != KIND_POLISH || !polishing) break;
+          reject = true;
+        }
+        // undo the prediction and the loop; run the plain phase
+        __syncthreads();
+        for (int i = t; i < n; i += T) {
+          const int g = b0w + i;
+          lds3_store(s_st, i, ld3(b.vel, S, g));
+          lds3_store(s_rot, i, ld3(b.rot, S, g));
+          b.motion[g] = a.vel0[6 * S + g];
+          s_flags[i] = b.flags[g];
+        }
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int j = T * s + t;
+          if (j < C) {
+            key[s] = c.ddv[base + j];
+            cv[s] = ld3(c.cvel, CS, base + j);
+          }
+        }
+        __syncthreads();
+        if (t == 0) a.world_mask[w] = 1;
+        polishing = false;
+        eps = R(kVelocityEpsilon);
+        budget = maxIter;
+        it = 0;
+        continue;
+      }
+      {
+        const uint32_t kid = uint32_t(s_resi[1]);
+        const int kl[2] = {id_lo(kid), id_hi(kid)};
+        const V3<R> vc[2] = {res_load(0), res_load(1)};
+        const V3<R> rc[2] = {res_load(2), res_load(3)};
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+          bool any = false;
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int d = 0; d < 2; ++d) any |= (jb[bb] >= 0 && jb[bb] == kl[d]);
+          if (!any) continue;
+          R mj[9];
+          basis_from_scale(cnrm(T * s + t), bsc[s], mj);
+          const V3<R> ps = cpt(T * s + t);
+          V3<R> cvj = cv[s];
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            if (jb[bb] < 0) continue;
+            const V3<R> rj = sub(ps, lds3(s_pos0, jb[bb]));
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+              if (jb[bb] == kl[d]) {
+                V3<R> dv = add(cross(rc[d], rj), vc[d]);
+                R sign = R(1);
+                if (bb == 1) sign = R(-1);
+                cvj = add(cvj, mul(m3tv(mj, dv), sign));
+              }
+            }
+          }
+          cv[s] = cvj;
+          R e = R(kRestitution);
+          if constexpr (MAT) e = rs[s];
+          key[s] = desired_dv_lds(s_flags, s_lad, jb[0], jb[1], V3<R>{mj[0], mj[1], mj[2]}, cvj.x, e);
+        }
+      }
+      ++it;
+      ++used;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int j = T * s + t;
+      if (j < C) {
+        c.ddv[base + j] = key[s];
+        st3(c.cvel, CS, base + j, cv[s]);
+      }
+    }
+    for (int i = t; i < n; i += T) {
+      const int g = b0w + i;
+      st3(b.vel, S, g, lds3(s_st, i));
+      st3(b.rot, S, g, lds3(s_rot, i));
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < n; i += T) b.flags[b0w + i] = s_flags[i];
+  if (t == 0 && a.world_iters) {
+    if (do_pos) a.world_iters[2 * w] = pos_iters;
+    if (do_vel) a.world_iters[2 * w + 1] = used;
+  }
+}
+
+template <typename R, int KIND>
+__device__ __forceinline__ void resolve_mw_world(const RS<R>& a, int w, int C, int b0w, int n, char* lds, int t) {
+  if (C <= kMwThreads * 4)
+    resolve_mw<R, KIND, 4>(a, w, C, b0w, n, lds, t);
+  else
+    resolve_mw<R, KIND, kMwMaxSlots>(a, w, C, b0w, n, lds, t);
+}
+
 constexpr size_t cmax(size_t x, size_t y) { return x > y ? x : y; }
 template <typename R, int KIND>
 constexpr size_t resolve_lds_bytes() {
@@ -1509,10 +2301,36 @@ __device__ __forceinline__ void resolve_block(const RS<R>& a) {
   if (w >= a.n_worlds) return;
   const int C = a.world_cnt[w];
   const int b0w = a.body_off[w], n = a.body_off[w + 1] - b0w;
+  if (a.mw_list && a.mw_flag[w]) return;  // the multi-wave kernel's
   int64_t* wt = a.world_time ? a.world_time + 4 * int64_t(w) + ((a.mode & RM_VEL) ? 2 : 0) : nullptr;
   if (wt && lane == 0) wt[0] = int64_t(wall_clock64());
   resolve_world_kind<R, KIND>(a, w, C, b0w, n, lds, lane);
   if (wt && lane == 0) wt[1] = int64_t(wall_clock64());
+}
+
+// The multi-wave kernel, persistent: each workgroup takes the next listed
+// world (largest first) from the queue until the list is done.
+template <typename R, int KIND>
+__device__ __forceinline__ void resolve_mw_block(const RS<R>& a) {
+  __shared__ __align__(16) char lds[MwLds<R>::bytes];
+  __shared__ int32_t s_next;
+  const int t = threadIdx.x;
+  const int nbig = a.mw_count[0];
+  while (true) {
+    if (t == 0) s_next = atomicAdd(&a.mw_count[1], 1);
+    __syncthreads();
+    const int k = s_next;
+    __syncthreads();
+    if (k >= nbig) break;
+    const int w = a.mw_list[k];
+    const int C = a.world_cnt[w];
+    const int b0w = a.body_off[w], n = a.body_off[w + 1] - b0w;
+    int64_t* wt = a.world_time ? a.world_time + 4 * int64_t(w) + ((a.mode & RM_VEL) ? 2 : 0) : nullptr;
+    if (wt && t == 0) wt[0] = int64_t(wall_clock64());
+    resolve_mw_world<R, KIND>(a, w, C, b0w, n, lds, t);
+    __syncthreads();
+    if (wt && t == 0) wt[1] = int64_t(wall_clock64());
+  }
 }
 
 // The FP64 kernel keeps a full wave's registers (its working set spills
@@ -1528,12 +2346,31 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   resolve_block<float, KIND>(a);
 }
 
+template <int KIND>
+__global__ void __launch_bounds__(kMwThreads) k_resolve_mw_f64(RS<double> a) {
+  resolve_mw_block<double, KIND>(a);
+}
+template <int KIND>
+__global__ void __launch_bounds__(kMwThreads) k_resolve_mw_f32(RS<float> a) {
+  resolve_mw_block<float, KIND>(a);
+}
+
 template <typename R, int KIND>
 void launch_kind(const RS<R>& a, hipStream_t st) {
   if constexpr (sizeof(R) == 8)
     k_resolve_f64<KIND><<<a.n_worlds, 64, 0, st>>>(a);
   else
     k_resolve_f32<KIND><<<a.n_worlds, 64, 0, st>>>(a);
+}
+
+// The multi-wave kernel over mw_list: at most one workgroup per CU (each
+// takes a whole CU's registers).
+template <typename R, int KIND>
+void launch_mw_kind(const RS<R>& a, int grid, hipStream_t st) {
+  if constexpr (sizeof(R) == 8)
+    k_resolve_mw_f64<KIND><<<grid, kMwThreads, 0, st>>>(a);
+  else
+    k_resolve_mw_f32<KIND><<<grid, kMwThreads, 0, st>>>(a);
 }
 
 // Worlds (workgroups) of one kind's kernel resident per CU, from its
@@ -1550,6 +2387,7 @@ hipError_t kind_occupancy(int* worlds_per_cu) {
 // (k_resolve_<precision>_<kind>.hip); other units only declare them.
 #define CBZ_RESOLVE_KIND(EXT, R, KIND)                                          \
   EXT template void launch_kind<R, KIND>(const RS<R>&, hipStream_t);           \
+  EXT template void launch_mw_kind<R, KIND>(const RS<R>&, int, hipStream_t);   \
   EXT template hipError_t kind_occupancy<R, KIND>(int*);
 #ifndef CBZ_RESOLVE_INSTANTIATE
 CBZ_RESOLVE_KIND(extern, double, KIND_PLAIN)

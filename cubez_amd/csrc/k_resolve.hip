@@ -5,6 +5,106 @@
 
 namespace cbz {
 
+// The worlds the multi-wave kernel takes: of the worlds it can take
+// (mw_fits), the mw_max largest by contact count (the whole last count
+// bucket of 32 contacts that reaches mw_max, then capped at mw_max in
+// `order`, larger worlds first).  Writes mw_list[0 .. mw_count[0]), mw_flag
+// for every world, and resets the queue head mw_count[1].  One workgroup;
+// thread t looks at kSelectPer consecutive entries of each chunk.
+constexpr int kSelectThreads = 1024, kSelectPer = 16, kSelectBins = kMwMaxContacts / 32 + 1;
+__global__ void __launch_bounds__(kSelectThreads)
+    k_mw_select(const int32_t* order, const int32_t* world_cnt, const int32_t* body_off, int32_t n_worlds,
+                int32_t mw_min, int32_t mw_max, int32_t* mw_list, int32_t* mw_count, uint8_t* mw_flag) {
+  __shared__ int32_t s_hist[kSelectBins];
+  __shared__ int32_t s_wave[kSelectThreads / 64];
+  __shared__ int32_t s_carry, s_thr;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int i = t; i < kSelectBins; i += kSelectThreads) s_hist[i] = 0;
+  if (t == 0) s_carry = 0;
+  __syncthreads();
+  // contact-count histogram of the worlds the kernel can take
+  for (int32_t w = t; w < n_worlds; w += kSelectThreads) {
+    const int C = world_cnt[w];
+    if (mw_fits(C, body_off[w + 1] - body_off[w], mw_min)) atomicAdd(&s_hist[C >> 5], 1);
+  }
+  __syncthreads();
+  if (t == 0) {
+    int thr = kSelectBins, cum = 0;
+    while (thr > 0 && cum < mw_max) cum += s_hist[--thr];
+    s_thr = thr;  // buckets >= thr (none when mw_max <= 0)
+  }
+  __syncthreads();
+  const int thr = s_thr;
+  for (int32_t b0 = 0; b0 < n_worlds; b0 += kSelectThreads * kSelectPer) {
+    int32_t w[kSelectPer];
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kSelectPer; ++k) {
+      const int32_t i = b0 + t * kSelectPer + k;
+      w[k] = i < n_worlds ? (order ? order[i] : i) : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kSelectPer; ++k) {
+      if (w[k] < 0) continue;
+      const int C = world_cnt[w[k]];
+      if (!mw_fits(C, body_off[w[k] + 1] - body_off[w[k]], mw_min) || (C >> 5) < thr) {
+        mw_flag[w[k]] = 0;
+        w[k] = -1;
+      }
+      cnt += w[k] >= 0;
+    }
+    const int inc = wave_incl_scan_i32(cnt);
+    if (lane == 63) s_wave[wv] = inc;
+    __syncthreads();
+    int pos = s_carry + inc - cnt;
+    for (int v = 0; v < wv; ++v) pos += s_wave[v];
+#pragma unroll
+    for (int k = 0; k < kSelectPer; ++k) {
+      if (w[k] < 0) continue;
+      const bool take = pos < mw_max;
+      if (take) mw_list[pos] = w[k];
+      mw_flag[w[k]] = take ? 1 : 0;
+      ++pos;
+    }
+    __syncthreads();
+    if (t == kSelectThreads - 1) s_carry = pos;
+    __syncthreads();
+  }
+  if (t == 0) {
+    mw_count[0] = s_carry < mw_max ? s_carry : mw_max;
+    mw_count[1] = 0;
+  }
+}
+
+// With the multi-wave kernel: the launch's stream forks to `side`, which
+// selects the worlds and runs the multi-wave kernel; the one-wave kernel
+// starts on the launch's stream once the selection is done (so that the
+// multi-wave workgroups, which each need a whole CU, are placed first), and
+// the launch's stream waits for `side` at the end.
+template <typename R, int KIND>
+hipError_t launch_both(const RS<R>& a, const ResolveLaunch<R>& L, hipStream_t st) {
+  hipError_t e;
+  const bool mw = a.mw_list != nullptr;
+  if (mw) {
+    if ((e = hipEventRecord(L.fork, st)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(L.side, L.fork, 0)) != hipSuccess) return e;
+    k_mw_select<<<1, kSelectThreads, 0, L.side>>>(a.order, a.world_cnt, a.body_off, a.n_worlds, L.mw_min,
+                                                   L.mw_max, L.mw_list, L.mw_count, L.mw_flag);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipEventRecord(L.sel, L.side)) != hipSuccess) return e;
+    int grid = L.mw_max < L.n_cu ? L.mw_max : L.n_cu;
+    if (grid > a.n_worlds) grid = a.n_worlds;
+    launch_mw_kind<R, KIND>(a, grid, L.side);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipEventRecord(L.join, L.side)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(st, L.sel, 0)) != hipSuccess) return e;
+  }
+  launch_kind<R, KIND>(a, st);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (mw) return hipStreamWaitEvent(st, L.join, 0);
+  return hipSuccess;
+}
+
 template <typename R>
 hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   RS<R> a;
@@ -17,6 +117,11 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   a.body_off = L.body_off;
   a.world_iters = L.world_iters;
   a.world_time = L.world_time;
+  const bool mw = L.mw_min > 0 && L.mw_max > 0 && L.mw_list && L.mw_count && L.mw_flag && L.side && L.fork && L.sel &&
+                  L.join;
+  a.mw_list = mw ? L.mw_list : nullptr;
+  a.mw_count = mw ? L.mw_count : nullptr;
+  a.mw_flag = mw ? L.mw_flag : nullptr;
   a.n_worlds = L.n_worlds;
   a.dt = L.dt;
   a.iter_mult = L.iter_mult;
@@ -35,15 +140,10 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   // per-contact materials: the plain resolver only
   if (a.c.fric && (!a.c.rest || (a.mode & (RM_POLISH | RM_LABEL)))) return hipErrorInvalidValue;
   if (a.n_worlds <= 0) return hipSuccess;
-  if (a.c.fric)
-    launch_kind<R, KIND_MAT>(a, st);
-  else if (a.mode & RM_POLISH)
-    launch_kind<R, KIND_POLISH>(a, st);
-  else if (a.mode & RM_LABEL)
-    launch_kind<R, KIND_LABEL>(a, st);
-  else
-    launch_kind<R, KIND_PLAIN>(a, st);
-  return hipGetLastError();
+  if (a.c.fric) return launch_both<R, KIND_MAT>(a, L, st);
+  if (a.mode & RM_POLISH) return launch_both<R, KIND_POLISH>(a, L, st);
+  if (a.mode & RM_LABEL) return launch_both<R, KIND_LABEL>(a, L, st);
+  return launch_both<R, KIND_PLAIN>(a, L, st);
 }
 
 // Worlds (workgroups) of the plain resolver kernel resident per CU, from

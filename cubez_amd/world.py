@@ -191,6 +191,35 @@ class World:
         self._chk(self._lib.cbz_resolver_occupancy(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def set_resolver_split(self, min_contacts, max_worlds=32):
+        """Of the worlds of at least min_contacts contacts (at most 2048, and
+        256 bodies), the max_worlds largest resolve on the kernel of four
+        wavefronts per world; either 0: none (cbz_set_resolver_split; the
+        same results either way)."""
+        self._chk(self._lib.cbz_set_resolver_split(self._h, int(min_contacts), int(max_worlds)))
+
+    def resolver_split(self):
+        a, b = ctypes.c_int32(0), ctypes.c_int32(0)
+        self._chk(self._lib.cbz_get_resolver_split(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune_resolver(self, dt, iter_per_contact=4, steps=5, candidates=((0, 0), (257, 16), (257, 32),
+                                                                           (257, 64), (257, 128)),
+                          surrogate=False, tol=0.06, polish=1):
+        """Times `steps` steps from the current state for each resolver split
+        (min_contacts, max_worlds), restoring the state around each run,
+        keeps the fastest (cbz_autotune_resolver) and returns
+        {"split": (min, max), "ms_per_step": {split: ms}}."""
+        cand = np.asarray(candidates, np.int32).reshape(-1, 2)
+        lo, hi = np.ascontiguousarray(cand[:, 0]), np.ascontiguousarray(cand[:, 1])
+        ms = np.zeros(len(cand), np.float64)
+        best = ctypes.c_int32(-1)
+        self._chk(self._lib.cbz_autotune_resolver(self._h, float(dt), int(iter_per_contact), int(steps),
+                                                  1 if surrogate else 0, float(tol), int(polish), len(cand),
+                                                  _ptr(lo), _ptr(hi), _ptr(ms), ctypes.byref(best)))
+        splits = [tuple(int(x) for x in r) for r in cand]
+        return {"split": splits[best.value], "ms_per_step": {f"{a},{b}": float(m) for (a, b), m in zip(splits, ms)}}
+
     def resolver_world_times(self):
         """[n_worlds, 4] microseconds: start and end of each world's
         workgroup in the last position-only and velocity launches (-1:

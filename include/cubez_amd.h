@@ -246,6 +246,26 @@ int cbz_resolver_world_times(cbz_ctx *ctx, double *us);
  * and LDS) and the device's CU count; the bench derives the time per
  * serial iteration from them. */
 int cbz_resolver_occupancy(cbz_ctx *ctx, int32_t *worlds_per_cu, int32_t *n_cu);
+/* The resolver's split between its two kernels.  Of the worlds of at least
+ * min_contacts contacts (and at most 2048 contacts and 256 bodies), the
+ * max_worlds largest resolve on a persistent kernel of four wavefronts per
+ * world, on a second stream beside the one-wavefront kernel that takes the
+ * others; both give the same bits.  Either value 0 sends every world to the
+ * one-wavefront kernel; the defaults are 257 and 32 (cbz_autotune_resolver
+ * picks it by measurement).  A scheduling choice of this implementation, with
+ * no counterpart in the reference (ResolveContacts, contact.go:208-222, is
+ * the same loop either way). */
+int cbz_set_resolver_split(cbz_ctx *ctx, int32_t min_contacts, int32_t max_worlds);
+int cbz_get_resolver_split(cbz_ctx *ctx, int32_t *min_contacts, int32_t *max_worlds);
+/* The autotuner of that split: from the current state, times n_steps of
+ * cbz_step (surrogate = 0) or of cbz_step_sur (surrogate = 1, with tol and
+ * polish_per_contact) for each candidate split (cand_min[i], cand_max[i]),
+ * restoring the body state before each run and at the end, then keeps the
+ * fastest: its index in *best, the times in ms_per_step[n_cand] (may be
+ * NULL).  Profiling must be off.  No counterpart in the reference. */
+int cbz_autotune_resolver(cbz_ctx *ctx, double dt, int64_t iter_per_contact, int32_t n_steps, int32_t surrogate,
+                          double tol, int64_t polish_per_contact, int32_t n_cand, const int32_t *cand_min,
+                          const int32_t *cand_max, double *ms_per_step, int32_t *best);
 
 /* --- fused stepping --------------------------------------------------------
  * n_steps of updateCallback (examples/cubedrop.go:69-75): integrate, the
