@@ -6,15 +6,29 @@
 namespace cbz {
 
 // The worlds the multi-wave kernel takes: of the worlds it can take
-// (mw_fits), the mw_max largest by contact count (the whole last count
-// bucket of 32 contacts that reaches mw_max, then capped at mw_max in
-// `order`, larger worlds first).  Writes mw_list[0 .. mw_count[0]), mw_flag
-// for every world, and resets the queue head mw_count[1].  One workgroup;
-// thread t looks at kSelectPer consecutive entries of each chunk.
-constexpr int kSelectThreads = 1024, kSelectPer = 16, kSelectBins = kMwMaxContacts / 32 + 1;
+// (mw_fits), the mw_max costliest, the cost being the iterations of the
+// phases this launch runs in the world's last resolve (world_iters), or its
+// contact count where the world has none yet.  The whole last cost bucket
+// that reaches mw_max is taken, then capped at mw_max in `order` (larger
+// worlds first).  Writes mw_list[0 .. mw_count[0]), mw_flag for every
+// world, and resets the queue head mw_count[1].  One workgroup; thread t
+// looks at kSelectPer consecutive entries of each chunk.
+constexpr int kSelectThreads = 1024, kSelectPer = 16, kSelectBins = 257, kSelectShift = 6;
+__device__ __forceinline__ int select_bin(const int64_t* world_iters, int32_t mode, int32_t w, int C) {
+  int64_t cost = 0;
+  if (world_iters) {
+    if (mode & RM_POS) cost += world_iters[2 * w];
+    if (mode & RM_VEL) cost += world_iters[2 * w + 1];
+  }
+  if (cost <= 0) cost = C;
+  const int64_t b = cost >> kSelectShift;
+  return int(b < kSelectBins - 1 ? b : kSelectBins - 1);
+}
+
 __global__ void __launch_bounds__(kSelectThreads)
-    k_mw_select(const int32_t* order, const int32_t* world_cnt, const int32_t* body_off, int32_t n_worlds,
-                int32_t mw_min, int32_t mw_max, int32_t* mw_list, int32_t* mw_count, uint8_t* mw_flag) {
+    k_mw_select(const int32_t* order, const int32_t* world_cnt, const int32_t* body_off, const int64_t* world_iters,
+                int32_t mode, int32_t n_worlds, int32_t mw_min, int32_t mw_max, int32_t* mw_list, int32_t* mw_count,
+                uint8_t* mw_flag) {
   __shared__ int32_t s_hist[kSelectBins];
   __shared__ int32_t s_wave[kSelectThreads / 64];
   __shared__ int32_t s_carry, s_thr;
@@ -22,10 +36,10 @@ __global__ void __launch_bounds__(kSelectThreads)
   for (int i = t; i < kSelectBins; i += kSelectThreads) s_hist[i] = 0;
   if (t == 0) s_carry = 0;
   __syncthreads();
-  // contact-count histogram of the worlds the kernel can take
+  // cost histogram of the worlds the kernel can take
   for (int32_t w = t; w < n_worlds; w += kSelectThreads) {
     const int C = world_cnt[w];
-    if (mw_fits(C, body_off[w + 1] - body_off[w], mw_min)) atomicAdd(&s_hist[C >> 5], 1);
+    if (mw_fits(C, body_off[w + 1] - body_off[w], mw_min)) atomicAdd(&s_hist[select_bin(world_iters, mode, w, C)], 1);
   }
   __syncthreads();
   if (t == 0) {
@@ -36,8 +50,9 @@ __global__ void __launch_bounds__(kSelectThreads)
   __syncthreads();
   const int thr = s_thr;
   for (int32_t b0 = 0; b0 < n_worlds; b0 += kSelectThreads * kSelectPer) {
+    // this thread's entries: world ids, and a bit mask of the candidates
     int32_t w[kSelectPer];
-    int cnt = 0;
+    uint32_t cand = 0;
 #pragma unroll
     for (int k = 0; k < kSelectPer; ++k) {
       const int32_t i = b0 + t * kSelectPer + k;
@@ -45,14 +60,15 @@ __global__ void __launch_bounds__(kSelectThreads)
     }
 #pragma unroll
     for (int k = 0; k < kSelectPer; ++k) {
-      if (w[k] < 0) continue;
-      const int C = world_cnt[w[k]];
-      if (!mw_fits(C, body_off[w[k] + 1] - body_off[w[k]], mw_min) || (C >> 5) < thr) {
-        mw_flag[w[k]] = 0;
-        w[k] = -1;
+      if (w[k] >= 0) {
+        const int C = world_cnt[w[k]];
+        const bool c = mw_fits(C, body_off[w[k] + 1] - body_off[w[k]], mw_min) &&
+                       select_bin(world_iters, mode, w[k], C) >= thr;
+        cand |= uint32_t(c) << k;
+        if (!c) mw_flag[w[k]] = 0;
       }
-      cnt += w[k] >= 0;
     }
+    const int cnt = __builtin_popcount(cand);
     const int inc = wave_incl_scan_i32(cnt);
     if (lane == 63) s_wave[wv] = inc;
     __syncthreads();
@@ -60,7 +76,7 @@ __global__ void __launch_bounds__(kSelectThreads)
     for (int v = 0; v < wv; ++v) pos += s_wave[v];
 #pragma unroll
     for (int k = 0; k < kSelectPer; ++k) {
-      if (w[k] < 0) continue;
+      if (!((cand >> k) & 1u)) continue;
       const bool take = pos < mw_max;
       if (take) mw_list[pos] = w[k];
       mw_flag[w[k]] = take ? 1 : 0;
@@ -88,8 +104,8 @@ hipError_t launch_both(const RS<R>& a, const ResolveLaunch<R>& L, hipStream_t st
   if (mw) {
     if ((e = hipEventRecord(L.fork, st)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(L.side, L.fork, 0)) != hipSuccess) return e;
-    k_mw_select<<<1, kSelectThreads, 0, L.side>>>(a.order, a.world_cnt, a.body_off, a.n_worlds, L.mw_min,
-                                                   L.mw_max, L.mw_list, L.mw_count, L.mw_flag);
+    k_mw_select<<<1, kSelectThreads, 0, L.side>>>(a.order, a.world_cnt, a.body_off, a.world_iters, a.mode, a.n_worlds,
+                                                   L.mw_min, L.mw_max, L.mw_list, L.mw_count, L.mw_flag);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipEventRecord(L.sel, L.side)) != hipSuccess) return e;
     int grid = L.mw_max < L.n_cu ? L.mw_max : L.n_cu;

@@ -135,3 +135,37 @@ def test_multiwave_polish_equals_one_wave():
             fb = w.step_sur(sc.dt, 4, 4, tol=0.06, polish=1)
             out[split] = {"fb": np.asarray(fb), **_snapshot(w)}
     _same(out[(0, 0)], out[(1, 4096)], "polish")
+
+
+def test_partial_split_on_many_worlds_resolves_every_world():
+    # 1024 settled worlds, a few dozen of them on the multi-wave kernel:
+    # the selection must hand every world to exactly one kernel (a world
+    # flagged for the multi-wave kernel but not in its list would not be
+    # resolved at all), whatever the split, in the exact and the surrogate
+    # step (whose two launches select by different costs)
+    sc = scenes.batched(0, 1024)
+    with World(sc, precision="f32") as w:
+        w.set_resolver_split(0, 0)
+        w.step(sc.dt, 240, 4)
+        ck = w.checkpoint()
+    params, norm = surrogate.load()
+    for surrogate_step in (False, True):
+        out = {}
+        for split in ((0, 0), (257, 32), (257, 100), (600, 7)):
+            with World(precision="f32") as w:
+                w.restore(ck)
+                w.set_resolver_split(*split)
+                if surrogate_step:
+                    w.sur_set_norm(norm)
+                    w.sur_set_params(params)
+                    fb, counts = w.step_sur(sc.dt, 4, 4, tol=0.06, polish=1, counts=True)
+                    out[split] = {"fb": np.asarray(fb), **_snapshot(w)}
+                else:
+                    counts = w.step(sc.dt, 4, 4, counts=True)
+                    out[split] = _snapshot(w)
+                # every world with penetrating contacts ran its position phase
+                it = w.resolver_iterations()
+                pen = w.contacts(-1)["pen"]
+                assert len(pen) and it[:, 0].sum() > 0
+        for split in ((257, 32), (257, 100), (600, 7)):
+            _same(out[(0, 0)], out[split], f"split {split}, surrogate {surrogate_step}")
