@@ -38,6 +38,10 @@ Prints one JSON line.  Besides the metric it reports:
   cpu_baseline           the FP64 oracle (a C restatement of the Go
                          reference) on a bounded sample of the same worlds,
                          on all host cores and on one
+  resolver_split         per leg, the split between the one- and the
+                         four-wavefront resolver kernels that the autotuner
+                         (cbz_autotune_resolver) picked on the settled state,
+                         with every candidate's ms/step
 """
 import argparse
 import json
@@ -204,7 +208,7 @@ def _log(msg):
 
 
 RESOLVER_SPLIT = "auto"  # --resolver-split: "auto", "default" or "min,max"
-TUNE_STEPS = 4
+TUNE_STEPS = 4  # timed steps per candidate split
 
 
 def _start(w, sc, ck):
@@ -217,12 +221,18 @@ def _start(w, sc, ck):
     w.synchronize()
 
 
+TUNE_CK = None  # the settled state the split is tuned on
+
+
 def _tune(w, sc, precision, surrogate=False, tol=0.06, polish=1):
-    """The resolver split of this leg: measured from the window's start
-    state by the autotuner (the state is restored), or as given."""
+    """The resolver split of this leg: measured by the autotuner from the
+    settled state (the world's state is then loaded from it; callers
+    restore their window's start afterwards), or as given."""
     if RESOLVER_SPLIT != "auto":
         return {"split": list(w.resolver_split()), "tuned": False}
     from cubez_amd import autotune
+    if TUNE_CK is not None:
+        w.restore(TUNE_CK)
     r = autotune.tune(w, sc.dt, sc.iter_mult, precision, surrogate, steps=TUNE_STEPS, tol=tol, polish=polish,
                       use_cache=False)
     ms = r["ms_per_step"]
@@ -472,6 +482,11 @@ def main():
 
     _log(f"bench: settling to step {start}")
     ck, settle_s = settle(World, sc, start)
+    # the resolver split is tuned on the settled regime whatever the window
+    # (from step 0 the worlds are in free fall with few contacts)
+    global TUNE_CK
+    TUNE_CK = ck if start >= DEFAULT_WINDOW_START or RESOLVER_SPLIT != "auto" else \
+        settle(World, sc, DEFAULT_WINDOW_START)[0]
     _log("bench: exact FP32")
     exact32 = run_path(World, sc, ck, "f32", args.steps, args.warmup)
     _log("bench: exact FP64")

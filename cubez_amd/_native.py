@@ -59,7 +59,7 @@ EXPORTS = (
     "cbz_calculate_derived", "cbz_integrate", "cbz_narrowphase", "cbz_narrowphase_pairs",
     "cbz_contact_counts", "cbz_get_contacts", "cbz_set_contacts", "cbz_resolve",
     "cbz_resolver_iterations", "cbz_step", "cbz_synchronize", "cbz_set_profiling",
-    "cbz_phase_times", "cbz_selftest_f64",
+    "cbz_phase_times", "cbz_selftest_f64", "cbz_selftest_sort",
     "cbz_sur_num_params", "cbz_sur_set_params", "cbz_sur_get_params", "cbz_sur_set_norm", "cbz_sur_get_norm",
     "cbz_sur_reserve", "cbz_sur_collect", "cbz_sur_dataset", "cbz_sur_load_dataset", "cbz_sur_init",
     "cbz_sur_train", "cbz_sur_grad", "cbz_sur_predict", "cbz_sur_resolve", "cbz_sur_world_mask", "cbz_sur_phase_times", "cbz_step_sur", "cbz_integrate_timed",
@@ -127,6 +127,7 @@ def load():
         "cbz_set_profiling": ([P, i32], i32),
         "cbz_phase_times": ([P, P, P], i32),
         "cbz_selftest_f64": ([P, i32, i64, P, P, P], i32),
+        "cbz_selftest_sort": ([P, i64, P, P, P, P], i32),
         "cbz_sur_num_params": ([], i64),
         "cbz_sur_set_params": ([P, P], i32),
         "cbz_sur_get_params": ([P, P], i32),

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "cbz_broad.h"
+#include "cbz_sort.h"
 #include "cbz_launch.h"
 #include "cbz_mlp.h"
 #include "cbz_types.h"
@@ -2180,6 +2181,33 @@ int cbz_selftest_f64(cbz_ctx* c, int32_t op, int64_t n, const double* a, const d
   (void)hipFree(db);
   (void)hipFree(dout);
   if (e != hipSuccess) return fail(c, CBZ_ERR_HIP, std::string("selftest: ") + hipGetErrorString(e));
+  return CBZ_OK;
+}
+
+int cbz_selftest_sort(cbz_ctx* c, int64_t n, const uint64_t* keys, const int32_t* vals, uint64_t* keys_out,
+                      int32_t* vals_out) {
+  if (!c || n < 0 || n >= (int64_t(1) << 31) || (n > 0 && (!keys || !keys_out)) || (!vals != !vals_out))
+    return fail(c, CBZ_ERR_ARG, "selftest_sort: bad arguments");
+  if (n == 0) return CBZ_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const bool v = vals != nullptr;
+  const size_t tb = radix_sort_bytes(n, v);
+  void *dk = nullptr, *dko = nullptr, *dv = nullptr, *dvo = nullptr, *tmp = nullptr;
+  hipError_t e = hipMalloc(&dk, n * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc(&dko, n * sizeof(uint64_t));
+  if (e == hipSuccess && v) e = hipMalloc(&dv, n * sizeof(int32_t));
+  if (e == hipSuccess && v) e = hipMalloc(&dvo, n * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc(&tmp, tb);
+  if (e == hipSuccess) e = hipMemcpy(dk, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && v) e = hipMemcpy(dv, vals, n * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = radix_sort_u64((const uint64_t*)dk, (uint64_t*)dko, (const int32_t*)dv, (int32_t*)dvo, n, tmp, tb, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(keys_out, dko, n * sizeof(uint64_t), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && v) e = hipMemcpy(vals_out, dvo, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+  for (void* q : {dk, dko, dv, dvo, tmp})
+    if (q) (void)hipFree(q);
+  if (e != hipSuccess) return fail(c, CBZ_ERR_HIP, std::string("selftest_sort: ") + hipGetErrorString(e));
   return CBZ_OK;
 }
 

@@ -12,11 +12,11 @@
 // every overlap, plus every body-plane pair, are sorted into the order of
 // examples/cubedrop.go:47-64 (row i ascending, its planes first, then j
 // ascending) and handed to the explicit-list narrowphase: its contacts are
-// those of the all-pairs narrowphase, in the same order.
-#include <hipcub/hipcub.hpp>
-
+// those of the all-pairs narrowphase, in the same order.  Both sorts are
+// the stable radix sort of k_sort.hip.
 #include "cbz_broad.h"
 #include "cbz_launch.h"
+#include "cbz_sort.h"
 #include "cbz_wave.h"
 
 namespace cbz {
@@ -174,9 +174,7 @@ hipError_t launch_bp_bounds_sort(const BroadArgs<R>& a, void* tmp, size_t tmp_by
   k_bp_bounds<R><<<(a.nb + 255) / 256, 256, 0, st>>>(a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  size_t bytes = tmp_bytes;
-  return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, a.key_in, a.key_out, a.val_in, a.val_out, a.nb, 0, 64,
-                                            st);
+  return radix_sort_u64(a.key_in, a.key_out, a.val_in, a.val_out, a.nb, tmp, tmp_bytes, st);
 }
 
 template <typename R>
@@ -190,10 +188,9 @@ hipError_t launch_bp_sweep(const BroadArgs<R>& a, const BroadPlanes& p, hipStrea
 }
 
 hipError_t launch_bp_list(const BroadList& L, void* tmp, size_t tmp_bytes, hipStream_t st) {
-  size_t bytes = tmp_bytes;
   hipError_t e = hipSuccess;
   if (L.n > 0) {
-    e = hipcub::DeviceRadixSort::SortKeys(tmp, bytes, L.pkey_in, L.pkey, int(L.n), 0, 64, st);
+    e = radix_sort_u64(L.pkey_in, L.pkey, nullptr, nullptr, L.n, tmp, tmp_bytes, st);
     if (e != hipSuccess) return e;
     k_bp_unpack<<<int((L.n + 255) / 256), 256, 0, st>>>(L);
   }
@@ -203,12 +200,8 @@ hipError_t launch_bp_list(const BroadList& L, void* tmp, size_t tmp_bytes, hipSt
 
 // Temporary storage the two radix sorts need for n bodies and m pair keys.
 hipError_t bp_sort_bytes(int32_t n, int64_t m, size_t* bytes) {
-  size_t b1 = 0, b2 = 0;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, b1, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                                    (const int32_t*)nullptr, (int32_t*)nullptr, n, 0, 64);
-  if (e != hipSuccess) return e;
-  e = hipcub::DeviceRadixSort::SortKeys(nullptr, b2, (const uint64_t*)nullptr, (uint64_t*)nullptr, int(m), 0, 64);
-  if (e != hipSuccess) return e;
+  if (m >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+  const size_t b1 = radix_sort_bytes(n, true), b2 = radix_sort_bytes(m, false);
   *bytes = b1 > b2 ? b1 : b2;
   return hipSuccess;
 }

@@ -76,9 +76,9 @@ def go_invert3(m):
         a[1] * a[6] - a[0] * a[7],
         a[0] * a[4] - a[1] * a[3],
     ], axis=1)
-    with np.errstate(divide="ignore", invalid="ignore"):
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
         s = 1 / det
-    out = out * s[:, None]
+        out = out * s[:, None]  # singular rows are zeroed below
     out[go_real_equal(det, 0.0)] = 0.0
     return out
 
@@ -267,6 +267,70 @@ def ballistic(seed=0, n_spheres=512, n_cubes=512):
     po, pn, pof = _ground(1)
     return Scene(np.array([0, n], np.int32), po, pn, pof, f, name=f"ballistic-{n}-s{seed}",
                  dt=1.0 / 60.0, steps=300, iter_mult=4)
+
+
+def ballistic_example(n_bullets=12):
+    """The scene of examples/ballistic.go, 1 world: the cube (half-extent 1,
+    mass 8, at (0,5,0), default acceleration, :162-171), the backboard
+    (half-extents (0.5,2,0.25) at (0,2,-10), infinite mass, never integrated:
+    :180-185, and updateObjects :24-41 integrates only the cube and the
+    bullets) and n_bullets bullets as fire() makes them (:205-227: spheres of
+    radius 0.2, mass 1.5, velocity (0,0,-40), acceleration (0,-2.5,0)).  The
+    example fires a bullet per key press from (0,1.5,20); here all bullets
+    exist from step 0, spread across the backboard and the cube (x in
+    {-0.9,-0.45,0,0.45,0.9}) and staggered in z (20 + 2.5 k), as if fired
+    one after another.  Step it with the example's pair order
+    (ballistic_example_pairs) and ResolveContacts(8 * contacts)."""
+    n = 2 + n_bullets
+    f = _new_bodies(n)  # NewRigidBody defaults: acceleration (0,-9.78,0)
+    shape = np.array([0, 0] + [1] * n_bullets, np.int32)
+    half = np.zeros((n, 3))
+    half[0] = 1.0
+    half[1] = (0.5, 2.0, 0.25)
+    half[2:, 0] = 0.2
+    pos = np.zeros((n, 3))
+    pos[0] = (0.0, 5.0, 0.0)
+    pos[1] = (0.0, 2.0, -10.0)
+    k = np.arange(n_bullets)
+    pos[2:, 0] = ((k % 5) - 2) * 0.45
+    pos[2:, 1] = 1.5
+    pos[2:, 2] = 20.0 + 2.5 * k
+    f["shape"][:] = shape
+    f["half"][:] = half
+    f["pos"][:] = pos
+    f["vel"][2:] = (0.0, 0.0, -40.0)
+    f["acc"][2:] = (0.0, -2.5, 0.0)
+    iner = np.zeros((n, 9))
+    iner[0] = block_inertia(half[:1], np.array([8.0]))[0]
+    coeff = 0.4 * 1.5 * 0.2 * 0.2
+    iner[2:] = inertia_coeffs(np.full(n_bullets, coeff), np.full(n_bullets, coeff), np.full(n_bullets, coeff))
+    f["inv_mass"][0] = 1.0 / 8.0
+    f["inv_mass"][1] = 0.0  # SetInfiniteMass, rigidbody.go:130-134
+    f["inv_mass"][2:] = 1.0 / 1.5
+    f["iit_body"][:] = go_invert3(iner)  # the backboard's tensor is never set: zero
+    f["iit_body"][1] = 0.0
+    f["flags"][1] |= 4  # the caller never integrates it
+    po, pn, pof = _ground(1)
+    return Scene(np.array([0, n], np.int32), po, pn, pof, f, name=f"ballistic-example-{n_bullets}",
+                 dt=1.0 / 60.0, steps=300, iter_mult=8)
+
+
+def ballistic_example_pairs(n_bullets):
+    """generateContacts of examples/ballistic.go:43-83 as an ordered pair list
+    for cbz_narrowphase_pairs: (cube, ground), (cube, backboard), then per
+    bullet b: (b, ground), (cube, b), (backboard, b), and (b2, b) for every
+    other bullet b2.  Bodies: 0 cube, 1 backboard, 2.. bullets; -1 is the
+    ground plane."""
+    one, two = [0, 0], [-1, 1]
+    bullets = list(range(2, 2 + n_bullets))
+    for b in bullets:
+        one += [b, 0, 1]
+        two += [-1, b, b]
+        for b2 in bullets:
+            if b2 != b:
+                one.append(b2)
+                two.append(b)
+    return np.array(one, np.int32), np.array(two, np.int32)
 
 
 def batched(seed=0, n_worlds=4096, n_min=8, n_max=256):

@@ -273,6 +273,19 @@ class World:
     # -- learned resolver surrogate ---------------------------------------------
     N_FEAT, N_OUT = 40, 12
 
+    def selftest_sort(self, keys, vals=None):
+        """The broadphase's device radix sort on host arrays
+        (cbz_selftest_sort): (sorted keys, values in the same permutation)."""
+        keys = np.ascontiguousarray(keys, np.uint64)
+        ko = np.empty_like(keys)
+        vo = None
+        if vals is not None:
+            vals = np.ascontiguousarray(vals, np.int32)
+            vo = np.empty_like(vals)
+        self._chk(self._lib.cbz_selftest_sort(self._h, len(keys), _ptr(keys), None if vals is None else _ptr(vals),
+                                              _ptr(ko), None if vo is None else _ptr(vo)))
+        return ko, vo
+
     def sur_num_params(self):
         return int(self._lib.cbz_sur_num_params())
 

@@ -295,6 +295,13 @@ int cbz_integrate_timed(cbz_ctx *ctx, double dt, int32_t n, double *ms_per_launc
 /* Device FP64 arithmetic self-test (the rounding the exact path relies on):
  * op 0: a/b, 1: sqrt(a), 2: a*b, 3: a+b, 4: a-b, elementwise over n. */
 int cbz_selftest_f64(cbz_ctx *ctx, int32_t op, int64_t n, const double *a, const double *b, double *out);
+/* Self-test of the broadphase's device sort (a stable radix sort of 64-bit
+ * keys, values carried along): keys_out = keys sorted, equal keys in input
+ * order, vals_out the values in the same permutation (vals and vals_out
+ * both NULL: keys only).  Host buffers, n < 2^31.  No counterpart in the
+ * reference. */
+int cbz_selftest_sort(cbz_ctx *ctx, int64_t n, const uint64_t *keys, const int32_t *vals, uint64_t *keys_out,
+                      int32_t *vals_out);
 
 /* --- learned resolver surrogate --------------------------------------------
  * A 40 -> 256 -> 256 -> 256 -> 12 MLP (SiLU, BF16 weights and activations,

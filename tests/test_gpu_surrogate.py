@@ -319,6 +319,21 @@ def test_training_gains_over_the_method_without_the_network():
     assert it_net < it_zero
 
 
+def test_collection_schedule_of_1024_worlds_trains():
+    """The collection that once ended in an illegal memory access at the
+    first training GEMM (64 seeds x 16 worlds, snapshots at steps 60, 120,
+    200, 300, 450 and 600, about 2 M rows), then training on it.  The fault
+    came from k_gemm reading operand rows and bias entries past its buffers
+    (layer-0 weight gradient, output-layer bias), which this schedule's
+    allocations exposed; the GEMMs now read only inside them."""
+    with World(scenes.batched(0, 2)) as w:
+        n = surrogate.collect(w, range(64), n_worlds=16, at=(60, 120, 200, 300, 450, 600), reserve=3_200_000)
+        assert n > 1_000_000
+        losses = surrogate.train(w, steps=200, batch=4096, lr=1e-3, seed=3)
+        assert np.all(np.isfinite(losses))
+        assert np.all(np.isfinite(w.sur_get_params()))
+
+
 def test_captured_surrogate_step_equals_eager_steps():
     """cbz_step_sur without counts replays one captured graph per step (no
     host round trip: the feature and network grids read the contact count on
