@@ -14,7 +14,7 @@ constexpr int kOut = 12;       // outputs: dv0, dw0, dv1, dw1
 constexpr int kOutPad = 32;    // padded to the MFMA tile (32)
 constexpr int kLayers = 4;
 constexpr int kSurBodies = 256;     // bodies per world the surrogate takes
-constexpr int kSurContacts = 1024;  // contacts per world the surrogate takes
+constexpr int kSurContacts = 2048;  // contacts per world the surrogate takes
 
 // layer l maps kIn[l] -> kOutL[l] features (unpadded)
 __host__ __device__ constexpr int layer_in(int l) { return l == 0 ? kFeat : kHidden; }
