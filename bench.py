@@ -224,7 +224,7 @@ def _start(w, sc, ck):
 TUNE_CK = None  # the settled state the split is tuned on
 
 
-def _tune(w, sc, precision, surrogate=False, tol=0.06, polish=1):
+def _tune(w, sc, precision, surrogate=False, tol=0.06, polish=4):
     """The resolver split of this leg: measured by the autotuner from the
     settled state (the world's state is then loaded from it; callers
     restore their window's start afterwards), or as given."""

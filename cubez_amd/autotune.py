@@ -48,7 +48,7 @@ def _load(path):
 
 
 def tune(world, dt, iter_per_contact=4, precision="f64", surrogate=False, steps=5, candidates=CANDIDATES,
-         tol=0.06, polish=1, use_cache=True):
+         tol=0.06, polish=4, use_cache=True):
     """Set the fastest split for this scene on `world` (from the cache when
     the scene's key is there, else by measurement from the current state,
     which is restored) and return {"split", "ms_per_step", "cached", "key"}."""

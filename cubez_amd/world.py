@@ -205,7 +205,7 @@ class World:
 
     def autotune_resolver(self, dt, iter_per_contact=4, steps=5, candidates=((0, 0), (257, 16), (257, 32),
                                                                            (257, 64), (257, 128)),
-                          surrogate=False, tol=0.06, polish=1):
+                          surrogate=False, tol=0.06, polish=4):
         """Times `steps` steps from the current state for each resolver split
         (min_contacts, max_worlds), restoring the state around each run,
         keeps the fastest (cbz_autotune_resolver) and returns
@@ -358,7 +358,7 @@ class World:
 
     SUR_ZERO_PREDICTION = 1
 
-    def sur_resolve(self, dt, iter_per_contact=4, tol=0.06, polish=1, zero_prediction=False):
+    def sur_resolve(self, dt, iter_per_contact=4, tol=0.06, polish=4, zero_prediction=False):
         """ResolveContacts with the surrogate: exact positions, predicted
         velocity changes polished by the worst-first loop to
         velocityEpsilon + tol within polish * C iterations, else the plain
@@ -383,7 +383,7 @@ class World:
         self._chk(self._lib.cbz_sur_world_mask(self._h, _ptr(m)))
         return m
 
-    def step_sur(self, dt, n_steps=1, iter_per_contact=4, tol=0.06, polish=1, zero_prediction=False,
+    def step_sur(self, dt, n_steps=1, iter_per_contact=4, tol=0.06, polish=4, zero_prediction=False,
                  counts=False):
         """n_steps with the surrogate resolver; returns the per-step fallback
         counts (and [n_steps, n_worlds] contact counts with counts=True)."""
