@@ -25,7 +25,7 @@ Prints one JSON line.  Besides the metric it reports:
   exact_f64, exact_f32   the exact resolver (hipGraph-replayed step)
   zero_prediction        the surrogate's polish started from a zero
                          prediction: the same method without the network
-  fallback_rate          worlds re-solved by the plain method / world-steps
+  fallback_rate          worlds re-solved by a plain phase / world-steps
   error                  penetration left after the window (narrowphase of
                          the final state: mean, p99, max) for the surrogate
                          and both exact paths, and the surrogate's final
@@ -369,14 +369,16 @@ def run_surrogate(World, sc, ck, model, steps, warmup, tol, polish, zero=False):
     ms, cnt = w.sur_phase_times()
     crit = critical_path(w)
     w.close()
-    n_res, contacts, fallbacks, vel_iters = (int(x) for x in cnt)
-    names = ("position_phase", "features", "network", "per_body_sums", "velocity_phase")
+    n_res, contacts, fallbacks, vel_iters, pos_fallbacks, pos_iters = (int(x) for x in cnt)
+    names = ("prepare", "features", "network", "per_body_sums", "position_phase", "velocity_phase")
     out.update({
         "transfer": {"wall_s": wall, "upload_s": upload_s, "download_state_s": download_s,
                      "download_pos_quat_s": pos_quat_s},
         "resolver_ms_per_step": {k: round(ms[i] / max(n_res, 1), 5) for i, k in enumerate(names)},
         "contacts_per_step": contacts / max(n_res, 1),
         "velocity_iterations_per_step": vel_iters / max(n_res, 1),
+        "position_iterations_per_step": pos_iters / max(n_res, 1),
+        "position_fallback_rate": pos_fallbacks / max(n_res * sc.n_worlds, 1),
         "network_ms_total": ms[2],
         "contacts_total": contacts,
         "residual_penetration": resid,
@@ -506,9 +508,11 @@ def main():
                                                                     f"into every leg)"))
     workload = (f"batched worlds: {args.worlds} worlds, plane + n~U{{8..256}} cubes each ({nb} bodies), {window} at "
                 f"dt=1/60, all-pairs narrowphase, maxIterations=4*C, FP32 state with FP64 SAT and contact basis, ")
-    workload += (f"resolver: exact position phase + BF16 MLP surrogate for the velocity phase, polished by "
-                 f"the exact worst-first loop to velocityEpsilon+{tol} m/s within {polish}*C iterations, "
-                 f"else the world is re-solved by the plain method" if use_sur else "resolver: exact")
+    ptol = surrogate.DEFAULT_POS_TOL
+    workload += (f"resolver: BF16 MLP surrogate for both phases, each polished by the exact worst-first loop "
+                 f"(positions to positionEpsilon+{ptol} m, velocities to velocityEpsilon+{tol} m/s) within "
+                 f"{polish}*C iterations, else the world's phase is re-solved by the plain method"
+                 if use_sur else "resolver: exact")
     main_path = sur if use_sur else exact32
     out = {
         "metric": METRIC,
@@ -527,7 +531,8 @@ def main():
             "workload": workload, "worlds": args.worlds, "bodies": nb, "seed": args.seed,
             "window_start": start, "window_steps": args.steps,
             "resolver": "surrogate+fallback" if use_sur else "exact", "tolerance_m_per_s": tol,
-            "polish_per_contact": polish, "network": "40-256-256-256-12 SiLU, BF16 MFMA, FP32 accumulate",
+            "polish_per_contact": polish, "position_tolerance_m": surrogate.DEFAULT_POS_TOL,
+            "network": "40-256-256-256-13 SiLU, BF16 MFMA, FP32 accumulate",
         },
         "settle_s": settle_s,
     }
@@ -540,6 +545,8 @@ def main():
         out["speedup_vs_exact_f32"] = sur["value"] / exact32["value"]
         out["fallback_rate"] = sur["fallback_rate"]
         out["velocity_iterations_per_step"] = {"surrogate": sur["velocity_iterations_per_step"]}
+        out["position_iterations_per_step"] = {"surrogate": sur["position_iterations_per_step"]}
+        out["position_fallback_rate"] = sur["position_fallback_rate"]
         out["critical_path_last_step"] = {"surrogate": sur["critical_path_last_step"],
                                           "exact_f64": exact64["critical_path_last_step"],
                                           "exact_f32": exact32["critical_path_last_step"]}
@@ -562,8 +569,11 @@ def main():
             "download_state_ms": 1000 * tr["download_state_s"], "download_pos_quat_ms": 1000 * tr["download_pos_quat_s"],
         }
         out["error"] = {
-            "bar": f"tolerance {tol} m/s of residual closing speed per contact = {tol / 60.0 * 1000:.1f} mm of "
-                   "extra penetration per 1/60 s step; worlds above it are re-solved exactly",
+            "bar": f"per contact, at most positionEpsilon + {ptol * 1000:.1f} mm of penetration after the "
+                   f"position phase and velocityEpsilon + {tol} m/s of residual closing speed (= "
+                   f"{tol / 60.0 * 1000:.1f} mm of extra penetration per 1/60 s step) after the velocity phase; "
+                   "a world above either, or whose position moves separate a contact by more than twice its "
+                   "deepest penetration, is re-solved by that plain phase",
             "residual_penetration": {"surrogate": sur["residual_penetration"],
                                      "exact_f32": exact32["residual_penetration"],
                                      "exact_f64": exact64["residual_penetration"]},
@@ -615,7 +625,8 @@ def main():
     if dom in ("position_phase", "velocity_phase", "resolve"):
         ms = phases[dom] if use_sur else kern32["resolve"]
         ach = res_bytes / (ms / 1000.0) / 1e9
-        roof = {"kernel": {"position_phase": "k_resolve_f32<plain> (prepareContacts + adjustPositions)",
+        roof = {"kernel": {"position_phase": "k_resolve_f32<polish> (adjustPositions from the network's moves, "
+                                             "with the exact fallback)",
                            "velocity_phase": "k_resolve_f32<polish> (adjustVelocities from the network's changes, "
                                              "with the exact fallback)",
                            "resolve": "k_resolve_f32<plain>"}[dom],

@@ -67,7 +67,7 @@ EXPORTS = (
     "cbz_checkpoint_size", "cbz_checkpoint_save", "cbz_checkpoint_load", "cbz_narrowphase_broad",
     "cbz_resolver_occupancy", "cbz_sur_check_gemms", "cbz_resolver_world_times", "cbz_set_resolver_split", "cbz_get_resolver_split",
     "cbz_autotune_resolver",
-    "cbz_sur_predict_device",
+    "cbz_sur_predict_device", "cbz_sur_position_mask", "cbz_sur_set_position", "cbz_sur_get_position",
 )
 
 
@@ -149,6 +149,9 @@ def load():
         "cbz_sur_predict_device": ([P, i32, P, P], i32),
         "cbz_sur_resolve": ([P, f64, i64, f64, i64, i32, P], i32),
         "cbz_sur_world_mask": ([P, P], i32),
+        "cbz_sur_position_mask": ([P, P], i32),
+        "cbz_sur_set_position": ([P, f64, i64], i32),
+        "cbz_sur_get_position": ([P, P, P], i32),
         "cbz_step_sur": ([P, f64, i32, i64, f64, i64, i32, P, P], i32),
         "cbz_sur_phase_times": ([P, P, P], i32),
         "cbz_integrate_timed": ([P, f64, i32, P, P], i32),

@@ -190,7 +190,7 @@ struct cbz_context {
     int64_t adam_t = 0;
     __bf16* Wnk[cbz::kLayers] = {};  // [Npad x Kpad] forward operand
     __bf16* Wkn[cbz::kLayers] = {};  // [Kpad x Npad] backward operand
-    float* norm = nullptr;  // in_mean 40, in_inv_std 40, out_mean 12, out_std 12, out_inv_std 12
+    float* norm = nullptr;  // in_mean 40, in_inv_std 40, out_mean 13, out_std 13, out_inv_std 13
     // dataset of (raw features, labels)
     float* dsX = nullptr;
     float* dsY = nullptr;
@@ -207,6 +207,16 @@ struct cbz_context {
     float* delta = nullptr;  // [6][n_bodies] predicted change per body
     void* vel0 = nullptr;    // [7][n_bodies] velocity, rotation, motion before it
     void* cvel0 = nullptr;   // [3][cap] contact velocities as prepare left them
+    // the position phase's polish: per-world mask, predicted change per body
+    // [6][n_bodies], snapshot [22][n_bodies] of the sleeping bodies
+    uint8_t* pmask = nullptr;
+    float* dpos = nullptr;
+    void* psnap = nullptr;
+    // position polish tolerance (m) above positionEpsilon and budget per
+    // contact; a negative tolerance gives every world the plain position phase
+    double pos_tol = 0.0;
+    int64_t pos_polish = 4;
+    bool last_pos = false;  // the last surrogate resolve polished positions (pmask is valid)
     // training buffers
     int batch_cap = 0;
     int64_t* idx = nullptr;
@@ -241,15 +251,16 @@ struct cbz_context {
 
   // profiling
   bool prof = false;
-  // ev[0..7]: cbz_step phases; ev[8..13]: surrogate resolve phases
-  hipEvent_t ev[14] = {};
+  // ev[0..7]: cbz_step phases; ev[8..14]: surrogate resolve phases
+  hipEvent_t ev[15] = {};
   double acc_ms[6] = {0, 0, 0, 0, 0, 0};
   int64_t acc_cnt[4] = {0, 0, 0, 0};  // steps, bodies integrated, contacts, resolver iterations
-  // surrogate resolves: ms [position phase, features, network, per-body
-  // sums, velocity phase], counts [resolves, contacts, fallbacks, velocity
-  // iterations]
-  double sur_ms[5] = {0, 0, 0, 0, 0};
-  int64_t sur_cnt[4] = {0, 0, 0, 0};
+  // surrogate resolves: ms [prepare, features, network, per-body sums,
+  // position phase, velocity phase], counts [resolves, contacts, worlds
+  // given a plain phase, velocity iterations, worlds given the plain
+  // position phase, position iterations]
+  double sur_ms[6] = {0, 0, 0, 0, 0, 0};
+  int64_t sur_cnt[6] = {0, 0, 0, 0, 0, 0};
   uint32_t* d_awake = nullptr;
   bool count_awake = false;
   int64_t* d_prof = nullptr;           // [4] device-side step counters (k_prof_accum)
@@ -367,6 +378,9 @@ void free_sur_buffers(cbz_ctx* c) {
   dfree(u.delta);
   dfree(u.vel0);
   dfree(u.cvel0);
+  dfree(u.pmask);
+  dfree(u.dpos);
+  dfree(u.psnap);
   u.inf_cap = 0;
   u.lay_cap = 0;
   u.generation += 1;
@@ -631,9 +645,14 @@ int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode 
   L.world_mask = mask;
   L.labels = labels;
   if (mode & RM_POLISH) {
-    L.delta = c->sur.delta;
-    L.vel0 = (R*)c->sur.vel0;
-    L.cvel0 = (R*)c->sur.cvel0;
+    if (mode & RM_POS) {
+      L.delta = c->sur.dpos;
+      L.vel0 = (R*)c->sur.psnap;
+    } else {
+      L.delta = c->sur.delta;
+      L.vel0 = (R*)c->sur.vel0;
+      L.cvel0 = (R*)c->sur.cvel0;
+    }
     L.polish_tol = (R)polish_tol;
     L.polish_mult = polish_mult;
   }
@@ -649,6 +668,10 @@ int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode 
   L.world_time = c->world_time;
   L.mw_min = c->mw_min;
   L.mw_max = c->mw_max;
+  // A launch without iterations (prepareContacts only) gains nothing from the
+  // multi-wave kernel, and the position polish runs in the one-wavefront
+  // kernel only.
+  if ((mult == 0 && fixed == 0) || ((mode & RM_POLISH) && (mode & RM_POS))) L.mw_max = 0;
   L.mw_list = c->mw_list;
   L.mw_count = c->mw_count;
   L.mw_flag = c->mw_flag;
@@ -1159,6 +1182,9 @@ int sur_ensure_inference(cbz_ctx* c, int64_t rows) {
   if ((rc = dalloc(c, &u.delta, 6 * nb))) return rc;
   if ((rc = dalloc_bytes(c, &u.vel0, 7 * nb * real_size(c)))) return rc;
   if ((rc = dalloc_bytes(c, &u.cvel0, 3 * (size_t)std::max<int64_t>(c->cap, 1) * real_size(c)))) return rc;
+  if ((rc = dalloc(c, &u.pmask, (size_t)std::max(c->nw, 1)))) return rc;
+  if ((rc = dalloc(c, &u.dpos, 6 * nb))) return rc;
+  if ((rc = dalloc_bytes(c, &u.psnap, 22 * nb * real_size(c)))) return rc;
   u.inf_cap = Mpad;
   u.generation += 1;  // buffers moved: a captured surrogate step is stale
   return CBZ_OK;
@@ -1177,90 +1203,113 @@ int sur_ensure_layered(cbz_ctx* c, int64_t rows) {
   return CBZ_OK;
 }
 
-// Profiling of one surrogate resolve (events ev[8..12] recorded, ev[13]
-// recorded here): accumulates the phase times, the contact count, the
-// fallbacks and the velocity-phase iterations.
-int sur_profile(cbz_ctx* c) {
+// Profiling of one surrogate resolve (events ev[8..13] recorded, ev[14]
+// recorded here): accumulates the phase times, the contact count, the worlds
+// given a plain phase and the iterations of both phases.
+int sur_profile(cbz_ctx* c, bool pos) {
   hipEvent_t* ev = c->ev + 8;
-  HIPCHK(c, hipEventRecord(ev[5], c->stream));
-  HIPCHK(c, hipEventSynchronize(ev[5]));
+  HIPCHK(c, hipEventRecord(ev[6], c->stream));
+  HIPCHK(c, hipEventSynchronize(ev[6]));
   int64_t contacts = 0;
   HIPCHK(c, hipMemcpy(&contacts, c->total, sizeof contacts, hipMemcpyDeviceToHost));
   contacts = std::min(contacts, c->cap);
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < 6; ++k) {
     float t = 0;
     HIPCHK(c, hipEventElapsedTime(&t, ev[k], ev[k + 1]));
     c->sur_ms[k] += t;
   }
-  std::vector<uint8_t> m((size_t)c->nw);
+  std::vector<uint8_t> m((size_t)c->nw), pm((size_t)c->nw, 1);
   std::vector<int64_t> it(2 * (size_t)c->nw);
   HIPCHK(c, hipMemcpy(m.data(), c->sur.mask, m.size(), hipMemcpyDeviceToHost));
+  if (pos) HIPCHK(c, hipMemcpy(pm.data(), c->sur.pmask, pm.size(), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(it.data(), c->world_iters, it.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-  int64_t fb = 0, vi = 0;
+  int64_t fb = 0, pfb = 0, vi = 0, pi = 0;
   for (int32_t w = 0; w < c->nw; ++w) {
-    fb += m[w] ? 1 : 0;
+    fb += (m[w] || (pos && pm[w])) ? 1 : 0;
+    pfb += pm[w] ? 1 : 0;
+    pi += it[2 * (size_t)w];
     vi += it[2 * (size_t)w + 1];
   }
   c->sur_cnt[0] += 1;
   c->sur_cnt[1] += contacts;
   c->sur_cnt[2] += fb;
   c->sur_cnt[3] += vi;
+  c->sur_cnt[4] += pfb;
+  c->sur_cnt[5] += pi;
   return CBZ_OK;
 }
 
-// One resolve with the surrogate: the exact position phase; the network's
-// per-contact velocity changes summed per body; then per world the velocity
-// loop polishes from there to velocityEpsilon + tol within polish_mult * C
-// iterations, or the world is restored and given the plain velocity phase.
+// One resolve with the surrogate:
+//   1. prepareContacts (a launch of the resolver without iterations);
+//   2. the network on the prepared contacts: per contact the velocity phase's
+//      changes of its bodies and the penetration the position phase resolves
+//      there, summed per body (with that penetration's moves);
+//   3. the position phase: per world the reference's loop polishes from the
+//      predicted moves to positionEpsilon + pos_tol within pos_polish * C
+//      iterations, or the world is restored and given the plain position
+//      phase (every world plain when the position polish is off);
+//   4. the velocity phase likewise, to velocityEpsilon + tol within
+//      polish_mult * C iterations.
+// tol < 0 gives every world both plain phases; CBZ_SUR_ZERO_PREDICTION
+// polishes from zero predictions (the same method without the network).
+// *nfall_dst counts the worlds that took a plain phase.
 template <typename R>
 int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int64_t polish_mult, int32_t flags,
                 int32_t* nfall_dst) {
   auto& u = c->sur;
-  hipEvent_t* ev = c->ev + 8;  // position phase, features, network, sums, velocity phase
-  if (c->prof) HIPCHK(c, hipEventRecord(ev[0], c->stream));
-  int rc = do_resolve<R>(c, dt, mult, 0, RM_POS);
+  hipEvent_t* ev = c->ev + 8;  // prepare, features, network, sums, position phase, velocity phase
+  const bool pos = tol >= 0 && u.pos_tol >= 0;
+  int rc = sur_ensure_inference(c, c->cap);
   if (rc) return rc;
+  u.last_pos = pos;
+  if (c->prof) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  if ((rc = do_resolve<R>(c, dt, 0, 0, RM_POS))) return rc;
   if (c->prof) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   if (tol < 0 || (flags & CBZ_SUR_ZERO_PREDICTION)) {
-    // every world plain (tol < 0), or polished from a zero prediction
+    // every world plain (tol < 0), or polished from zero predictions
     if (c->prof) {
       HIPCHK(c, hipEventRecord(ev[2], c->stream));
       HIPCHK(c, hipEventRecord(ev[3], c->stream));
     }
     HIPCHK(c, launch_fill_u8(u.mask, c->nw, tol < 0 ? 1 : 0, c->stream));
     HIPCHK(c, launch_fill_f32(u.delta, 6 * c->nb, 0.f, c->stream));
-    if (c->prof) HIPCHK(c, hipEventRecord(ev[4], c->stream));
-    rc = do_resolve<R>(c, dt, mult, 0, RM_VEL | RM_POLISH, u.mask, nullptr, tol, polish_mult);
-    if (rc) return rc;
-    if (nfall_dst) HIPCHK(c, launch_count_u8(u.mask, c->nw, nfall_dst, c->stream));
-    return c->prof ? sur_profile(c) : CBZ_OK;
+    if (pos) {
+      HIPCHK(c, launch_fill_u8(u.pmask, c->nw, 0, c->stream));
+      HIPCHK(c, launch_fill_f32(u.dpos, 6 * c->nb, 0.f, c->stream));
+    }
+  } else {
+    // The feature and network grids cover the contact capacity and read the
+    // contact count on the device: no host round trip, so the step can be
+    // captured in a graph.
+    HIPCHK(c, launch_features<R>(bodies_view<R>(c), contacts_view<R>(c), c->cap, c->total, u.deg, u.norm,
+                                 u.norm + kFeat, u.X, nullptr, c->stream));
+    if (c->prof) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    if ((rc = sur_infer(c, u.X, c->cap, c->total))) return rc;
+    if (c->prof) HIPCHK(c, hipEventRecord(ev[3], c->stream));
+    SurApply<R> a;
+    a.b = bodies_view<R>(c);
+    a.c = contacts_view<R>(c);
+    a.world_cnt = c->world_cnt;
+    a.world_coff = c->world_coff;
+    a.body_off = c->d_body_off;
+    a.overflow = c->overflow;
+    a.Y = u.Yp;
+    a.delta = u.delta;
+    a.mask = u.mask;
+    a.dpos = pos ? u.dpos : nullptr;
+    a.pmask = pos ? u.pmask : nullptr;
+    a.max_contacts = kSurContacts;
+    HIPCHK(c, launch_sur_scatter<R>(a, c->nw, c->stream));
   }
-  // The feature and network grids cover the contact capacity and read the
-  // contact count on the device: no host round trip, so the step can be
-  // captured in a graph.
-  if ((rc = sur_ensure_inference(c, c->cap))) return rc;
-  HIPCHK(c, launch_features<R>(bodies_view<R>(c), contacts_view<R>(c), c->cap, c->total, u.deg, u.norm,
-                               u.norm + kFeat, u.X, nullptr, c->stream));
-  if (c->prof) HIPCHK(c, hipEventRecord(ev[2], c->stream));
-  if ((rc = sur_infer(c, u.X, c->cap, c->total))) return rc;
-  if (c->prof) HIPCHK(c, hipEventRecord(ev[3], c->stream));
-  SurApply<R> a;
-  a.b = bodies_view<R>(c);
-  a.c = contacts_view<R>(c);
-  a.world_cnt = c->world_cnt;
-  a.world_coff = c->world_coff;
-  a.body_off = c->d_body_off;
-  a.overflow = c->overflow;
-  a.Y = u.Yp;
-  a.delta = u.delta;
-  a.mask = u.mask;
-  a.max_contacts = kSurContacts;
-  HIPCHK(c, launch_sur_scatter<R>(a, c->nw, c->stream));
   if (c->prof) HIPCHK(c, hipEventRecord(ev[4], c->stream));
+  rc = pos ? do_resolve<R>(c, dt, mult, 0, RM_POS | RM_PREPARED | RM_POLISH, u.pmask, nullptr, u.pos_tol, u.pos_polish)
+           : do_resolve<R>(c, dt, mult, 0, RM_POS | RM_PREPARED);
+  if (rc) return rc;
+  if (c->prof) HIPCHK(c, hipEventRecord(ev[5], c->stream));
   rc = do_resolve<R>(c, dt, mult, 0, RM_VEL | RM_POLISH, u.mask, nullptr, tol, polish_mult);
   if (rc) return rc;
-  if (nfall_dst) HIPCHK(c, launch_count_u8(u.mask, c->nw, nfall_dst, c->stream));
-  return c->prof ? sur_profile(c) : CBZ_OK;
+  if (nfall_dst) HIPCHK(c, launch_count_u8(u.mask, c->nw, nfall_dst, c->stream, pos ? u.pmask : nullptr));
+  return c->prof ? sur_profile(c, pos) : CBZ_OK;
 }
 
 }  // namespace
@@ -2153,12 +2202,12 @@ int cbz_integrate_timed(cbz_ctx* c, double dt, int32_t n, double* ms_per_launch,
   return CBZ_OK;
 }
 
-int cbz_sur_phase_times(cbz_ctx* c, double* ms5, int64_t* counts4) {
+int cbz_sur_phase_times(cbz_ctx* c, double* ms6, int64_t* counts6) {
   if (!c) return CBZ_ERR_ARG;
-  if (ms5)
-    for (int k = 0; k < 5; ++k) ms5[k] = c->sur_ms[k];
-  if (counts4)
-    for (int k = 0; k < 4; ++k) counts4[k] = c->sur_cnt[k];
+  if (ms6)
+    for (int k = 0; k < 6; ++k) ms6[k] = c->sur_ms[k];
+  if (counts6)
+    for (int k = 0; k < 6; ++k) counts6[k] = c->sur_cnt[k];
   for (double& v : c->sur_ms) v = 0;
   for (int64_t& v : c->sur_cnt) v = 0;
   return CBZ_OK;
@@ -2316,7 +2365,9 @@ int cbz_sur_collect(cbz_ctx* c, double dt, int64_t iter_per_contact, int64_t* ad
   float* X = u.dsX + u.ds_rows * kFeat;
   float* Y = u.dsY + u.ds_rows * kOut;
   const bool f64 = c->precision == CBZ_F64;
-  rc = f64 ? do_resolve<double>(c, dt, iter_per_contact, 0, RM_POS) : do_resolve<float>(c, dt, iter_per_contact, 0, RM_POS);
+  // the features of the prepared contacts (what the surrogate sees), then
+  // the exact position and velocity phases with their labels
+  rc = f64 ? do_resolve<double>(c, dt, 0, 0, RM_POS) : do_resolve<float>(c, dt, 0, 0, RM_POS);
   if (rc) return rc;
   if (f64)
     HIPCHK(c, launch_features<double>(bodies_view<double>(c), contacts_view<double>(c), tot, nullptr, u.deg, nullptr, nullptr,
@@ -2325,6 +2376,9 @@ int cbz_sur_collect(cbz_ctx* c, double dt, int64_t iter_per_contact, int64_t* ad
     HIPCHK(c, launch_features<float>(bodies_view<float>(c), contacts_view<float>(c), tot, nullptr, u.deg, nullptr, nullptr,
                                      nullptr, X, c->stream));
   HIPCHK(c, launch_fill_f32(Y, tot * kOut, 0.f, c->stream));
+  rc = f64 ? do_resolve<double>(c, dt, iter_per_contact, 0, RM_POS | RM_PREPARED | RM_LABEL, nullptr, Y)
+           : do_resolve<float>(c, dt, iter_per_contact, 0, RM_POS | RM_PREPARED | RM_LABEL, nullptr, Y);
+  if (rc) return rc;
   rc = f64 ? do_resolve<double>(c, dt, iter_per_contact, 0, RM_VEL | RM_LABEL, nullptr, Y)
            : do_resolve<float>(c, dt, iter_per_contact, 0, RM_VEL | RM_LABEL, nullptr, Y);
   if (rc) return rc;
@@ -2579,12 +2633,41 @@ int cbz_sur_resolve(cbz_ctx* c, double dt, int64_t iter_per_contact, double tol,
   return CBZ_OK;
 }
 
+int cbz_sur_set_position(cbz_ctx* c, double tol, int64_t polish_per_contact) {
+  if (!c || !(tol == tol) || polish_per_contact < 0) return fail(c, CBZ_ERR_ARG, "sur_set_position: bad arguments");
+  if (tol != c->sur.pos_tol || polish_per_contact != c->sur.pos_polish) {
+    c->sur.pos_tol = tol;
+    c->sur.pos_polish = polish_per_contact;
+    c->sur.generation += 1;  // a captured surrogate step is stale
+  }
+  return CBZ_OK;
+}
+
+int cbz_sur_get_position(cbz_ctx* c, double* tol, int64_t* polish_per_contact) {
+  if (!c) return CBZ_ERR_ARG;
+  if (tol) *tol = c->sur.pos_tol;
+  if (polish_per_contact) *polish_per_contact = c->sur.pos_polish;
+  return CBZ_OK;
+}
+
 int cbz_sur_world_mask(cbz_ctx* c, uint8_t* mask) {
   if (!c || !mask) return CBZ_ERR_ARG;
   if (!c->sur.mask) return fail(c, CBZ_ERR_STATE, "sur_world_mask: no surrogate resolve yet");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(mask, c->sur.mask, c->nw, hipMemcpyDeviceToHost));
+  return CBZ_OK;
+}
+
+int cbz_sur_position_mask(cbz_ctx* c, uint8_t* mask) {
+  if (!c || !mask) return CBZ_ERR_ARG;
+  if (!c->sur.mask) return fail(c, CBZ_ERR_STATE, "sur_position_mask: no surrogate resolve yet");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->sur.last_pos)
+    HIPCHK(c, hipMemcpy(mask, c->sur.pmask, c->nw, hipMemcpyDeviceToHost));
+  else
+    std::fill(mask, mask + c->nw, uint8_t(1));
   return CBZ_OK;
 }
 

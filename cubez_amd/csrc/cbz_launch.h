@@ -12,17 +12,28 @@ namespace cbz {
 // Run together they are ResolveContacts; run apart, the velocity phase can
 // be given to a subset of worlds.  RM_LABEL records, per contact, the sum of
 // the velocity and rotation changes applyVelocityChange gave its two bodies.
-// RM_POLISH (with RM_VEL): worlds whose mask byte is 0 first take the
+// With RM_LABEL the position phase also records, per contact, the sum of
+// the penetrations it resolved there (every selection's depth).
+// RM_POLISH with RM_VEL: worlds whose mask byte is 0 first take the
 // per-body velocity changes in `delta`, then run the velocity loop until
 // every contact is within velocityEpsilon + polish_tol, at most
 // polish_mult * C iterations; a world that does not get there is restored
 // (velocities, rotations, motion, awake flags) and re-solved by the plain
 // velocity phase, and its mask byte set to 1.  Worlds whose mask byte is 1
 // run the plain velocity phase.
+// RM_POLISH with RM_POS: the same for the position phase, from the per-body
+// position and orientation changes in `delta`, to positionEpsilon +
+// polish_tol; the restore covers positions, orientations, the awake flags
+// and the motion, transform and world inverse inertia of the bodies asleep
+// when the phase starts (snapshot in vel0).  A single launch polishes one
+// phase.
+// RM_PREPARED (with RM_POS): prepareContacts already ran, in an earlier
+// launch with RM_POS and an iteration budget of 0; this launch reads the
+// contact internals and prepare-time positions it stored.
 // RM_FRICTIONLESS2: the frictionless impulse of a two-body contact includes
 // its second body (the reference omits it, contact.go:512); only contacts
 // whose friction is 0 (per-contact materials) take that impulse.
-enum : int32_t { RM_POS = 1, RM_VEL = 2, RM_LABEL = 4, RM_POLISH = 8, RM_FRICTIONLESS2 = 16 };
+enum : int32_t { RM_POS = 1, RM_VEL = 2, RM_LABEL = 4, RM_POLISH = 8, RM_FRICTIONLESS2 = 16, RM_PREPARED = 32 };
 
 template <typename R>
 struct NarrowLaunch {
@@ -77,10 +88,10 @@ struct ResolveLaunch {
   int32_t mode = 3;                      // RM_POS | RM_VEL: the whole of ResolveContacts
   uint8_t* world_mask = nullptr;         // velocity phase only where set (RM_POLISH: in/out)
   R* pos0 = nullptr;                     // [3][n_bodies] positions at prepare time
-  float* labels = nullptr;               // [cap][12] velocity-phase changes per contact
-  const float* delta = nullptr;          // RM_POLISH: [6][n_bodies] velocity, rotation changes
-  R* vel0 = nullptr;                     // RM_POLISH: [7][n_bodies] snapshot (vel, rot, motion)
-  R* cvel0 = nullptr;                    // RM_POLISH: [3][cap] contact velocities after prepare
+  float* labels = nullptr;               // [cap][13] velocity-phase changes, resolved penetration per contact
+  const float* delta = nullptr;          // RM_POLISH: [6][n_bodies] velocity, rotation (or position, orientation) changes
+  R* vel0 = nullptr;                     // RM_POLISH: snapshot, [7][n_bodies] (RM_VEL) or [22][n_bodies] (RM_POS)
+  R* cvel0 = nullptr;                    // RM_POLISH with RM_VEL: [3][cap] contact velocities after prepare
   R polish_tol = R(0);
   int64_t polish_mult = 0;
 };

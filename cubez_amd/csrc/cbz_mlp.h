@@ -10,11 +10,16 @@ namespace cbz {
 constexpr int kFeat = 40;      // input features per contact
 constexpr int kFeatPad = 48;   // padded to the MFMA K step (16)
 constexpr int kHidden = 256;   // hidden width
-constexpr int kOut = 12;       // outputs: dv0, dw0, dv1, dw1
+constexpr int kOutVel = 12;    // outputs 0-11: dv0, dw0, dv1, dw1 (the velocity phase's changes)
+constexpr int kOutPos = 12;    // output 12: the penetration the position phase resolves at the contact
+constexpr int kOut = 13;       // outputs per contact
 constexpr int kOutPad = 32;    // padded to the MFMA tile (32)
 constexpr int kLayers = 4;
 constexpr int kSurBodies = 256;     // bodies per world the surrogate takes
 constexpr int kSurContacts = 2048;  // contacts per world the surrogate takes
+// contacts per world the position polish takes (the one-wavefront
+// kernel's widest register class); larger worlds get the plain position phase
+constexpr int kSurPosContacts = 2048;
 
 // layer l maps kIn[l] -> kOutL[l] features (unpadded)
 __host__ __device__ constexpr int layer_in(int l) { return l == 0 ? kFeat : kHidden; }
@@ -68,9 +73,11 @@ struct SurApply {
   const int64_t* world_coff;
   const int32_t* body_off;
   const int32_t* overflow;
-  const float* Y;               // [contacts][kOut] predicted changes
-  float* delta;                 // out: [6][n_bodies] summed change per body
+  const float* Y;               // [contacts][kOut] predictions
+  float* delta;                 // out: [6][n_bodies] summed velocity and rotation change per body
   uint8_t* mask;                // out: 1 = plain velocity phase, 0 = polish
+  float* dpos;                  // out: [6][n_bodies] summed position and orientation change per body
+  uint8_t* pmask;               // out: 1 = plain position phase, 0 = polish (null: no position polish)
   int max_contacts;
 };
 
@@ -89,7 +96,8 @@ hipError_t launch_silu_bwd(const float* dA, const float* Z, int rows, int valid_
                            __bf16* dZt, int ldt, hipStream_t st);
 hipError_t launch_sample_idx(int64_t* idx, int n, uint64_t seed, uint64_t step, int64_t rows, hipStream_t st);
 hipError_t launch_col_stats(const float* A, int64_t rows, int cols, float* mean, float* stdv, hipStream_t st);
-hipError_t launch_count_u8(const uint8_t* m, int n, int32_t* out, hipStream_t st);
+// count of worlds whose byte is set in m or (when not null) in m2
+hipError_t launch_count_u8(const uint8_t* m, int n, int32_t* out, hipStream_t st, const uint8_t* m2 = nullptr);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t st);
 hipError_t launch_record_count(const int32_t* value, int32_t* hist, int32_t* ctr, int64_t cap, hipStream_t st);
 hipError_t launch_mse(const float* Y, const float* T, int rows, int valid_rows, float* row_loss, __bf16* dY,

@@ -21,6 +21,7 @@
 
 #include "cbz_devmath.h"
 #include "cbz_launch.h"
+#include "cbz_mlp.h"
 #include "cbz_types.h"
 #include "cbz_wave.h"
 
@@ -36,11 +37,20 @@ constexpr double kEnergyRel = 1e-5;
 // Compile-time variants of the resolver kernel, so that the plain
 // resolver's hot loops carry no code (and no register pressure) of the
 // others: KIND_PLAIN (ResolveContacts, or one of its phases), KIND_LABEL
-// (also records every impulse's body changes per contact), KIND_POLISH (the
-// surrogate's polish with the plain fallback), KIND_MAT (KIND_PLAIN with
-// per-contact friction and restitution, and the frictionless impulse for
-// contacts whose friction is 0).
-enum : int { KIND_PLAIN = 0, KIND_LABEL = 1, KIND_POLISH = 2, KIND_MAT = 3 };
+// (also records every impulse's body changes and every resolved
+// penetration per contact), KIND_POLISH (the surrogate's velocity polish
+// with the plain fallback), KIND_MAT (KIND_PLAIN with per-contact friction
+// and restitution, and the frictionless impulse for contacts whose
+// friction is 0), KIND_PPOLISH (the surrogate's position polish with the
+// plain fallback).
+enum : int { KIND_PLAIN = 0, KIND_LABEL = 1, KIND_POLISH = 2, KIND_MAT = 3, KIND_PPOLISH = 4 };
+
+// The position polish's separation bound: a polished world is rejected if a
+// contact ends separated by more than kSepFactor times the world's deepest
+// penetration at the start (+ the tolerance).  The plain phase stays within
+// 1.4 times it (every world of 4096 at six points of a 600-step run; 0.1 %
+// of worlds pass 1.0 times).
+constexpr double kSepFactor = 2.0;
 
 template <typename R>
 struct RS {
@@ -68,10 +78,17 @@ struct RS {
   int32_t mode;                // RM_* bits
   uint8_t* world_mask;         // velocity phase only where set (null: every world); RM_POLISH: in/out
   R* pos0;                     // [3][n_bodies] body positions at prepare time
-  float* labels;               // [cap][12] accumulated velocity-phase changes (null: off)
-  const float* delta;          // RM_POLISH: [6][n_bodies] velocity and rotation changes
-  R* vel0;                     // RM_POLISH: [7][n_bodies] vel, rot, motion before them
-  R* cvel0;                    // RM_POLISH: [3][cap] contact velocities as prepare left them
+  // [cap][kOut] per contact: accumulated velocity-phase changes (0-11) and
+  // resolved penetration (12) (null: off)
+  float* labels;
+  // RM_POLISH: [6][n_bodies] velocity and rotation changes (RM_VEL) or
+  // position and orientation changes (RM_POS)
+  const float* delta;
+  // RM_POLISH snapshot: [7][n_bodies] vel, rot, motion before the changes
+  // (RM_VEL); [22][n_bodies] motion, transform, iit_world of the bodies
+  // asleep when the phase starts (RM_POS)
+  R* vel0;
+  R* cvel0;                    // RM_POLISH with RM_VEL: [3][cap] contact velocities as prepare left them
   R polish_tol;
   int64_t polish_mult;
 };
@@ -79,7 +96,7 @@ struct RS {
 
 template <typename R>
 __device__ __forceinline__ void add_label(const RS<R>& a, int64_t g, const V3<R> vc[2], const V3<R> rc[2]) {
-  float* L = a.labels + g * 12;
+  float* L = a.labels + g * kOut;
   const V3<R>* v[4] = {&vc[0], &rc[0], &vc[1], &rc[1]};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -87,6 +104,12 @@ __device__ __forceinline__ void add_label(const RS<R>& a, int64_t g, const V3<R>
     L[3 * k + 1] += float(v[k]->y);
     L[3 * k + 2] += float(v[k]->z);
   }
+}
+
+// The position phase's label: the penetration resolved at the contact.
+template <typename R>
+__device__ __forceinline__ void add_pos_label(const RS<R>& a, int64_t g, R depth) {
+  a.labels[g * kOut + kOutPos] += float(depth);
 }
 
 template <typename R>
@@ -502,12 +525,13 @@ __device__ __forceinline__ void resolve_world(const RS<R>& a, int w, int C, int6
   const int64_t maxIter = a.iter_fixed > 0 ? a.iter_fixed : a.iter_mult * int64_t(C);
 
   const bool do_pos = (a.mode & RM_POS) != 0;
+  const bool do_prep = do_pos && !(a.mode & RM_PREPARED);
   const bool do_vel = (a.mode & RM_VEL) != 0 && (KIND == KIND_POLISH || !a.world_mask || a.world_mask[w]);
-  // worlds this path takes are never polished: plain velocity phase
-  if (KIND == KIND_POLISH && lane == 0) a.world_mask[w] = 1;
+  // worlds this path takes are never polished: the plain phase
+  if ((KIND == KIND_POLISH || KIND == KIND_PPOLISH) && lane == 0) a.world_mask[w] = 1;
   // prepareContacts, contact.go:225-229
   for (int j = lane; j < C; j += 64) {
-    if (do_pos) prepare_contact<R, KIND>(a, base + j);
+    if (do_prep) prepare_contact<R, KIND>(a, base + j);
     if (LDS) {
       key[j] = c.pen[base + j];
       kb0[j] = c.body0[base + j];
@@ -538,6 +562,8 @@ __device__ __forceinline__ void resolve_world(const RS<R>& a, int w, int C, int6
     match_awake(a, kb[0], kb[1], aw[0], aw[1], lane);
     V3<R> lin[2], ang[2];
     apply_position_change(a, kb, aw, n, rel, best, lin, ang, lane);
+    if constexpr (KIND == KIND_LABEL)
+      if (lane == 0) add_pos_label(a, g, best);
     for (int j = lane; j < C; j += 64) {
       const int jb[2] = {kb0[j], kb1[j]};
       bool any = false;
@@ -974,6 +1000,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
   double* s_bs = (double*)(lds + L::bs);  // narrow classes only
   uint8_t* s_flags = (uint8_t*)(lds + L::flags);
   const bool do_pos = (a.mode & RM_POS) != 0;
+  const bool do_prep = do_pos && !(a.mode & RM_PREPARED);
   const bool do_vel = (a.mode & RM_VEL) != 0 && (KIND == KIND_POLISH || !a.world_mask || a.world_mask[w]);
   if (a.world_iters && lane == 0) {
     if (do_pos) a.world_iters[2 * w] = 0;
@@ -994,8 +1021,8 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
     const int g = b0w + i;
     // body positions at prepare time: the current ones when this launch
     // prepares, else the snapshot the preparing launch stored
-    const V3<R> p = ld3(do_pos ? b.pos : a.pos0, S, g);
-    if (do_pos) st3(a.pos0, S, g, p);
+    const V3<R> p = ld3(do_prep ? b.pos : a.pos0, S, g);
+    if (do_prep) st3(a.pos0, S, g, p);
     lds3_store(s_pos0, i, p);
     lds3_store(s_lad, i, mul(ld3(b.last_acc, S, g), a.dt));
     s_flags[i] = b.flags[g];
@@ -1010,7 +1037,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
   // prepareContacts / calculateInternals, contact.go:225-229, :59-85, over
   // this lane's contacts (a rolled loop: the body is large), into the
   // contact arrays
-  if (do_pos) {
+  if (do_prep) {
     for (int j = lane; j < C; j += 64) {
       const int64_t g = base + j;
       int c0 = c.body0[g], c1 = c.body1[g];
@@ -1044,7 +1071,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
       c.ddv[g] = ddv;
       if constexpr (!WIDE) s_bs[j] = sc;
     }
-  } else if constexpr (!WIDE) {  // velocity phase only: the basis scales of prepare's normals
+  } else if constexpr (!WIDE) {  // prepared by an earlier launch: the basis scales of prepare's normals
     for (int j = lane; j < C; j += 64) s_bs[j] = basis_scale(ld3(c.normal, CS, base + j));
   }
   // this lane's contacts into registers (the lane wrote what it reads here)
@@ -1077,14 +1104,135 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
   }
   __syncthreads();
 
+  // The surrogate's position polish (KIND_PPOLISH, worlds whose mask byte
+  // is 0): snapshot what the loop may write to HBM for the bodies
+  // asleep now (the motion matchAwakeState sets, the derived data a
+  // sleeping body's move recomputes), add the predicted changes to the
+  // awake bodies' positions and orientations, update every contact's
+  // penetration for them by the loop's own formula, then run the loop to
+  // positionEpsilon + polish_tol within polish_mult * C iterations.
+  bool ppolish = false;
+  R sep_floor = R(0);  // the polished world's least penetration the acceptance test allows
+  // the world's least penetration (every lane gets it)
+  auto key_min = [&]() {
+    R kmin = R(0);
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s)
+      if (64 * s + lane < C) kmin = key[s] < kmin ? key[s] : kmin;
+    return dpp::allreduce(kmin, R(0), [](R x, R y) { return y < x ? y : x; });
+  };
+  // Undo a rejected position polish: positions, orientations and awake
+  // flags from HBM (untouched until the phase ends), the sleeping bodies'
+  // motion and derived data from the snapshot, the penetrations from the
+  // contact array; then the plain phase runs.
+  auto restore_positions = [&]() {
+    // lane 0's HBM stores of the loop (motion, a sleeping body's derived
+    // data) complete before the other lanes restore over them
+    __syncthreads();
+    for (int i = lane; i < n; i += 64) {
+      const int g = b0w + i;
+      lds3_store(s_st, i, ld3(b.pos, S, g));
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s_st[(3 + k) * KB + i] = b.quat[k * S + g];
+      const uint8_t f = b.flags[g];
+      s_flags[i] = f;
+      if (!(f & F_AWAKE)) {
+        b.motion[g] = a.vel0[g];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) b.transform[k * S + g] = a.vel0[(1 + k) * S + g];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) b.iit_world[k * S + g] = a.vel0[(13 + k) * S + g];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < SLOTS; ++s) {
+      const int j = 64 * s + lane;
+      key[s] = j < C ? c.pen[base + j] : R(0);
+    }
+    __syncthreads();
+    if (lane == 0) a.world_mask[w] = 1;
+  };
+  if constexpr (KIND == KIND_PPOLISH) {
+    ppolish = do_pos && a.world_mask[w] == 0;
+    if (ppolish) {
+      R pmax = R(0);
+#pragma unroll
+      for (int s = 0; s < SLOTS; ++s) pmax = key[s] > pmax ? key[s] : pmax;
+      pmax = dpp::allreduce(pmax, R(0), [](R x, R y) { return y > x ? y : x; });
+      sep_floor = -(R(kSepFactor) * pmax + a.polish_tol);
+      const float* dl = a.delta;
+      for (int i = lane; i < n; i += 64) {
+        const int g = b0w + i;
+        if (!(s_flags[i] & F_AWAKE)) {
+          a.vel0[g] = b.motion[g];
+#pragma unroll
+          for (int k = 0; k < 12; ++k) a.vel0[(1 + k) * S + g] = b.transform[k * S + g];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) a.vel0[(13 + k) * S + g] = b.iit_world[k * S + g];
+          continue;
+        }
+        const V3<R> dx{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
+        const V3<R> dq{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
+        if (dx.x == R(0) && dx.y == R(0) && dx.z == R(0) && dq.x == R(0) && dq.y == R(0) && dq.z == R(0)) continue;
+        lds3_store(s_st, i, add(lds3(s_st, i), dx));
+        Q4<R> q{s_st[3 * KB + i], s_st[4 * KB + i], s_st[5 * KB + i], s_st[6 * KB + i]};
+        q = q_normalize(q_add_scaled_vector(q, dq, R(1)));
+        s_st[3 * KB + i] = q.w;
+        s_st[4 * KB + i] = q.x;
+        s_st[5 * KB + i] = q.y;
+        s_st[6 * KB + i] = q.z;
+      }
+#pragma unroll
+      for (int s = 0; s < SLOTS; ++s) {
+        asm volatile("" ::: "memory");
+        if (64 * s + lane >= C) continue;
+        const int64_t gj = base + 64 * s + lane;
+        const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+        V3<R> nj;
+        if constexpr (WIDE)
+          nj = ld3(c.normal, CS, gj);
+        else
+          nj = nrm[s];
+        R pen = key[s];
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+          if (jb[bb] < 0) continue;
+          const int g = b0w + jb[bb];
+          const V3<R> dx{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
+          const V3<R> dq{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
+          if (dx.x == R(0) && dx.y == R(0) && dx.z == R(0) && dq.x == R(0) && dq.y == R(0) && dq.z == R(0)) continue;
+          V3<R> rj;
+          if constexpr (WIDE)
+            rj = ld3(c.rel + 3 * bb * CS, CS, gj);
+          else
+            rj = sub(ptr[s], lds3(s_pos0, jb[bb]));
+          const V3<R> dp = add(cross(dq, rj), dx);
+          R sign = R(1);
+          if (bb == 0) sign = R(-1);
+          pen += dot(dp, nj) * sign;
+        }
+        key[s] = pen;
+      }
+      __syncthreads();
+      // a prediction that already separates a contact beyond the bound is
+      // rejected before the loop spends iterations on it
+      if (!(key_min() >= sep_floor)) {
+        restore_positions();
+        ppolish = false;
+      }
+    }
+  }
+
   // adjustPositions, contact.go:233-283
+  R peps = ppolish ? R(kPositionEpsilon) + a.polish_tol : R(kPositionEpsilon);
+  int64_t pbudget = ppolish ? a.polish_mult * int64_t(C) : maxIter;
   int64_t used = 0;
   // A slot past the world's contacts keeps key 0 (never above the epsilon)
   // and ids 0xFFFFFFFF (never a body of the world), so the loops below need
   // no bounds test.
-  while (do_pos && used < maxIter) {
+  while (do_pos) {
     opaque_ids(ids);
-    R best = R(kPositionEpsilon);
+    R best = peps;
     int bi = INT_MAX;
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
@@ -1094,7 +1242,32 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
       }
     }
     wave_argmax_dpp(best, bi);
-    if (bi == INT_MAX) break;
+    bool reject = false;  // the position polish did not give an acceptable result
+    if (bi == INT_MAX) {
+      if constexpr (KIND == KIND_PPOLISH) {
+        // Every contact is within positionEpsilon + tol.  The plain phase
+        // moves bodies only to resolve penetration; a polished world in
+        // which a contact ends separated by more than kSepFactor times the
+        // world's deepest penetration at the start (+ tol) is rejected: that
+        // is the mark of a prediction that pushed bodies apart.
+        if (ppolish) reject = !(key_min() >= sep_floor);
+      }
+      if (!reject) break;
+    } else if (used >= pbudget) {
+      if (KIND != KIND_PPOLISH || !ppolish) break;
+      reject = true;
+    }
+    if constexpr (KIND == KIND_PPOLISH) {
+      if (reject) {
+        restore_positions();
+        ppolish = false;
+        peps = R(kPositionEpsilon);
+        pbudget = maxIter;
+        used = 0;
+        continue;
+      }
+    }
+    if (bi == INT_MAX || used >= pbudget) break;
     const int ss = bi >> 6, sl = bi & 63;
     const uint32_t kid = lane_value(pick(ids, ss), sl);
     const int kl[2] = {id_lo(kid), id_hi(kid)};
@@ -1125,6 +1298,8 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
     }
     V3<R> lin[2], ang[2];
     const bool rederived = apply_position_change_lds(a, kb, kl, aw, n, rel, best, iw, im, lin, ang, s_st, lane);
+    if constexpr (KIND == KIND_LABEL)
+      if (lane == 0) add_pos_label(a, base + bi, best);
     const int ln = opaque_lane(lane);
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s) {
@@ -1693,11 +1868,14 @@ __device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w
   auto res_load = [&](int k) { return V3<R>{s_res[3 * k], s_res[3 * k + 1], s_res[3 * k + 2]}; };
   const int lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const bool do_pos = (a.mode & RM_POS) != 0;
+  const bool do_prep = do_pos && !(a.mode & RM_PREPARED);
   const bool do_vel = (a.mode & RM_VEL) != 0 && (KIND == KIND_POLISH || !a.world_mask || a.world_mask[w]);
   if (a.world_iters && t == 0) {
     if (do_pos) a.world_iters[2 * w] = 0;
     if (a.mode & RM_VEL) a.world_iters[2 * w + 1] = 0;
   }
+  // this kernel does not polish positions: the plain position phase
+  if (KIND == KIND_PPOLISH && t == 0) a.world_mask[w] = 1;
   // ResolveContacts' guard, contact.go:210-212
   if (a.overflow[0] != 0 || C <= 0 || a.dt <= R(0)) return;
   if (!do_pos && !do_vel) return;
@@ -1711,8 +1889,8 @@ __device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w
 
   for (int i = t; i < n; i += T) {
     const int g = b0w + i;
-    const V3<R> p = ld3(do_pos ? b.pos : a.pos0, S, g);
-    if (do_pos) st3(a.pos0, S, g, p);
+    const V3<R> p = ld3(do_prep ? b.pos : a.pos0, S, g);
+    if (do_prep) st3(a.pos0, S, g, p);
     lds3_store(s_pos0, i, p);
     lds3_store(s_lad, i, mul(ld3(b.last_acc, S, g), a.dt));
     s_flags[i] = b.flags[g];
@@ -1724,7 +1902,7 @@ __device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w
 
   // prepareContacts / calculateInternals, contact.go:225-229, :59-85, as in
   // resolve_reg
-  if (do_pos) {
+  if (do_prep) {
     for (int j = t; j < C; j += T) {
       const int64_t g = base + j;
       int c0 = c.body0[g], c1 = c.body1[g];
@@ -1877,6 +2055,7 @@ __device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w
           res_store(1, lin[1]);
           res_store(2, ang[0]);
           res_store(3, ang[1]);
+          if constexpr (KIND == KIND_LABEL) add_pos_label(a, base + gi, gk);
           s_resi[1] = int32_t(kid);
           s_resi[0] = MW_APPLY;
         }
@@ -2394,10 +2573,12 @@ CBZ_RESOLVE_KIND(extern, double, KIND_PLAIN)
 CBZ_RESOLVE_KIND(extern, double, KIND_LABEL)
 CBZ_RESOLVE_KIND(extern, double, KIND_POLISH)
 CBZ_RESOLVE_KIND(extern, double, KIND_MAT)
+CBZ_RESOLVE_KIND(extern, double, KIND_PPOLISH)
 CBZ_RESOLVE_KIND(extern, float, KIND_PLAIN)
 CBZ_RESOLVE_KIND(extern, float, KIND_LABEL)
 CBZ_RESOLVE_KIND(extern, float, KIND_POLISH)
 CBZ_RESOLVE_KIND(extern, float, KIND_MAT)
+CBZ_RESOLVE_KIND(extern, float, KIND_PPOLISH)
 #endif
 
 }  // namespace cbz

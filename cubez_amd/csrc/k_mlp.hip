@@ -166,10 +166,10 @@ __global__ void k_col_stats(const float* A, int64_t rows, int cols, float* mean,
   }
 }
 
-__global__ void k_count_u8(const uint8_t* m, int n, int32_t* out) {
+__global__ void k_count_u8(const uint8_t* m, const uint8_t* m2, int n, int32_t* out) {
   __shared__ int part[256];
   int s = 0;
-  for (int i = threadIdx.x; i < n; i += 256) s += m[i] ? 1 : 0;
+  for (int i = threadIdx.x; i < n; i += 256) s += (m[i] || (m2 && m2[i])) ? 1 : 0;
   part[threadIdx.x] = s;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -323,7 +323,7 @@ __global__ void k_pack_weights(const float* W, int N, int K, __bf16* Wnk, int kp
 // ---------------------------------------------------------------------------
 // One workgroup (4 waves) per block of kFuseRows rows runs all four layers:
 // the block's activations stay in LDS (one 128 x 256 BF16 tile, updated in
-// place), so HBM sees only the BF16 features in and the 12 outputs out; the
+// place), so HBM sees only the BF16 features in and the 13 outputs out; the
 // weights stream from the L2, once per 128 rows.  Each wave computes a
 // 128-row x 64-column slab of a hidden layer as 4 x 2 MFMA 32x32x16 tiles
 // (eight independent accumulators).  The k order of every dot product is
@@ -412,8 +412,8 @@ __device__ __forceinline__ void mlp_block(const MlpArgs& f, int64_t M, int64_t m
     store_hidden(f.bias[l]);
     __syncthreads();
   }
-  // layer 3: 12 outputs (one 32-feature tile); wave t takes row tile t; lane
-  // (r, h) holds features 8q + 4h .. +3 of row 32*wave + r
+  // layer 3: kOut outputs (one 32-feature tile); wave t takes row tile t;
+  // lane (r, h) holds features 8q + 4h .. +3 of row 32*wave + r
   if (wave < kRowTiles) {
     f32x16 y = {};
     const __bf16* W = f.W[kLayers - 1];
@@ -425,18 +425,15 @@ __device__ __forceinline__ void mlp_block(const MlpArgs& f, int64_t M, int64_t m
     }
     const int64_t row = m0 + 32 * wave + r;
     if (row < M) {
+      // rows of kOut (odd) floats: scalar stores
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int col = 8 * q + 4 * h;
-        if (col + 4 <= kOut) {
-          float4 o;
-          o.x = (y[4 * q + 0] + f.bias[kLayers - 1][col + 0]) * f.y_scale[col + 0] + f.y_shift[col + 0];
-          o.y = (y[4 * q + 1] + f.bias[kLayers - 1][col + 1]) * f.y_scale[col + 1] + f.y_shift[col + 1];
-          o.z = (y[4 * q + 2] + f.bias[kLayers - 1][col + 2]) * f.y_scale[col + 2] + f.y_shift[col + 2];
-          o.w = (y[4 * q + 3] + f.bias[kLayers - 1][col + 3]) * f.y_scale[col + 3] + f.y_shift[col + 3];
-          *(float4*)(f.Y + row * kOut + col) = o;
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int col = 8 * q + 4 * h + j;
+          if (col < kOut)
+            f.Y[row * kOut + col] = (y[4 * q + j] + f.bias[kLayers - 1][col]) * f.y_scale[col] + f.y_shift[col];
         }
-      }
     }
   }
 }
@@ -461,36 +458,91 @@ hipError_t launch_mlp_forward(const MlpArgs& f, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // per-body sums of the predictions
 // ---------------------------------------------------------------------------
-// One workgroup per world.  Each contact's predicted changes (dv0, dw0 for
-// its first body, dv1, dw1 for its second) are added to its bodies in 32.32
-// fixed point with LDS integer atomics: integer addition is associative, so
-// the sums do not depend on the order the atomics land in and a replay gives
-// the same bits.  Sleeping bodies get no change (the exact loop wakes them
-// when a contact needs it).  A world with a non-finite or implausibly large
-// prediction, or too large for the surrogate, is marked for the plain
-// velocity phase (mask 1); the others are marked for polishing (mask 0).
-constexpr float kMaxPredicted = 1.0e4f;  // |change| in m/s or rad/s
+// One workgroup per world.  Each contact's predicted velocity changes (dv0,
+// dw0 for its first body, dv1, dw1 for its second) are added to its bodies
+// in 32.32 fixed point with LDS integer atomics: integer addition is
+// associative, so the sums do not depend on the order the atomics land in
+// and a replay gives the same bits.  The predicted position output is the
+// penetration the position phase resolves at the contact over all its
+// selections; the contact's move for that amount is applyPositionChange's
+// (contact.go:286-386: the inertia-weighted split between the bodies'
+// linear and angular moves, the angular limit of 0.2 |r x n|-projection),
+// summed per body the same way.  Sleeping bodies get no change (the exact
+// loop wakes them when a contact needs it).  A world with a non-finite or
+// implausibly large prediction, or too large for the surrogate, is marked for
+// the plain phases (mask bytes 1); the others are marked for polishing (0).
+constexpr float kMaxPredicted = 1.0e4f;  // |change| in m/s or rad/s, or the resolved penetration in m
 constexpr double kFix = 4294967296.0;     // 2^32
+
+// applyPositionChange's moves of a contact's two bodies for `amount` of
+// penetration resolved (contact.go:286-386), with the bodies' world inverse
+// inertia iw and inverse masses im (zeros for an absent second body).
+template <typename R>
+__device__ __forceinline__ void position_move(const V3<R>& n, const V3<R> rel[2], bool has1, const R (&iw)[2][9],
+                                              const R im[2], R amount, V3<R> lin[2], V3<R> ang[2]) {
+  R angI[2] = {R(0), R(0)}, linI[2] = {R(0), R(0)};
+  R total = R(0);
+  for (int i = 0; i < 2; ++i) {
+    lin[i] = ang[i] = V3<R>{R(0), R(0), R(0)};
+    if (i == 1 && !has1) continue;
+    V3<R> aiw = cross(rel[i], n);
+    aiw = m3v(iw[i], aiw);
+    aiw = cross(aiw, rel[i]);
+    angI[i] = dot(aiw, n);
+    linI[i] = im[i];
+    total += linI[i] + angI[i];
+  }
+  if (!(total > R(0))) return;  // two bodies of infinite mass: nothing moves
+  for (int i = 0; i < 2; ++i) {
+    if (i == 1 && !has1) continue;
+    const R sign = i == 0 ? R(1) : R(-1);
+    R angM = sign * amount * (angI[i] / total);
+    R linM = sign * amount * (linI[i] / total);
+    const V3<R> projection = add_scaled(rel[i], n, -dot(rel[i], n));
+    const R maxMagnitude = R(0.2) * mag(projection);
+    if (angM < -maxMagnitude) {
+      const R totalMove = angM + linM;
+      angM = -maxMagnitude;
+      linM = totalMove - angM;
+    } else if (angM > maxMagnitude) {
+      const R totalMove = angM + linM;
+      angM = maxMagnitude;
+      linM = totalMove - angM;
+    }
+    if (angM != R(0)) ang[i] = mul(m3v(iw[i], cross(rel[i], n)), angM / angI[i]);
+    lin[i] = mul(n, linM);
+  }
+}
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_sur_scatter(SurApply<R> a) {
-  __shared__ unsigned long long acc[6][kSurBodies];
+  constexpr int NA = 12;  // accumulators per body: velocity 3, rotation 3, position 3, orientation 3
+  __shared__ unsigned long long acc[NA][kSurBodies];
   __shared__ int bad;
   const int w = blockIdx.x, t = threadIdx.x;
   const int C = a.world_cnt[w];
   const int b0w = a.body_off[w], n = a.body_off[w + 1] - b0w;
+  const bool pos = a.pmask != nullptr;
   if (a.overflow[0] != 0 || C <= 0) {  // nothing to resolve (the resolver's guard)
-    if (t == 0) a.mask[w] = 0;
+    if (t == 0) {
+      a.mask[w] = 0;
+      if (pos) a.pmask[w] = 0;
+    }
     return;
   }
   if (n > kSurBodies || C > a.max_contacts) {
-    if (t == 0) a.mask[w] = 1;
+    if (t == 0) {
+      a.mask[w] = 1;
+      if (pos) a.pmask[w] = 1;
+    }
     return;
   }
-  for (int i = t; i < 6 * kSurBodies; i += 256) acc[i / kSurBodies][i % kSurBodies] = 0ull;
+  const int na = pos ? NA : 6;
+  for (int i = t; i < na * kSurBodies; i += 256) acc[i / kSurBodies][i % kSurBodies] = 0ull;
   if (t == 0) bad = 0;
   __syncthreads();
   const int64_t base = a.world_coff[w];
+  const int64_t S = a.b.stride, CS = a.c.cap;
   const uint8_t* fl = a.b.flags;
   for (int j = t; j < C; j += 256) {
     const int64_t g = base + j;
@@ -502,18 +554,44 @@ __global__ void __launch_bounds__(256) k_sur_scatter(SurApply<R> a) {
       bad = 1;
       continue;
     }
+    bool aw[2];
     for (int bb = 0; bb < 2; ++bb) {
-      if (gb[bb] < 0 || !(fl[gb[bb]] & F_AWAKE)) continue;
+      aw[bb] = gb[bb] >= 0 && (fl[gb[bb]] & F_AWAKE);
+      if (!aw[bb]) continue;
       const int l = gb[bb] - b0w;
       for (int k = 0; k < 6; ++k)
         atomicAdd(&acc[k][l], (unsigned long long)__double2ll_rn(double(y[6 * bb + k]) * kFix));
     }
+    const R amount = R(y[kOutPos]);
+    if (!pos || !(amount > R(0)) || !(aw[0] || aw[1])) continue;
+    const bool has1 = gb[1] >= 0;
+    R iw[2][9] = {}, im[2] = {R(0), R(0)};
+    for (int bb = 0; bb < (has1 ? 2 : 1); ++bb) {
+      for (int k = 0; k < 9; ++k) iw[bb][k] = a.b.iit_world[k * S + gb[bb]];
+      im[bb] = a.b.inv_mass[gb[bb]];
+    }
+    const V3<R> nrm = ld3c(a.c.normal, CS, g);
+    const V3<R> rel[2] = {ld3c(a.c.rel, CS, g), ld3c(a.c.rel + 3 * CS, CS, g)};
+    V3<R> lin[2], ang[2];
+    position_move(nrm, rel, has1, iw, im, amount, lin, ang);
+    for (int bb = 0; bb < 2; ++bb) {
+      if (!aw[bb]) continue;
+      const int l = gb[bb] - b0w;
+      const R v[6] = {lin[bb].x, lin[bb].y, lin[bb].z, ang[bb].x, ang[bb].y, ang[bb].z};
+      for (int k = 0; k < 6; ++k)
+        atomicAdd(&acc[6 + k][l], (unsigned long long)__double2ll_rn(double(v[k]) * kFix));
+    }
   }
   __syncthreads();
-  const int64_t S = a.b.stride;
-  for (int i = t; i < n; i += 256)
+  for (int i = t; i < n; i += 256) {
     for (int k = 0; k < 6; ++k) a.delta[k * S + b0w + i] = float(double((long long)acc[k][i]) / kFix);
-  if (t == 0) a.mask[w] = bad ? 1 : 0;
+    if (pos)
+      for (int k = 0; k < 6; ++k) a.dpos[k * S + b0w + i] = float(double((long long)acc[6 + k][i]) / kFix);
+  }
+  if (t == 0) {
+    a.mask[w] = bad ? 1 : 0;
+    if (pos) a.pmask[w] = (bad || C > kSurPosContacts) ? 1 : 0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -569,8 +647,8 @@ hipError_t launch_col_stats(const float* A, int64_t rows, int cols, float* mean,
   k_col_stats<<<cols, 256, 0, st>>>(A, rows, cols, mean, stdv);
   return hipGetLastError();
 }
-hipError_t launch_count_u8(const uint8_t* m, int n, int32_t* out, hipStream_t st) {
-  k_count_u8<<<1, 256, 0, st>>>(m, n, out);
+hipError_t launch_count_u8(const uint8_t* m, int n, int32_t* out, hipStream_t st, const uint8_t* m2) {
+  k_count_u8<<<1, 256, 0, st>>>(m, m2, n, out);
   return hipGetLastError();
 }
 hipError_t launch_record_count(const int32_t* value, int32_t* hist, int32_t* ctr, int64_t cap, hipStream_t st) {

@@ -151,13 +151,20 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   a.cvel0 = L.cvel0;
   a.polish_tol = L.polish_tol;
   a.polish_mult = L.polish_mult;
-  if ((a.mode & RM_POLISH) && (!a.world_mask || !a.delta || !a.vel0 || !a.cvel0)) return hipErrorInvalidValue;
+  if (a.mode & RM_POLISH) {
+    // one phase per polish launch; the velocity polish also needs the
+    // contact-velocity snapshot
+    const bool pos = (a.mode & RM_POS) != 0, vel = (a.mode & RM_VEL) != 0;
+    if (pos == vel || !a.world_mask || !a.delta || !a.vel0 || (vel && !a.cvel0)) return hipErrorInvalidValue;
+  }
+  if ((a.mode & RM_PREPARED) && !(a.mode & RM_POS)) return hipErrorInvalidValue;
   if ((a.mode & RM_LABEL) && !a.labels) return hipErrorInvalidValue;
   // per-contact materials: the plain resolver only
   if (a.c.fric && (!a.c.rest || (a.mode & (RM_POLISH | RM_LABEL)))) return hipErrorInvalidValue;
   if (a.n_worlds <= 0) return hipSuccess;
   if (a.c.fric) return launch_both<R, KIND_MAT>(a, L, st);
-  if (a.mode & RM_POLISH) return launch_both<R, KIND_POLISH>(a, L, st);
+  if (a.mode & RM_POLISH)
+    return (a.mode & RM_POS) ? launch_both<R, KIND_PPOLISH>(a, L, st) : launch_both<R, KIND_POLISH>(a, L, st);
   if (a.mode & RM_LABEL) return launch_both<R, KIND_LABEL>(a, L, st);
   return launch_both<R, KIND_PLAIN>(a, L, st);
 }

@@ -73,6 +73,8 @@ def _run(sc, mode, model, steps, tol, polish):
             ms, cnt = w.sur_phase_times()
             out["fallback_rate"] = float(fb.sum()) / (steps * sc.n_worlds)
             out["velocity_iterations_per_step"] = float(cnt[3]) / steps
+            out["position_iterations_per_step"] = float(cnt[5]) / steps
+            out["position_fallback_rate"] = float(cnt[4]) / (steps * sc.n_worlds)
             out["note"] = "timed with per-step profiling events (the surrogate step waits once per step anyway)"
         pos = w.get("pos").copy()
         out["penetration_after"] = _penetration(w, sc.n_worlds)

@@ -271,7 +271,8 @@ class World:
         return out
 
     # -- learned resolver surrogate ---------------------------------------------
-    N_FEAT, N_OUT = 40, 12
+    N_FEAT, N_OUT = 40, 13  # outputs: 12 velocity-phase changes, 1 resolved penetration
+    N_NORM = 2 * N_FEAT + 2 * N_OUT
 
     def selftest_sort(self, keys, vals=None):
         """The broadphase's device radix sort on host arrays
@@ -301,11 +302,11 @@ class World:
 
     def sur_set_norm(self, n):
         n = np.ascontiguousarray(n, np.float32)
-        assert n.size == 104
+        assert n.size == self.N_NORM
         self._chk(self._lib.cbz_sur_set_norm(self._h, _ptr(n)))
 
     def sur_get_norm(self):
-        n = np.empty(104, np.float32)
+        n = np.empty(self.N_NORM, np.float32)
         self._chk(self._lib.cbz_sur_get_norm(self._h, _ptr(n)))
         return n
 
@@ -359,10 +360,12 @@ class World:
     SUR_ZERO_PREDICTION = 1
 
     def sur_resolve(self, dt, iter_per_contact=4, tol=0.06, polish=4, zero_prediction=False):
-        """ResolveContacts with the surrogate: exact positions, predicted
-        velocity changes polished by the worst-first loop to
-        velocityEpsilon + tol within polish * C iterations, else the plain
-        velocity phase.  Returns the number of worlds that fell back."""
+        """ResolveContacts with the surrogate: per phase, the predicted
+        changes polished by the worst-first loop (positions to
+        positionEpsilon + the position tolerance, see sur_set_position;
+        velocities to velocityEpsilon + tol within polish * C iterations),
+        else that plain phase.  Returns the number of worlds that fell back
+        in either phase."""
         n = ctypes.c_int64(0)
         flags = self.SUR_ZERO_PREDICTION if zero_prediction else 0
         self._chk(self._lib.cbz_sur_resolve(self._h, float(dt), int(iter_per_contact), float(tol), int(polish), flags,
@@ -370,18 +373,40 @@ class World:
         return n.value
 
     def sur_phase_times(self):
-        """Accumulated ms [position phase, features, network, per-body sums,
-        velocity phase] and counts [resolves, contacts, fallbacks, velocity
-        iterations] of profiled surrogate resolves since the last call."""
-        ms = np.zeros(5)
-        cnt = np.zeros(4, np.int64)
+        """Accumulated ms [prepare, features, network, per-body sums,
+        position phase, velocity phase] and counts [resolves, contacts,
+        fallbacks (either phase), velocity iterations, position fallbacks,
+        position iterations] of profiled surrogate resolves since the last
+        call."""
+        ms = np.zeros(6)
+        cnt = np.zeros(6, np.int64)
         self._chk(self._lib.cbz_sur_phase_times(self._h, _ptr(ms), _ptr(cnt)))
         return ms, cnt
 
     def sur_world_mask(self):
+        """Per world, 1 where the last surrogate resolve's velocity phase was
+        the plain one."""
         m = np.empty(self.n_worlds, np.uint8)
         self._chk(self._lib.cbz_sur_world_mask(self._h, _ptr(m)))
         return m
+
+    def sur_position_mask(self):
+        """Per world, 1 where the last surrogate resolve's position phase was
+        the plain one."""
+        m = np.empty(self.n_worlds, np.uint8)
+        self._chk(self._lib.cbz_sur_position_mask(self._h, _ptr(m)))
+        return m
+
+    def sur_set_position(self, tol=0.0, polish=4):
+        """The position phase's polish: tolerance in m above positionEpsilon
+        (negative: the plain position phase everywhere) and budget per
+        contact."""
+        self._chk(self._lib.cbz_sur_set_position(self._h, float(tol), int(polish)))
+
+    def sur_position(self):
+        t, p = ctypes.c_double(0), ctypes.c_int64(0)
+        self._chk(self._lib.cbz_sur_get_position(self._h, ctypes.byref(t), ctypes.byref(p)))
+        return t.value, p.value
 
     def step_sur(self, dt, n_steps=1, iter_per_contact=4, tol=0.06, polish=4, zero_prediction=False,
                  counts=False):

@@ -304,34 +304,39 @@ int cbz_selftest_sort(cbz_ctx *ctx, int64_t n, const uint64_t *keys, const int32
                       int32_t *vals_out);
 
 /* --- learned resolver surrogate --------------------------------------------
- * A 40 -> 256 -> 256 -> 256 -> 12 MLP (SiLU, BF16 weights and activations,
+ * A 40 -> 256 -> 256 -> 256 -> 13 MLP (SiLU, BF16 weights and activations,
  * FP32 accumulation, on the matrix cores) predicts, per contact, the
  * velocity and rotation changes the velocity phase of ResolveContacts
- * (adjustVelocities, contact.go:390-445) gives the contact's two bodies.
- * No counterpart in the reference: it is the learned part of this
- * implementation.  The exact method stays the judge: per world, after the
- * exact position phase the summed predictions are applied only if every
- * contact's remaining desired velocity change is at most
- * velocityEpsilon (0.01, contact.go:12) + tol; otherwise that world runs the
- * exact velocity phase from its unchanged state, i.e. gets exactly the
- * plain ResolveContacts result.
+ * (adjustVelocities, contact.go:390-445) gives the contact's two bodies,
+ * and the penetration the position phase (adjustPositions,
+ * contact.go:233-283) resolves at the contact, whose moves are
+ * applyPositionChange's (contact.go:286-386).  No counterpart in the
+ * reference: it is the learned part of this implementation.  The exact
+ * method stays the judge: per world and phase, the reference's loop runs
+ * from the predicted state until every contact is within the phase's
+ * epsilon (positionEpsilon, velocityEpsilon: 0.01, contact.go:12-13) plus a
+ * tolerance; a world that does not get there is restored and runs the exact
+ * phase from its unchanged state.
  *
- * Features (40, per contact): normal 3, relativeContactPosition[0] 3, [1] 3,
- * body0 - body1 velocity 3 and rotation 3, inverse masses 2, world inverse
- * inertias 9 + 9, penetration after the position phase, desiredDeltaVelocity,
- * contactVelocity.x, 1/(contacts of body0), 1/(contacts of body1).
- * Outputs (12): dv0 3, dw0 3, dv1 3, dw1 3.
+ * Features (40, per contact, after prepareContacts): normal 3,
+ * relativeContactPosition[0] 3, [1] 3, body0 - body1 velocity 3 and
+ * rotation 3, inverse masses 2, world inverse inertias 9 + 9, penetration,
+ * desiredDeltaVelocity, contactVelocity.x, 1/(contacts of body0),
+ * 1/(contacts of body1).
+ * Outputs (13): dv0 3, dw0 3, dv1 3, dw1 3, resolved penetration 1.
  * Parameters: per layer W [out x in] row-major then b [out], FP32.
- * Normalisation: in_mean 40, in_std 40, out_mean 12, out_std 12. */
+ * Normalisation (106 floats): in_mean 40, in_std 40, out_mean 13,
+ * out_std 13. */
 int64_t cbz_sur_num_params(void);
 int cbz_sur_set_params(cbz_ctx *ctx, const float *params);
 int cbz_sur_get_params(cbz_ctx *ctx, float *params);
-int cbz_sur_set_norm(cbz_ctx *ctx, const float *norm104);
-int cbz_sur_get_norm(cbz_ctx *ctx, float *norm104);
+int cbz_sur_set_norm(cbz_ctx *ctx, const float *norm106);
+int cbz_sur_get_norm(cbz_ctx *ctx, float *norm106);
 /* Training data: cbz_sur_collect resolves the current contacts exactly (the
  * same result as cbz_resolve) and appends every contact's features and
- * labels (the summed changes the exact velocity phase gave its bodies) to a
- * device dataset of cbz_sur_reserve rows. */
+ * labels (the summed changes the exact velocity phase gave its bodies, the
+ * penetration the exact position phase resolved at it) to a device dataset
+ * of cbz_sur_reserve rows. */
 int cbz_sur_reserve(cbz_ctx *ctx, int64_t rows);
 int cbz_sur_collect(cbz_ctx *ctx, double dt, int64_t iter_per_contact, int64_t *rows_added);
 int cbz_sur_dataset(cbz_ctx *ctx, int64_t *rows, float *features, float *labels);
@@ -353,7 +358,7 @@ int cbz_sur_predict(cbz_ctx *ctx, int32_t rows, const float *features, float *ou
  * nothing.  No counterpart in the reference: it is for callers that cannot
  * allocate device memory -- code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc.  features: device pointer to
- * [rows][40] raw FP32 features; out: device pointer to [rows][12] FP32
+ * [rows][40] raw FP32 features; out: device pointer to [rows][13] FP32
  * predictions (denormalised), both caller-owned.  rows <= 65536
  * (CBZ_ERR_ARG beyond).  The normalised BF16 rows go to a fixed 6 MB
  * device array the library holds per GPU for the process's lifetime:
@@ -368,28 +373,48 @@ int cbz_sur_predict_device(cbz_ctx *ctx, int32_t rows, const float *features, fl
  * operands; returns CBZ_OK, or CBZ_ERR_STATE with the first GEMM whose reads
  * or writes would leave a buffer described in msg[msg_len]. */
 int cbz_sur_check_gemms(int32_t rows, char *msg, int32_t msg_len);
-/* ResolveContacts with the surrogate.  The position phase is the exact
- * one.  The network's per-contact velocity changes are summed per body and
- * applied; then, per world, the reference's worst-first velocity loop runs
- * from that state until every contact's desired velocity change is within
- * velocityEpsilon + tol (tol in m/s), for at most polish_per_contact * C
- * iterations.  A world that does not get there is restored and re-solved by
- * the plain velocity phase (a fallback).  tol < 0 sends every world to the
- * plain phase; flags CBZ_SUR_ZERO_PREDICTION polishes from a zero prediction
- * (the same method without the network).  *n_fallback receives the number
- * of worlds that fell back, cbz_sur_world_mask which.
+/* ResolveContacts with the surrogate.  prepareContacts runs, then the
+ * network on the prepared contacts; its per-contact predictions are summed
+ * per body.
+ * Position phase: per world, the predicted moves are applied and the
+ * reference's worst-first position loop runs from there until every
+ * contact's penetration is within positionEpsilon + the position tolerance
+ * (cbz_sur_set_position), for at most its budget per contact * C
+ * iterations; the polished world is accepted only if, besides, no contact
+ * ends separated by more than the world's deepest penetration at the start
+ * plus that tolerance.  Worlds of more than 1536 contacts, and every world
+ * when the position tolerance is negative, take the plain position phase.
+ * Velocity phase: likewise from the predicted changes, to velocityEpsilon +
+ * tol (tol in m/s) within polish_per_contact * C iterations, accepted only
+ * if the world's kinetic energy did not grow (beyond 1/2 tol^2 per kg).
+ * A world that a polish does not bring there is restored and re-solved by
+ * that plain phase (a fallback).  tol < 0 sends every world to both plain
+ * phases; flags CBZ_SUR_ZERO_PREDICTION polishes from zero predictions (the
+ * same method without the network).  *n_fallback receives the number of
+ * worlds that fell back in either phase; cbz_sur_world_mask gives the
+ * velocity phase's fallbacks per world, cbz_sur_position_mask the position
+ * phase's (1: the plain phase ran).
  * Replaces ResolveContacts (contact.go:208-222) for callers that accept the
- * tolerance. */
+ * tolerances. */
 #define CBZ_SUR_ZERO_PREDICTION 1
 int cbz_sur_resolve(cbz_ctx *ctx, double dt, int64_t iter_per_contact, double tol, int64_t polish_per_contact,
                     int32_t flags, int64_t *n_fallback);
 int cbz_sur_world_mask(cbz_ctx *ctx, uint8_t *mask);
+int cbz_sur_position_mask(cbz_ctx *ctx, uint8_t *mask);
+/* The position phase's polish: tolerance in m above positionEpsilon
+ * (default 0: the plain phase's own stopping rule; negative: the plain
+ * position phase everywhere) and budget per contact (default 4, the plain
+ * phase's own).  No counterpart in the reference. */
+int cbz_sur_set_position(cbz_ctx *ctx, double tol, int64_t polish_per_contact);
+int cbz_sur_get_position(cbz_ctx *ctx, double *tol, int64_t *polish_per_contact);
 /* With profiling on (cbz_set_profiling), every surrogate resolve records HIP
  * events on the context's stream; this returns the accumulated ms
- * [exact position phase, features, network forward, per-body sums, velocity
- * phase (polish and fallbacks)] and counts [resolves, contacts, fallbacks,
- * velocity-phase iterations], and resets them. */
-int cbz_sur_phase_times(cbz_ctx *ctx, double *ms5, int64_t *counts4);
+ * [prepareContacts, features, network forward, per-body sums, position phase
+ * (polish and fallbacks), velocity phase (polish and fallbacks)] and counts
+ * [resolves, contacts, worlds that fell back in either phase, velocity-phase
+ * iterations, worlds that fell back in the position phase, position-phase
+ * iterations], and resets them. */
+int cbz_sur_phase_times(cbz_ctx *ctx, double *ms6, int64_t *counts6);
 /* cbz_step with the surrogate resolver; fallbacks[n_steps] (may be NULL)
  * receives the fallback count of every step */
 int cbz_step_sur(cbz_ctx *ctx, double dt, int32_t n_steps, int64_t iter_per_contact, double tol,

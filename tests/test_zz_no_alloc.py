@@ -46,7 +46,7 @@ def test_predict_device_equals_predict_without_allocating():
             # features around the training data's statistics
             X = (norm[:40] + rng.standard_normal((rows, 40)) * norm[40:80]).astype(np.float32)
             want = w.sur_predict(X)
-            got = np.full((rows, 12), np.nan, dtype=np.float32)
+            got = np.full((rows, 13), np.nan, dtype=np.float32)
             xd, yd = _dev(hip, X), _dev(hip, got)
             try:
                 assert hip.hipDeviceSynchronize() == 0
