@@ -7,7 +7,8 @@ step -- Integrate + CalculateDerivedData, the all-pairs narrowphase,
 ResolveContacts(4 x contacts) -- at dt = 1/60.  A step is one pass of that
 hot path over all worlds.  The metric is body-steps/s (bodies x steps /
 seconds) with the state resident in HBM, for the learned-surrogate resolver
-with exact fallback (FP32 state, tolerance 0.06 m/s of residual closing
+with exact fallback (FP32 state; both phases polished by the exact loop,
+positions to positionEpsilon, velocities to 0.06 m/s of residual closing
 speed = 1 mm of penetration per step), next to the exact FP64 GPU resolver
 it is compared with.
 

@@ -438,6 +438,34 @@ def test_training_gains_over_the_method_without_the_network():
     assert it_net < it_zero
 
 
+def test_packaged_model_gains_in_both_phases():
+    """The packaged model on fresh seeds (settled worlds): the polish from its
+    predictions leaves the exact loops fewer position and fewer velocity
+    iterations than the same polish from zero predictions."""
+    params, norm = surrogate.load()
+
+    def iterations(zero):
+        pos = vel = 0
+        for seed in (1000, 1001):
+            sc = scenes.batched(seed, 24)
+            with World(sc, precision="f32") as e:
+                e.step(sc.dt, 300, 4)
+                e.sur_set_norm(norm)
+                e.sur_set_params(params)
+                for _ in range(10):
+                    e.step_sur(sc.dt, 1, 4, surrogate.DEFAULT_TOL, surrogate.DEFAULT_POLISH, zero_prediction=zero)
+                    it = e.resolver_iterations()
+                    pos += int(it[:, 0].sum())
+                    vel += int(it[:, 1].sum())
+        return pos, vel
+
+    pos_net, vel_net = iterations(False)
+    pos_zero, vel_zero = iterations(True)
+    print("position iterations", pos_net, pos_zero, "velocity iterations", vel_net, vel_zero)
+    assert pos_net < pos_zero
+    assert vel_net < vel_zero
+
+
 def test_collection_schedule_of_1024_worlds_trains():
     """The collection that once ended in an illegal memory access at the
     first training GEMM (64 seeds x 16 worlds, snapshots at steps 60, 120,
