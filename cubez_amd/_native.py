@@ -68,6 +68,7 @@ EXPORTS = (
     "cbz_resolver_occupancy", "cbz_sur_check_gemms", "cbz_resolver_world_times", "cbz_set_resolver_split", "cbz_get_resolver_split",
     "cbz_autotune_resolver",
     "cbz_sur_predict_device", "cbz_sur_position_mask", "cbz_sur_set_position", "cbz_sur_get_position",
+    "cbz_pair_contact_counts",
 )
 
 
@@ -150,6 +151,7 @@ def load():
         "cbz_sur_resolve": ([P, f64, i64, f64, i64, i32, P], i32),
         "cbz_sur_world_mask": ([P, P], i32),
         "cbz_sur_position_mask": ([P, P], i32),
+        "cbz_pair_contact_counts": ([P, i64, P], i32),
         "cbz_sur_set_position": ([P, f64, i64], i32),
         "cbz_sur_get_position": ([P, P, P], i32),
         "cbz_step_sur": ([P, f64, i32, i64, f64, i64, i32, P, P], i32),

@@ -134,6 +134,8 @@ struct cbz_context {
   int32_t* l_off = nullptr;
   int32_t* l_one = nullptr;
   int32_t* l_two = nullptr;
+  int32_t* l_cnt = nullptr;   // contacts per listed pair of the last list narrowphase
+  int64_t l_last = -1;        // pairs in that list (-1: the last narrowphase had none)
   // sort-and-sweep broadphase (cbz_narrowphase_broad), allocated on first use
   struct Broad {
     bool ready = false;
@@ -434,6 +436,7 @@ void free_scene(cbz_ctx* c) {
   dfree(c->l_off);
   dfree(c->l_one);
   dfree(c->l_two);
+  dfree(c->l_cnt);
   dfree(c->tile_cnt);
   dfree(c->tile_off);
   dfree(c->tile_mask);
@@ -579,6 +582,7 @@ template <typename R>
 int do_narrowphase(cbz_ctx* c, int32_t list_tiles, hipEvent_t* ev = nullptr) {
   int rc = refresh_brad(c);
   if (rc) return rc;
+  if (list_tiles < 0) c->l_last = -1;  // no pair list: no per-pair counts
   NarrowLaunch<R> L;
   L.b = bodies_view<R>(c);
   L.pl.normal = (const R*)c->plane_normal;
@@ -597,6 +601,7 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles, hipEvent_t* ev = nullptr) {
     L.ps.world_task0 = c->d_world_task0;
     L.ps.list_off = nullptr;
     L.ps.list_one = nullptr;
+    L.ps.list_cnt = nullptr;
     L.ps.list_two = nullptr;
   } else {
     L.ps.n_tiles = list_tiles;
@@ -610,6 +615,7 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles, hipEvent_t* ev = nullptr) {
     L.ps.list_off = c->l_off;
     L.ps.list_one = c->l_one;
     L.ps.list_two = c->l_two;
+    L.ps.list_cnt = c->l_cnt;
   }
   L.c = contacts_view<R>(c);
   L.brad = c->brad;
@@ -694,6 +700,7 @@ int ensure_list(cbz_ctx* c, int64_t n_pairs) {
   if (n_pairs > c->l_pairs_cap) {
     if ((rc = dalloc(c, &c->l_one, (size_t)n_pairs))) return rc;
     if ((rc = dalloc(c, &c->l_two, (size_t)n_pairs))) return rc;
+    if ((rc = dalloc(c, &c->l_cnt, (size_t)n_pairs))) return rc;
     c->l_pairs_cap = (int32_t)n_pairs;
   }
   if (!c->l_world_tile0) {
@@ -1715,7 +1722,21 @@ int cbz_narrowphase_pairs(cbz_ctx* c, int64_t n_pairs, const int32_t* one, const
     HIPCHK(c, hipMemcpy(c->l_two, two, n_pairs * sizeof(int32_t), hipMemcpyHostToDevice));
   }
   HIPCHK(c, hipMemcpy(c->l_off, loff.data(), loff.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-  return run_list_narrowphase(c, wq, total);
+  c->l_last = -1;
+  if ((rc = run_list_narrowphase(c, wq, total))) return rc;
+  c->l_last = n_pairs;
+  return CBZ_OK;
+}
+
+int cbz_pair_contact_counts(cbz_ctx* c, int64_t n_pairs, int32_t* counts) {
+  if (!c || n_pairs < 0 || (n_pairs > 0 && !counts)) return fail(c, CBZ_ERR_ARG, "pair_contact_counts: bad arguments");
+  if (c->l_last < 0) return fail(c, CBZ_ERR_STATE, "pair_contact_counts: the last narrowphase had no pair list");
+  if (n_pairs != c->l_last)
+    return fail(c, CBZ_ERR_ARG, "pair_contact_counts: n_pairs differs from the last cbz_narrowphase_pairs");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (n_pairs > 0) HIPCHK(c, hipMemcpy(counts, c->l_cnt, n_pairs * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return CBZ_OK;
 }
 
 int cbz_narrowphase_broad(cbz_ctx* c, int64_t* total, int64_t* pairs_checked) {

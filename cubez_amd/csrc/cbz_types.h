@@ -125,6 +125,7 @@ struct PairSpace {
   const int32_t* list_off;
   const int32_t* list_one;
   const int32_t* list_two;
+  int32_t* list_cnt;  // out (or null): contacts each listed pair produced
 };
 
 }  // namespace cbz

@@ -599,6 +599,7 @@ __global__ void __launch_bounds__(256) k_np_count(NP<R> a) {
     int one, two;
     pair_of(a.ps, w, q, one, two);
     cnt = pair_eval<R, false>(a, one, two, 0);
+    if (a.ps.list_cnt) a.ps.list_cnt[a.ps.list_off[w] + q] = cnt;
   }
   const uint64_t m = __ballot(cnt > 0);
   const int tot = wave_sum_i32(cnt);

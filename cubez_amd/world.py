@@ -107,6 +107,13 @@ class World:
         self._chk(self._lib.cbz_narrowphase_broad(self._h, ctypes.byref(t), ctypes.byref(n)))
         return t.value, n.value
 
+    def pair_contact_counts(self, n_pairs):
+        """Contacts each pair of the last narrowphase_pairs call appended
+        (its CheckForCollisions bool is count > 0)."""
+        out = np.empty(n_pairs, np.int32)
+        self._chk(self._lib.cbz_pair_contact_counts(self._h, int(n_pairs), _ptr(out)))
+        return out
+
     def narrowphase_pairs(self, one, two):
         one = np.ascontiguousarray(one, np.int32)
         two = np.ascontiguousarray(two, np.int32)

@@ -145,6 +145,13 @@ int cbz_narrowphase(cbz_ctx *ctx, int64_t *total);
  * non-decreasing world order; both operands of a pair in the same world. */
 int cbz_narrowphase_pairs(cbz_ctx *ctx, int64_t n_pairs, const int32_t *one, const int32_t *two,
                           int64_t *total);
+/* The contacts each pair of the last cbz_narrowphase_pairs call appended,
+ * in list order (n_pairs must be that call's): counts[k] > 0 is the bool
+ * CheckForCollisions returns for pair k ("at least one contact appended",
+ * colliders.go:720-747), and pair k's contacts are the counts[k] that follow
+ * the earlier pairs' in cbz_get_contacts order -- what a Go caller needs to
+ * answer its per-pair calls from one batched narrowphase. */
+int cbz_pair_contact_counts(cbz_ctx *ctx, int64_t n_pairs, int32_t *counts);
 /* cbz_narrowphase with a sort-and-sweep broadphase in front, the first item
  * of the reference's roadmap (README.md:98-99): per world, the bodies are
  * sorted by the lower x bound of a conservative box (sqrt(3.003) times the

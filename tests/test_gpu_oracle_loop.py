@@ -54,6 +54,49 @@ def test_ballistic_example_pair_order_with_static_backboard():
               "bullet-bullet", hits["bullets"])
 
 
+def test_per_call_answers_of_a_batched_pair_list():
+    """What a Go caller's per-pair calls need from one batched narrowphase
+    (the lazy flush of a drop-in package): the ballistic example's frame,
+    its pairs run as one cbz_narrowphase_pairs, then cbz_pair_contact_counts
+    splits the contacts back into the example's calls.  Every call's bool
+    ("at least one contact appended", colliders.go:720-747) and the contacts
+    it appends equal the oracle's CheckForCollisions on that one pair, bit
+    for bit, for 120 frames."""
+    nb = 12
+    sc = scenes.ballistic_example(nb)
+    one, two = scenes.ballistic_example_pairs(nb)
+    found_calls = 0
+    with World(sc) as w:
+        o = OracleWorld(sc)
+        o.calculate_derived()
+        for step in range(120):
+            w.integrate(sc.dt)
+            o.integrate(sc.dt)
+            n = w.narrowphase_pairs(one, two)
+            counts = w.pair_contact_counts(len(one))
+            assert int(counts.sum()) == n, step
+            gc = w.contacts(0, internals=False) if n else None
+            off = 0
+            for k in range(len(one)):
+                oc = o.narrowphase_pairs(one[k:k + 1], two[k:k + 1])
+                assert (counts[k] > 0) == (len(oc) > 0), (step, k)
+                assert counts[k] == len(oc), (step, k)
+                if len(oc):
+                    found_calls += 1
+                    sl = slice(off, off + counts[k])
+                    assert np.array_equal(gc["body0"][sl], oc["body0"]) and np.array_equal(gc["body1"][sl], oc["body1"])
+                    assert same(gc["pen"][sl], oc["pen"]) and same(gc["normal"][sl], oc["normal"].reshape(-1, 3))
+                    assert same(gc["point"][sl], oc["point"].reshape(-1, 3))
+                off += counts[k]
+            # the frame's resolve, from the same batched contacts
+            oc = o.narrowphase_pairs(one, two)
+            if n:
+                w.resolve(sc.dt, 8, 0)
+                o.resolve(oc, sc.dt, 8 * len(oc))
+        assert_state_equal(w, o, 0, sc.n_bodies)
+    assert found_calls > 100
+
+
 def test_broadphase_matches_the_oracle_all_pairs():
     """cbz_narrowphase_broad in the step loop of a 1,024-body ballistic
     world against the oracle's all-pairs step: the same contact counts every
