@@ -91,9 +91,19 @@ __device__ __forceinline__ void contact_features(const Bodies<R>& b, const Conta
   if (raw)
     for (int i = 0; i < kFeat; ++i) raw[g * kFeat + i] = f[i];
   if (X) {
-    __bf16* x = X + g * kFeatPad;
-    for (int i = 0; i < kFeat; ++i) x[i] = __bf16((f[i] - mean[i]) * inv_std[i]);
-    for (int i = kFeat; i < kFeatPad; ++i) x[i] = __bf16(0.f);
+    // the row's 48 BF16 values as six 16-byte stores (kFeatPad * 2 bytes
+    // per row keeps every row 16-byte aligned)
+    bf16x8* x = (bf16x8*)(X + g * kFeatPad);
+#pragma unroll
+    for (int q = 0; q < kFeatPad / 8; ++q) {
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int i = 8 * q + j;
+        v[j] = __bf16(i < kFeat ? (f[i] - mean[i]) * inv_std[i] : 0.f);
+      }
+      x[q] = v;
+    }
   }
 }
 
