@@ -675,9 +675,12 @@ int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode 
   L.mw_min = c->mw_min;
   L.mw_max = c->mw_max;
   // A launch without iterations (prepareContacts only) gains nothing from the
-  // multi-wave kernel, and the position polish runs in the one-wavefront
-  // kernel only.
-  if ((mult == 0 && fixed == 0) || ((mode & RM_POLISH) && (mode & RM_POS))) L.mw_max = 0;
+  // multi-wave kernel, and leaves the worlds' iteration counts (the next
+  // position launch picks its multi-wave worlds by the last ones) alone.
+  if (mult == 0 && fixed == 0) {
+    L.mw_max = 0;
+    L.world_iters = nullptr;
+  }
   L.mw_list = c->mw_list;
   L.mw_count = c->mw_count;
   L.mw_flag = c->mw_flag;

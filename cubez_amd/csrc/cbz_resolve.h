@@ -1728,6 +1728,20 @@ struct MwLds {
   static constexpr size_t bytes = flags + kRegBodies;
 };
 
+// The largest (MAX) or least value over the workgroup; every thread gets it.
+template <bool MAX>
+__device__ __forceinline__ double mw_extreme(double v, double* s_sum, int t) {
+  v = dpp::allreduce(v, MAX ? -__builtin_inf() : __builtin_inf(),
+                     [](double x, double y) { return MAX ? (y > x ? y : x) : (y < x ? y : x); });
+  if ((t & 63) == 0) s_sum[t >> 6] = v;
+  __syncthreads();
+  double r = s_sum[0];
+#pragma unroll
+  for (int k = 1; k < kMwWaves; ++k) r = MAX ? (s_sum[k] > r ? s_sum[k] : r) : (s_sum[k] < r ? s_sum[k] : r);
+  __syncthreads();
+  return r;
+}
+
 // Sum over the workgroup in a fixed order (each wave's butterfly, then the
 // waves in index order); every thread gets the result.
 __device__ __forceinline__ double mw_sum(double v, double* s_sum, int t) {
@@ -1874,8 +1888,6 @@ __device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w
     if (do_pos) a.world_iters[2 * w] = 0;
     if (a.mode & RM_VEL) a.world_iters[2 * w + 1] = 0;
   }
-  // this kernel does not polish positions: the plain position phase
-  if (KIND == KIND_PPOLISH && t == 0) a.world_mask[w] = 1;
   // ResolveContacts' guard, contact.go:210-212
   if (a.overflow[0] != 0 || C <= 0 || a.dt <= R(0)) return;
   if (!do_pos && !do_vel) return;
@@ -1973,12 +1985,117 @@ __device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w
   }
   __syncthreads();
 
+  // The surrogate's position polish, as in resolve_reg: snapshot the
+  // sleeping bodies' motion and derived data, take the predicted moves, update
+  // the penetrations, reject a world they over-separate, then the loop to
+  // positionEpsilon + polish_tol within polish_mult * C iterations.
+  bool ppolish = KIND == KIND_PPOLISH && do_pos && a.world_mask[w] == 0;
+  R sep_floor = R(0);
+  auto key_min = [&]() {
+    R kmin = R(0);
+#pragma unroll
+    for (int s = 0; s < SL; ++s)
+      if (T * s + t < C) kmin = key[s] < kmin ? key[s] : kmin;
+    return R(mw_extreme<false>(double(kmin), s_sum, t));
+  };
+  auto restore_positions = [&]() {
+    // wave 0's HBM stores of the loop (motion, a sleeping body's derived
+    // data) complete before the other threads restore over them
+    __syncthreads();
+    for (int i = t; i < n; i += T) {
+      const int g = b0w + i;
+      lds3_store(s_st, i, ld3(b.pos, S, g));
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s_st[(3 + k) * KB + i] = b.quat[k * S + g];
+      const uint8_t f = b.flags[g];
+      s_flags[i] = f;
+      if (!(f & F_AWAKE)) {
+        b.motion[g] = a.vel0[g];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) b.transform[k * S + g] = a.vel0[(1 + k) * S + g];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          const R v = a.vel0[(13 + k) * S + g];
+          b.iit_world[k * S + g] = v;
+          s_iit[k * KB + i] = v;
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int j = T * s + t;
+      key[s] = j < C ? c.pen[base + j] : R(0);
+    }
+    __syncthreads();
+    if (t == 0) a.world_mask[w] = 1;
+  };
+  if constexpr (KIND == KIND_PPOLISH) {
+    if (ppolish) {
+      R pmax = R(0);
+#pragma unroll
+      for (int s = 0; s < SL; ++s) pmax = key[s] > pmax ? key[s] : pmax;
+      pmax = R(mw_extreme<true>(double(pmax), s_sum, t));
+      sep_floor = -(R(kSepFactor) * pmax + a.polish_tol);
+      const float* dl = a.delta;
+      for (int i = t; i < n; i += T) {
+        const int g = b0w + i;
+        if (!(s_flags[i] & F_AWAKE)) {
+          a.vel0[g] = b.motion[g];
+#pragma unroll
+          for (int k = 0; k < 12; ++k) a.vel0[(1 + k) * S + g] = b.transform[k * S + g];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) a.vel0[(13 + k) * S + g] = b.iit_world[k * S + g];
+          continue;
+        }
+        const V3<R> dx{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
+        const V3<R> dq{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
+        if (dx.x == R(0) && dx.y == R(0) && dx.z == R(0) && dq.x == R(0) && dq.y == R(0) && dq.z == R(0)) continue;
+        lds3_store(s_st, i, add(lds3(s_st, i), dx));
+        Q4<R> q{s_st[3 * KB + i], s_st[4 * KB + i], s_st[5 * KB + i], s_st[6 * KB + i]};
+        q = q_normalize(q_add_scaled_vector(q, dq, R(1)));
+        s_st[3 * KB + i] = q.w;
+        s_st[4 * KB + i] = q.x;
+        s_st[5 * KB + i] = q.y;
+        s_st[6 * KB + i] = q.z;
+      }
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int j = T * s + t;
+        if (j >= C) continue;
+        const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+        const V3<R> nj = cnrm(j), ps = cpt(j);
+        R pen = key[s];
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+          if (jb[bb] < 0) continue;
+          const int g = b0w + jb[bb];
+          const V3<R> dx{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
+          const V3<R> dq{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
+          if (dx.x == R(0) && dx.y == R(0) && dx.z == R(0) && dq.x == R(0) && dq.y == R(0) && dq.z == R(0)) continue;
+          const V3<R> rj = sub(ps, lds3(s_pos0, jb[bb]));
+          const V3<R> dp = add(cross(dq, rj), dx);
+          R sign = R(1);
+          if (bb == 0) sign = R(-1);
+          pen += dot(dp, nj) * sign;
+        }
+        key[s] = pen;
+      }
+      __syncthreads();
+      if (!(key_min() >= sep_floor)) {
+        restore_positions();
+        ppolish = false;
+      }
+    }
+  }
+
   // adjustPositions, contact.go:233-283.  A slot past the world's contacts
   // keeps key 0 and ids 0xFFFFFFFF, so the loops need no bounds test.
+  R peps = ppolish ? R(kPositionEpsilon) + a.polish_tol : R(kPositionEpsilon);
+  int64_t pbudget = ppolish ? a.polish_mult * int64_t(C) : maxIter;
   int64_t used = 0;
-  while (do_pos && used < maxIter) {
+  while (do_pos) {
     opaque_ids(ids);
-    R best = R(kPositionEpsilon);
+    R best = peps;
     int bi = INT_MAX;
 #pragma unroll
     for (int s = 0; s < SL; ++s) {
@@ -2000,11 +2117,13 @@ __device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w
     }
     __syncthreads();  // A
     if (wv == 0) {
-      R gk = R(kPositionEpsilon);
+      R gk = peps;
       int gi = INT_MAX;
       const int gw = mw_best(s_pkey, s_pidx, gk, gi);
       if (gw < 0) {
         if (lane == 0) s_resi[0] = MW_DONE;
+      } else if (used >= pbudget) {
+        if (lane == 0) s_resi[0] = MW_BUDGET;
       } else {
         const uint32_t kid = s_pkid[gw];
         const int kl[2] = {id_lo(kid), id_hi(kid)};
@@ -2062,7 +2181,22 @@ __device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w
       }
     }
     __syncthreads();  // C
-    if (s_resi[0] != MW_APPLY) break;
+    const int state = s_resi[0];
+    if (state != MW_APPLY) {
+      bool reject = false;
+      if constexpr (KIND == KIND_PPOLISH) {
+        // every contact within positionEpsilon + tol: the separation test;
+        // out of budget: rejected
+        if (ppolish) reject = state == MW_BUDGET || !(key_min() >= sep_floor);
+      }
+      if (!reject) break;
+      restore_positions();
+      ppolish = false;
+      peps = R(kPositionEpsilon);
+      pbudget = maxIter;
+      used = 0;
+      continue;
+    }
     {
       const uint32_t kid = uint32_t(s_resi[1]);
       const int kl[2] = {id_lo(kid), id_hi(kid)};

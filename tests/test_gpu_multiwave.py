@@ -169,3 +169,25 @@ def test_partial_split_on_many_worlds_resolves_every_world():
                 assert len(pen) and it[:, 0].sum() > 0
         for split in ((257, 32), (257, 100), (600, 7)):
             _same(out[(0, 0)], out[split], f"split {split}, surrogate {surrogate_step}")
+
+
+def test_multiwave_position_polish_rejections_equal_one_wave():
+    """The position polish in both kernels, with predictions shifted up so
+    that many worlds over-separate a contact and are rejected: the same bits
+    through either kernel, the same worlds rejected."""
+    params, norm = surrogate.load()
+    norm = norm.copy()
+    norm[80 + 12] += 0.02  # out_mean of the resolved penetration
+    sc, ck = _settled("f32")
+    out = {}
+    for split in (0, 1):
+        with World(precision="f32") as w:
+            w.restore(ck)
+            w.set_resolver_split(split)
+            w.sur_set_norm(norm)
+            w.sur_set_params(params)
+            w.sur_set_position(0.0, 4)
+            fb = w.step_sur(sc.dt, 3, 4, tol=0.06, polish=4)
+            out[split] = {"fb": np.asarray(fb), "pmask": w.sur_position_mask(), **_snapshot(w)}
+    _same(out[0], out[1], "position rejections")
+    print("worlds given the plain position phase:", int(out[0]["pmask"].sum()), "of", len(out[0]["pmask"]))
