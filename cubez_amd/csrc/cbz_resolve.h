@@ -51,6 +51,10 @@ enum : int { KIND_PLAIN = 0, KIND_LABEL = 1, KIND_POLISH = 2, KIND_MAT = 3, KIND
 // 1.4 times it (every world of 4096 at six points of a 600-step run; 0.1 %
 // of worlds pass 1.0 times).
 constexpr double kSepFactor = 2.0;
+// A prediction that breaks that bound is tried once more at this fraction of
+// its moves before the world is rejected: most over-separations come from
+// moves that are too large, and the loop finishes what smaller ones leave.
+constexpr double kPosRetryScale = 1.0 / 3.0;
 
 template <typename R>
 struct RS {
@@ -1169,56 +1173,70 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
           for (int k = 0; k < 12; ++k) a.vel0[(1 + k) * S + g] = b.transform[k * S + g];
 #pragma unroll
           for (int k = 0; k < 9; ++k) a.vel0[(13 + k) * S + g] = b.iit_world[k * S + g];
-          continue;
         }
-        const V3<R> dx{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
-        const V3<R> dq{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
-        if (dx.x == R(0) && dx.y == R(0) && dx.z == R(0) && dq.x == R(0) && dq.y == R(0) && dq.z == R(0)) continue;
-        lds3_store(s_st, i, add(lds3(s_st, i), dx));
-        Q4<R> q{s_st[3 * KB + i], s_st[4 * KB + i], s_st[5 * KB + i], s_st[6 * KB + i]};
-        q = q_normalize(q_add_scaled_vector(q, dq, R(1)));
-        s_st[3 * KB + i] = q.w;
-        s_st[4 * KB + i] = q.x;
-        s_st[5 * KB + i] = q.y;
-        s_st[6 * KB + i] = q.z;
       }
-#pragma unroll
-      for (int s = 0; s < SLOTS; ++s) {
-        asm volatile("" ::: "memory");
-        if (64 * s + lane >= C) continue;
-        const int64_t gj = base + 64 * s + lane;
-        const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
-        V3<R> nj;
-        if constexpr (WIDE)
-          nj = ld3(c.normal, CS, gj);
-        else
-          nj = nrm[s];
-        R pen = key[s];
-#pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {
-          if (jb[bb] < 0) continue;
-          const int g = b0w + jb[bb];
-          const V3<R> dx{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
-          const V3<R> dq{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
+      // the predicted moves times `scale`, from the phase's starting state
+      // (positions and orientations in HBM, penetrations in the contact
+      // array): the awake bodies' state, then every contact's penetration
+      // by the loop's own formula
+      auto take_moves = [&](R scale) {
+        for (int i = lane; i < n; i += 64) {
+          const int g = b0w + i;
+          if (!(s_flags[i] & F_AWAKE)) continue;
+          const V3<R> dx = mul(V3<R>{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])}, scale);
+          const V3<R> dq = mul(V3<R>{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])}, scale);
           if (dx.x == R(0) && dx.y == R(0) && dx.z == R(0) && dq.x == R(0) && dq.y == R(0) && dq.z == R(0)) continue;
-          V3<R> rj;
-          if constexpr (WIDE)
-            rj = ld3(c.rel + 3 * bb * CS, CS, gj);
-          else
-            rj = sub(ptr[s], lds3(s_pos0, jb[bb]));
-          const V3<R> dp = add(cross(dq, rj), dx);
-          R sign = R(1);
-          if (bb == 0) sign = R(-1);
-          pen += dot(dp, nj) * sign;
+          lds3_store(s_st, i, add(ld3(b.pos, S, g), dx));
+          Q4<R> q{b.quat[g], b.quat[S + g], b.quat[2 * S + g], b.quat[3 * S + g]};
+          q = q_normalize(q_add_scaled_vector(q, dq, R(1)));
+          s_st[3 * KB + i] = q.w;
+          s_st[4 * KB + i] = q.x;
+          s_st[5 * KB + i] = q.y;
+          s_st[6 * KB + i] = q.z;
         }
-        key[s] = pen;
-      }
-      __syncthreads();
-      // a prediction that already separates a contact beyond the bound is
-      // rejected before the loop spends iterations on it
+#pragma unroll
+        for (int s = 0; s < SLOTS; ++s) {
+          asm volatile("" ::: "memory");
+          if (64 * s + lane >= C) continue;
+          const int64_t gj = base + 64 * s + lane;
+          const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+          V3<R> nj;
+          if constexpr (WIDE)
+            nj = ld3(c.normal, CS, gj);
+          else
+            nj = nrm[s];
+          R pen = c.pen[gj];
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            if (jb[bb] < 0) continue;
+            const int g = b0w + jb[bb];
+            const V3<R> dx = mul(V3<R>{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])}, scale);
+            const V3<R> dq = mul(V3<R>{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])}, scale);
+            if (dx.x == R(0) && dx.y == R(0) && dx.z == R(0) && dq.x == R(0) && dq.y == R(0) && dq.z == R(0)) continue;
+            V3<R> rj;
+            if constexpr (WIDE)
+              rj = ld3(c.rel + 3 * bb * CS, CS, gj);
+            else
+              rj = sub(ptr[s], lds3(s_pos0, jb[bb]));
+            const V3<R> dp = add(cross(dq, rj), dx);
+            R sign = R(1);
+            if (bb == 0) sign = R(-1);
+            pen += dot(dp, nj) * sign;
+          }
+          key[s] = pen;
+        }
+        __syncthreads();
+      };
+      // a prediction that separates a contact beyond the bound is retried
+      // at a fraction of its moves, then rejected, before the loop spends
+      // iterations on it
+      take_moves(R(1));
       if (!(key_min() >= sep_floor)) {
-        restore_positions();
-        ppolish = false;
+        take_moves(R(kPosRetryScale));
+        if (!(key_min() >= sep_floor)) {
+          restore_positions();
+          ppolish = false;
+        }
       }
     }
   }
@@ -2045,45 +2063,56 @@ __device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w
           for (int k = 0; k < 12; ++k) a.vel0[(1 + k) * S + g] = b.transform[k * S + g];
 #pragma unroll
           for (int k = 0; k < 9; ++k) a.vel0[(13 + k) * S + g] = b.iit_world[k * S + g];
-          continue;
         }
-        const V3<R> dx{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
-        const V3<R> dq{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
-        if (dx.x == R(0) && dx.y == R(0) && dx.z == R(0) && dq.x == R(0) && dq.y == R(0) && dq.z == R(0)) continue;
-        lds3_store(s_st, i, add(lds3(s_st, i), dx));
-        Q4<R> q{s_st[3 * KB + i], s_st[4 * KB + i], s_st[5 * KB + i], s_st[6 * KB + i]};
-        q = q_normalize(q_add_scaled_vector(q, dq, R(1)));
-        s_st[3 * KB + i] = q.w;
-        s_st[4 * KB + i] = q.x;
-        s_st[5 * KB + i] = q.y;
-        s_st[6 * KB + i] = q.z;
       }
-#pragma unroll
-      for (int s = 0; s < SL; ++s) {
-        const int j = T * s + t;
-        if (j >= C) continue;
-        const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
-        const V3<R> nj = cnrm(j), ps = cpt(j);
-        R pen = key[s];
-#pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {
-          if (jb[bb] < 0) continue;
-          const int g = b0w + jb[bb];
-          const V3<R> dx{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])};
-          const V3<R> dq{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])};
+      // the predicted moves times `scale` from the phase's starting state,
+      // as in resolve_reg
+      auto take_moves = [&](R scale) {
+        for (int i = t; i < n; i += T) {
+          const int g = b0w + i;
+          if (!(s_flags[i] & F_AWAKE)) continue;
+          const V3<R> dx = mul(V3<R>{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])}, scale);
+          const V3<R> dq = mul(V3<R>{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])}, scale);
           if (dx.x == R(0) && dx.y == R(0) && dx.z == R(0) && dq.x == R(0) && dq.y == R(0) && dq.z == R(0)) continue;
-          const V3<R> rj = sub(ps, lds3(s_pos0, jb[bb]));
-          const V3<R> dp = add(cross(dq, rj), dx);
-          R sign = R(1);
-          if (bb == 0) sign = R(-1);
-          pen += dot(dp, nj) * sign;
+          lds3_store(s_st, i, add(ld3(b.pos, S, g), dx));
+          Q4<R> q{b.quat[g], b.quat[S + g], b.quat[2 * S + g], b.quat[3 * S + g]};
+          q = q_normalize(q_add_scaled_vector(q, dq, R(1)));
+          s_st[3 * KB + i] = q.w;
+          s_st[4 * KB + i] = q.x;
+          s_st[5 * KB + i] = q.y;
+          s_st[6 * KB + i] = q.z;
         }
-        key[s] = pen;
-      }
-      __syncthreads();
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int j = T * s + t;
+          if (j >= C) continue;
+          const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
+          const V3<R> nj = cnrm(j), ps = cpt(j);
+          R pen = c.pen[base + j];
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            if (jb[bb] < 0) continue;
+            const int g = b0w + jb[bb];
+            const V3<R> dx = mul(V3<R>{R(dl[g]), R(dl[S + g]), R(dl[2 * S + g])}, scale);
+            const V3<R> dq = mul(V3<R>{R(dl[3 * S + g]), R(dl[4 * S + g]), R(dl[5 * S + g])}, scale);
+            if (dx.x == R(0) && dx.y == R(0) && dx.z == R(0) && dq.x == R(0) && dq.y == R(0) && dq.z == R(0)) continue;
+            const V3<R> rj = sub(ps, lds3(s_pos0, jb[bb]));
+            const V3<R> dp = add(cross(dq, rj), dx);
+            R sign = R(1);
+            if (bb == 0) sign = R(-1);
+            pen += dot(dp, nj) * sign;
+          }
+          key[s] = pen;
+        }
+        __syncthreads();
+      };
+      take_moves(R(1));
       if (!(key_min() >= sep_floor)) {
-        restore_positions();
-        ppolish = false;
+        take_moves(R(kPosRetryScale));
+        if (!(key_min() >= sep_floor)) {
+          restore_positions();
+          ppolish = false;
+        }
       }
     }
   }
