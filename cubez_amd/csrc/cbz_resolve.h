@@ -582,14 +582,14 @@ __device__ __forceinline__ void resolve_world(const RS<R>& a, int w, int C, int6
       for (int bb = 0; bb < 2; ++bb) {
         if (jb[bb] < 0) continue;
         const V3<R> rj = ld3(c.rel + 3 * bb * CS, CS, base + j);
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-          if (jb[bb] == kb[d]) {
-            V3<R> dp = add(cross(ang[d], rj), lin[d]);
-            R sign = R(1);
-            if (bb == 0) sign = R(-1);
-            p += dot(dp, nj) * sign;
-          }
+        // a side is at most one of the two moved bodies: its change is
+        // selected and applied once
+        if (jb[bb] == kb[0] || jb[bb] == kb[1]) {
+          const bool d0 = jb[bb] == kb[0];
+          V3<R> dp = add(cross(sel3(d0, ang[0], ang[1]), rj), sel3(d0, lin[0], lin[1]));
+          R sign = R(1);
+          if (bb == 0) sign = R(-1);
+          p += dot(dp, nj) * sign;
         }
       }
       key[j] = p;
@@ -654,14 +654,14 @@ __device__ __forceinline__ void resolve_world(const RS<R>& a, int w, int C, int6
       for (int bb = 0; bb < 2; ++bb) {
         if (jb[bb] < 0) continue;
         const V3<R> rj = ld3(c.rel + 3 * bb * CS, CS, gj);
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-          if (jb[bb] == kb[d]) {
-            V3<R> dv = add(cross(rc[d], rj), vc[d]);
-            R sign = R(1);
-            if (bb == 1) sign = R(-1);
-            cvj = add(cvj, mul(m3tv(mj, dv), sign));
-          }
+        // a side is at most one of the two moved bodies: its change is
+        // selected and applied once
+        if (jb[bb] == kb[0] || jb[bb] == kb[1]) {
+          const bool d0 = jb[bb] == kb[0];
+          V3<R> dv = add(cross(sel3(d0, rc[0], rc[1]), rj), sel3(d0, vc[0], vc[1]));
+          R sign = R(1);
+          if (bb == 1) sign = R(-1);
+          cvj = add(cvj, mul(m3tv(mj, dv), sign));
         }
       }
       st3(c.cvel, CS, gj, cvj);
@@ -842,6 +842,11 @@ __device__ __forceinline__ V3<R> pick_impl(const V3<R> (&a)[N], int s, std::inte
 template <typename T, int N>
 __device__ __forceinline__ T pick(const T (&a)[N], int s) {
   return pick_impl(a, s, std::make_integer_sequence<int, N>{});
+}
+// a if c else b, component by component
+template <typename R>
+__device__ __forceinline__ V3<R> sel3(bool c, const V3<R>& a, const V3<R>& b) {
+  return V3<R>{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z};
 }
 
 template <typename R, int SLOTS, int KIND>
@@ -1343,14 +1348,14 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
           rj = ld3(c.rel + 3 * bb * CS, CS, gj);
         else
           rj = sub(ptr[s], lds3(s_pos0, jb[bb]));
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-          if (jb[bb] == kl[d]) {
-            V3<R> dp = add(cross(ang[d], rj), lin[d]);
-            R sign = R(1);
-            if (bb == 0) sign = R(-1);
-            pen += dot(dp, nj) * sign;
-          }
+        // a side is at most one of the two moved bodies: its change is
+        // selected and applied once
+        if (jb[bb] == kl[0] || jb[bb] == kl[1]) {
+          const bool d0 = jb[bb] == kl[0];
+          V3<R> dp = add(cross(sel3(d0, ang[0], ang[1]), rj), sel3(d0, lin[0], lin[1]));
+          R sign = R(1);
+          if (bb == 0) sign = R(-1);
+          pen += dot(dp, nj) * sign;
         }
       }
       key[s] = pen;
@@ -1649,14 +1654,14 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
             rj = ld3(c.rel + 3 * bb * CS, CS, gj);
           else
             rj = sub(ptr[s], lds3(s_pos0, jb[bb]));
-#pragma unroll
-          for (int d = 0; d < 2; ++d) {
-            if (jb[bb] == kl[d]) {
-              V3<R> dv = add(cross(rc[d], rj), vc[d]);
-              R sign = R(1);
-              if (bb == 1) sign = R(-1);
-              cvj = add(cvj, mul(m3tv(mj, dv), sign));
-            }
+          // a side is at most one of the two moved bodies: its change is
+          // selected and applied once
+          if (jb[bb] == kl[0] || jb[bb] == kl[1]) {
+            const bool d0 = jb[bb] == kl[0];
+            V3<R> dv = add(cross(sel3(d0, rc[0], rc[1]), rj), sel3(d0, vc[0], vc[1]));
+            R sign = R(1);
+            if (bb == 1) sign = R(-1);
+            cvj = add(cvj, mul(m3tv(mj, dv), sign));
           }
         }
         cv_set(s, gj, cvj);
@@ -2246,14 +2251,14 @@ __device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w
         for (int bb = 0; bb < 2; ++bb) {
           if (jb[bb] < 0) continue;
           const V3<R> rj = sub(ps, lds3(s_pos0, jb[bb]));
-#pragma unroll
-          for (int d = 0; d < 2; ++d) {
-            if (jb[bb] == kl[d]) {
-              V3<R> dp = add(cross(ang[d], rj), lin[d]);
-              R sign = R(1);
-              if (bb == 0) sign = R(-1);
-              pen += dot(dp, ns) * sign;
-            }
+          // a side is at most one of the two moved bodies: its change is
+          // selected and applied once
+          if (jb[bb] == kl[0] || jb[bb] == kl[1]) {
+            const bool d0 = jb[bb] == kl[0];
+            V3<R> dp = add(cross(sel3(d0, ang[0], ang[1]), rj), sel3(d0, lin[0], lin[1]));
+            R sign = R(1);
+            if (bb == 0) sign = R(-1);
+            pen += dot(dp, ns) * sign;
           }
         }
         key[s] = pen;
@@ -2546,14 +2551,14 @@ __device__ __forceinline__ void resolve_mw(const RS<R>& a, int w, int C, int b0w
           for (int bb = 0; bb < 2; ++bb) {
             if (jb[bb] < 0) continue;
             const V3<R> rj = sub(ps, lds3(s_pos0, jb[bb]));
-#pragma unroll
-            for (int d = 0; d < 2; ++d) {
-              if (jb[bb] == kl[d]) {
-                V3<R> dv = add(cross(rc[d], rj), vc[d]);
-                R sign = R(1);
-                if (bb == 1) sign = R(-1);
-                cvj = add(cvj, mul(m3tv(mj, dv), sign));
-              }
+            // a side is at most one of the two moved bodies: its change is
+            // selected and applied once
+            if (jb[bb] == kl[0] || jb[bb] == kl[1]) {
+              const bool d0 = jb[bb] == kl[0];
+              V3<R> dv = add(cross(sel3(d0, rc[0], rc[1]), rj), sel3(d0, vc[0], vc[1]));
+              R sign = R(1);
+              if (bb == 1) sign = R(-1);
+              cvj = add(cvj, mul(m3tv(mj, dv), sign));
             }
           }
           cv[s] = cvj;
