@@ -14,6 +14,7 @@ namespace cbz {
 // world, and resets the queue head mw_count[1].  One workgroup; thread t
 // looks at kSelectPer consecutive entries of each chunk.
 constexpr int kSelectThreads = 1024, kSelectPer = 16, kSelectBins = 257, kSelectShift = 6;
+static_assert(kSelectBins <= kSelectThreads, "one thread per bucket in the threshold scan");
 __device__ __forceinline__ int select_bin(const int64_t* world_iters, int32_t mode, int32_t w, int C) {
   int64_t cost = 0;
   if (world_iters) {
@@ -41,11 +42,20 @@ __global__ void __launch_bounds__(kSelectThreads)
     const int C = world_cnt[w];
     if (mw_fits(C, body_off[w + 1] - body_off[w], mw_min)) atomicAdd(&s_hist[select_bin(world_iters, mode, w, C)], 1);
   }
+  if (t == 0) s_thr = mw_max <= 0 ? kSelectBins : 0;
   __syncthreads();
-  if (t == 0) {
-    int thr = kSelectBins, cum = 0;
-    while (thr > 0 && cum < mw_max) cum += s_hist[--thr];
-    s_thr = thr;  // buckets >= thr (none when mw_max <= 0)
+  // The threshold: the highest bucket whose suffix count (its worlds and
+  // every costlier bucket's) reaches mw_max; 0 when none does, kSelectBins
+  // (no world) when mw_max <= 0.  Thread r holds bucket kSelectBins-1-r, so
+  // an inclusive scan over threads is the suffix count.
+  {
+    const int r = t;
+    const int inc = wave_incl_scan_i32(r < kSelectBins ? s_hist[kSelectBins - 1 - r] : 0);
+    if (lane == 63) s_wave[wv] = inc;
+    __syncthreads();
+    int suffix = inc;
+    for (int v = 0; v < wv; ++v) suffix += s_wave[v];
+    if (mw_max > 0 && r < kSelectBins && suffix >= mw_max) atomicMax(&s_thr, kSelectBins - 1 - r);
   }
   __syncthreads();
   const int thr = s_thr;
