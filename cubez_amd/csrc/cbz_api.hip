@@ -170,6 +170,7 @@ struct cbz_context {
   // point 3, normal 3, pen 1, c2w 9, rel 6, cvel 3, ddv 1, friction 1,
   // restitution 1, then the resolver's impulse data 12 (40 planes)
   void* c_real = nullptr;
+  void* c_rec = nullptr;  // [cap][16] reals: Contacts::rec
   int32_t* world_cnt = nullptr;
   int64_t* world_coff = nullptr;
   int64_t* total = nullptr;
@@ -443,6 +444,7 @@ void free_scene(cbz_ctx* c) {
   dfree(c->c_body0);
   dfree(c->c_body1);
   dfree(c->c_real);
+  dfree(c->c_rec);
   dfree(c->world_cnt);
   dfree(c->world_coff);
   dfree(c->total);
@@ -506,6 +508,7 @@ Contacts<R> contacts_view(cbz_ctx* c) {
   v.fric = mat ? base + 26 * S : nullptr;
   v.rest = mat ? base + 27 * S : nullptr;
   v.imp = base + 28 * S;
+  v.rec = (R*)c->c_rec;
   return v;
 }
 
@@ -1485,6 +1488,7 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc_bytes(c, &c->c_body0, sizeof(int32_t) * (size_t)c->cap))) return rc;
   if ((rc = dalloc_bytes(c, &c->c_body1, sizeof(int32_t) * (size_t)c->cap))) return rc;
   if ((rc = dalloc_bytes(c, &c->c_real, rs * kContactPlanes * (size_t)c->cap))) return rc;
+  if ((rc = dalloc_bytes(c, &c->c_rec, rs * 16 * (size_t)c->cap))) return rc;
   if ((rc = dalloc(c, &c->hits, (size_t)c->cap))) return rc;
   if ((rc = dalloc(c, &c->n_hits, 1))) return rc;
   // candidate pairs of the cull: 4 per contact slot plus the scene's ordered

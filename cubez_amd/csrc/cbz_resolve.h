@@ -849,6 +849,63 @@ __device__ __forceinline__ V3<R> sel3(bool c, const V3<R>& a, const V3<R>& b) {
   return V3<R>{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z};
 }
 
+// A contact's record (Contacts::rec): 16 reals from one 64-byte (FP32) or
+// 128-byte (FP64) aligned address, in 16-byte loads.
+template <typename R>
+__device__ __forceinline__ void load_rec(const R* q, R (&v)[16]) {
+  if constexpr (sizeof(R) == 4) {
+    const float4* p = reinterpret_cast<const float4*>(q);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float4 x = p[i];
+      v[4 * i] = x.x;
+      v[4 * i + 1] = x.y;
+      v[4 * i + 2] = x.z;
+      v[4 * i + 3] = x.w;
+    }
+  } else {
+    const double2* p = reinterpret_cast<const double2*>(q);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const double2 x = p[i];
+      v[2 * i] = x.x;
+      v[2 * i + 1] = x.y;
+    }
+  }
+}
+// contactToWorld and both relative positions
+template <typename R>
+__device__ __forceinline__ void rec_frame(const R* q, R (&m)[9], V3<R> (&r)[2]) {
+  R v[16];
+  load_rec(q, v);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) m[k] = v[k];
+  r[0] = V3<R>{v[9], v[10], v[11]};
+  r[1] = V3<R>{v[12], v[13], v[14]};
+}
+// the normal (contactToWorld's first column) and both relative positions
+template <typename R>
+__device__ __forceinline__ void rec_normal(const R* q, V3<R>& n, V3<R> (&r)[2]) {
+  R v[16];
+  load_rec(q, v);
+  n = V3<R>{v[0], v[1], v[2]};
+  r[0] = V3<R>{v[9], v[10], v[11]};
+  r[1] = V3<R>{v[12], v[13], v[14]};
+}
+template <typename R>
+__device__ __forceinline__ void store_rec(R* q, const R (&m)[9], const V3<R>& r0, const V3<R>& r1) {
+  const R v[16] = {m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, R(0)};
+  if constexpr (sizeof(R) == 4) {
+    float4* p = reinterpret_cast<float4*>(q);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) p[i] = float4{v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]};
+  } else {
+    double2* p = reinterpret_cast<double2*>(q);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) p[i] = double2{v[2 * i], v[2 * i + 1]};
+  }
+}
+
 template <typename R, int SLOTS, int KIND>
 struct RegLds {
   // the wide classes keep only key, ids and contact velocity in VGPRs; the
@@ -1078,6 +1135,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
       st3(c.rel + 3 * CS, CS, g, r1);
       st3(c.cvel, CS, g, cv);
       c.ddv[g] = ddv;
+      if constexpr (WIDE) store_rec(c.rec + 16 * g, m, r0, r1);
       if constexpr (!WIDE) s_bs[j] = sc;
     }
   } else if constexpr (!WIDE) {  // prepared by an earlier launch: the basis scales of prepare's normals
@@ -1297,9 +1355,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
     const int kb[2] = {b0w + kl[0], kl[1] >= 0 ? b0w + kl[1] : -1};
     V3<R> n, rel[2];
     if constexpr (WIDE) {
-      n = ld3(c.normal, CS, base + bi);
-      rel[0] = ld3(c.rel, CS, base + bi);
-      rel[1] = ld3(c.rel + 3 * CS, CS, base + bi);
+      rec_normal(c.rec + 16 * (base + bi), n, rel);
     } else {
       n = lane_value(pick(nrm, ss), sl);
       const V3<R> p = lane_value(pick(ptr, ss), sl);
@@ -1335,9 +1391,9 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
       if (!any) continue;
       const int64_t gj = base + 64 * s + ln;
       R pen = key[s];
-      V3<R> nj;
+      V3<R> nj, rw[2];
       if constexpr (WIDE)
-        nj = ld3(c.normal, CS, gj);
+        rec_normal(c.rec + 16 * gj, nj, rw);
       else
         nj = nrm[s];
 #pragma unroll
@@ -1345,7 +1401,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
         if (jb[bb] < 0) continue;
         V3<R> rj;
         if constexpr (WIDE)
-          rj = ld3(c.rel + 3 * bb * CS, CS, gj);
+          rj = rw[bb];
         else
           rj = sub(ptr[s], lds3(s_pos0, jb[bb]));
         // a side is at most one of the two moved bodies: its change is
@@ -1595,10 +1651,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
       R m[9], imp[kImpPlanes];
       V3<R> rel[2];
       if constexpr (WIDE) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) m[k] = c.c2w[k * CS + base + bi];
-        rel[0] = ld3(c.rel, CS, base + bi);
-        rel[1] = ld3(c.rel + 3 * CS, CS, base + bi);
+        rec_frame(c.rec + 16 * (base + bi), m, rel);
       } else {
         const V3<R> n = lane_value(pick(nrm, ss), sl);
         const V3<R> p = lane_value(pick(ptr, ss), sl);
@@ -1639,9 +1692,9 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
         if (!any) continue;
         const int64_t gj = base + 64 * s + ln;
         R mj[9];
+        V3<R> rw[2];
         if constexpr (WIDE) {
-#pragma unroll
-          for (int k = 0; k < 9; ++k) mj[k] = c.c2w[k * CS + gj];
+          rec_frame(c.rec + 16 * gj, mj, rw);
         } else {
           basis_from_scale(nrm[s], s_bs[64 * s + lane], mj);
         }
@@ -1651,7 +1704,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
           if (jb[bb] < 0) continue;
           V3<R> rj;
           if constexpr (WIDE)
-            rj = ld3(c.rel + 3 * bb * CS, CS, gj);
+            rj = rw[bb];
           else
             rj = sub(ptr[s], lds3(s_pos0, jb[bb]));
           // a side is at most one of the two moved bodies: its change is

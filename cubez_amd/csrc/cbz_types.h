@@ -81,6 +81,11 @@ struct Contacts {
   // (impulseMatrix 9, deltaVelocity entries 0, 3, 6), written when the
   // phase starts
   R* imp;          // [12][cap]
+  // per contact one 16-value record, contactToWorld 9,
+  // relativeContactPosition[0] 3, [1] 3, 1 unused: a copy of c2w and rel
+  // that the one-wavefront resolver's wide classes write in prepare and read
+  // in their loops from one address
+  R* rec;          // [cap][16]
 };
 
 // Sources of the per-contact materials the narrowphase writes: friction and
