@@ -872,6 +872,15 @@ int check_overflow(cbz_ctx* c) {
   return CBZ_OK;
 }
 
+// The most bodies in one world, and the bound up to which the feature rows
+// come from the per-world kernel (its per-body counts in LDS).
+constexpr int32_t kFeatWorldBodiesMax = 4096;
+int32_t max_world_bodies(const cbz_ctx* c) {
+  int32_t m = 0;
+  for (int32_t w = 0; w < c->nw; ++w) m = std::max(m, c->body_off[w + 1] - c->body_off[w]);
+  return m;
+}
+
 // World order for the resolver: larger worlds first, so the longest
 // serial chains start earliest.
 void build_order(cbz_ctx* c, std::vector<int32_t>& ord) {
@@ -1294,8 +1303,14 @@ int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int64_t polish_
     // The feature and network grids cover the contact capacity and read the
     // contact count on the device: no host round trip, so the step can be
     // captured in a graph.
-    HIPCHK(c, launch_features<R>(bodies_view<R>(c), contacts_view<R>(c), c->cap, c->total, u.deg, u.norm,
-                                 u.norm + kFeat, u.X, nullptr, c->stream));
+    const int32_t max_nb = max_world_bodies(c);
+    if (max_nb <= kFeatWorldBodiesMax)
+      HIPCHK(c, launch_features_world<R>(bodies_view<R>(c), contacts_view<R>(c), c->world_cnt, c->world_coff,
+                                         c->d_body_off, c->overflow, c->nw, max_nb, u.norm, u.norm + kFeat, u.X,
+                                         nullptr, c->stream));
+    else
+      HIPCHK(c, launch_features<R>(bodies_view<R>(c), contacts_view<R>(c), c->cap, c->total, u.deg, u.norm,
+                                   u.norm + kFeat, u.X, nullptr, c->stream));
     if (c->prof) HIPCHK(c, hipEventRecord(ev[2], c->stream));
     if ((rc = sur_infer(c, u.X, c->cap, c->total))) return rc;
     if (c->prof) HIPCHK(c, hipEventRecord(ev[3], c->stream));
@@ -2397,12 +2412,23 @@ int cbz_sur_collect(cbz_ctx* c, double dt, int64_t iter_per_contact, int64_t* ad
   // the exact position and velocity phases with their labels
   rc = f64 ? do_resolve<double>(c, dt, 0, 0, RM_POS) : do_resolve<float>(c, dt, 0, 0, RM_POS);
   if (rc) return rc;
-  if (f64)
+  const int32_t max_nb = max_world_bodies(c);
+  if (max_nb <= kFeatWorldBodiesMax) {
+    if (f64)
+      HIPCHK(c, launch_features_world<double>(bodies_view<double>(c), contacts_view<double>(c), c->world_cnt,
+                                              c->world_coff, c->d_body_off, c->overflow, c->nw, max_nb, nullptr,
+                                              nullptr, nullptr, X, c->stream));
+    else
+      HIPCHK(c, launch_features_world<float>(bodies_view<float>(c), contacts_view<float>(c), c->world_cnt,
+                                             c->world_coff, c->d_body_off, c->overflow, c->nw, max_nb, nullptr,
+                                             nullptr, nullptr, X, c->stream));
+  } else if (f64) {
     HIPCHK(c, launch_features<double>(bodies_view<double>(c), contacts_view<double>(c), tot, nullptr, u.deg, nullptr, nullptr,
                                       nullptr, X, c->stream));
-  else
+  } else {
     HIPCHK(c, launch_features<float>(bodies_view<float>(c), contacts_view<float>(c), tot, nullptr, u.deg, nullptr, nullptr,
                                      nullptr, X, c->stream));
+  }
   HIPCHK(c, launch_fill_f32(Y, tot * kOut, 0.f, c->stream));
   rc = f64 ? do_resolve<double>(c, dt, iter_per_contact, 0, RM_POS | RM_PREPARED | RM_LABEL, nullptr, Y)
            : do_resolve<float>(c, dt, iter_per_contact, 0, RM_POS | RM_PREPARED | RM_LABEL, nullptr, Y);

@@ -87,6 +87,15 @@ template <typename R>
 hipError_t launch_features(const Bodies<R>& b, const Contacts<R>& c, int64_t n_cap, const int64_t* n_dev,
                            int32_t* deg, const float* mean, const float* inv_std, __bf16* X, float* raw,
                            hipStream_t st);
+// The feature rows of the step's contacts (normalised BF16 rows into X and/or
+// raw FP32 rows into raw, either may be null), one workgroup per world, the
+// per-body contact counts in LDS; worlds of at most 4096 bodies (else
+// hipErrorInvalidValue: use launch_features).
+template <typename R>
+hipError_t launch_features_world(const Bodies<R>& b, const Contacts<R>& c, const int32_t* world_cnt,
+                                 const int64_t* world_coff, const int32_t* body_off, const int32_t* overflow,
+                                 int n_worlds, int max_world_bodies, const float* mean, const float* inv_std,
+                                 __bf16* X, float* raw, hipStream_t st);
 template <typename R>
 hipError_t launch_sur_scatter(const SurApply<R>& a, int n_worlds, hipStream_t st);
 hipError_t launch_gather_batch(const float* Xraw, const float* Yraw, const int64_t* idx, int rows, int valid_rows,

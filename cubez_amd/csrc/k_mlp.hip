@@ -67,10 +67,12 @@ __global__ void k_degree(const int32_t* body0, const int32_t* body1, int64_t n_c
 // Feature layout (kFeat = 40): normal 3, rel0 3, rel1 3, v0-v1 3, w0-w1 3,
 // invMass0, invMass1, iitWorld0 9, iitWorld1 9, penetration,
 // desiredDeltaVelocity, contactVelocity.x, 1/deg0, 1/deg1.
+// deg0, deg1: the contact counts of the contact's bodies (deg1 unused when
+// there is no second body).
 template <typename R>
-__device__ __forceinline__ void contact_features(const Bodies<R>& b, const Contacts<R>& c, int64_t g,
-                                                 const int32_t* deg, const float* mean, const float* inv_std,
-                                                 __bf16* X, float* raw) {
+__device__ __forceinline__ void contact_features(const Bodies<R>& b, const Contacts<R>& c, int64_t g, int deg0,
+                                                 int deg1, const float* mean, const float* inv_std, __bf16* X,
+                                                 float* raw) {
   const int64_t S = b.stride, CS = c.cap;
   const int b0 = c.body0[g], b1 = c.body1[g];
   float f[kFeat];
@@ -86,8 +88,8 @@ __device__ __forceinline__ void contact_features(const Bodies<R>& b, const Conta
   f[k++] = float(c.pen[g]);
   f[k++] = float(c.ddv[g]);
   f[k++] = float(c.cvel[g]);
-  f[k++] = 1.0f / float(deg[b0]);
-  f[k++] = b1 >= 0 ? 1.0f / float(deg[b1]) : 0.f;
+  f[k++] = 1.0f / float(deg0);
+  f[k++] = b1 >= 0 ? 1.0f / float(deg1) : 0.f;
   if (raw)
     for (int i = 0; i < kFeat; ++i) raw[g * kFeat + i] = f[i];
   if (X) {
@@ -111,8 +113,46 @@ template <typename R>
 __global__ void k_features(Bodies<R> b, Contacts<R> c, int64_t n_cap, const int64_t* n_dev, const int32_t* deg,
                            const float* mean, const float* inv_std, __bf16* X, float* raw) {
   const int64_t n = rows_of(n_cap, n_dev);
-  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < n; g += int64_t(gridDim.x) * blockDim.x)
-    contact_features(b, c, g, deg, mean, inv_std, X, raw);
+  for (int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < n; g += int64_t(gridDim.x) * blockDim.x) {
+    const int b0 = c.body0[g], b1 = c.body1[g];
+    contact_features(b, c, g, deg[b0], b1 >= 0 ? deg[b1] : 0, mean, inv_std, X, raw);
+  }
+}
+
+// The same rows for the step's contacts, one workgroup per world: the
+// world's contacts are counted per body in LDS (integer counts, so the same
+// degrees as k_degree's), then its rows written.  Worlds of at most
+// kFeatWorldBodies bodies; the launcher takes the per-contact kernels for
+// larger ones.
+constexpr int kFeatWorldBodies = 4096;
+template <typename R>
+__global__ void __launch_bounds__(256) k_features_world(Bodies<R> b, Contacts<R> c, const int32_t* world_cnt,
+                                                        const int64_t* world_coff, const int32_t* body_off,
+                                                        const int32_t* overflow, const float* mean,
+                                                        const float* inv_std, __bf16* X, float* raw) {
+  __shared__ int32_t s_deg[kFeatWorldBodies];
+  const int w = blockIdx.x, t = threadIdx.x;
+  if (overflow[0] != 0) return;  // the step reports the overflow; no row is read
+  const int C = world_cnt[w];
+  const int b0w = body_off[w], n = body_off[w + 1] - b0w;
+  const int64_t base = world_coff[w];
+  for (int i = t; i < n; i += 256) s_deg[i] = 0;
+  __syncthreads();
+  // prepare has made body0 a body of the world for every contact; the
+  // index tests keep a malformed contact from touching other LDS
+  for (int j = t; j < C; j += 256) {
+    const int64_t g = base + j;
+    const unsigned l0 = unsigned(c.body0[g] - b0w), l1 = unsigned(c.body1[g] - b0w);
+    if (l0 < unsigned(n)) atomicAdd(&s_deg[l0], 1);
+    if (c.body1[g] >= 0 && l1 < unsigned(n)) atomicAdd(&s_deg[l1], 1);
+  }
+  __syncthreads();
+  for (int j = t; j < C; j += 256) {
+    const int64_t g = base + j;
+    const unsigned l0 = unsigned(c.body0[g] - b0w), l1 = unsigned(c.body1[g] - b0w);
+    const int d0 = l0 < unsigned(n) ? s_deg[l0] : 0, d1 = l1 < unsigned(n) ? s_deg[l1] : 0;
+    contact_features(b, c, g, d0, d1, mean, inv_std, X, raw);
+  }
 }
 
 // Batch sampling on the device: row i of training step `step` is
@@ -635,6 +675,18 @@ hipError_t launch_features(const Bodies<R>& b, const Contacts<R>& c, int64_t n_c
 }
 
 template <typename R>
+hipError_t launch_features_world(const Bodies<R>& b, const Contacts<R>& c, const int32_t* world_cnt,
+                                 const int64_t* world_coff, const int32_t* body_off, const int32_t* overflow,
+                                 int n_worlds, int max_world_bodies, const float* mean, const float* inv_std,
+                                 __bf16* X, float* raw, hipStream_t st) {
+  if (n_worlds <= 0) return hipSuccess;
+  if (max_world_bodies > kFeatWorldBodies) return hipErrorInvalidValue;
+  k_features_world<R><<<n_worlds, 256, 0, st>>>(b, c, world_cnt, world_coff, body_off, overflow, mean, inv_std, X,
+                                                raw);
+  return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_sur_scatter(const SurApply<R>& a, int n_worlds, hipStream_t st) {
   if (n_worlds <= 0) return hipSuccess;
   k_sur_scatter<R><<<n_worlds, 256, 0, st>>>(a);
@@ -706,6 +758,12 @@ template hipError_t launch_features<double>(const Bodies<double>&, const Contact
 template hipError_t launch_features<float>(const Bodies<float>&, const Contacts<float>&, int64_t, const int64_t*,
                                            int32_t*,
                                            const float*, const float*, __bf16*, float*, hipStream_t);
+template hipError_t launch_features_world<double>(const Bodies<double>&, const Contacts<double>&, const int32_t*,
+                                                  const int64_t*, const int32_t*, const int32_t*, int, int,
+                                                  const float*, const float*, __bf16*, float*, hipStream_t);
+template hipError_t launch_features_world<float>(const Bodies<float>&, const Contacts<float>&, const int32_t*,
+                                                 const int64_t*, const int32_t*, const int32_t*, int, int,
+                                                 const float*, const float*, __bf16*, float*, hipStream_t);
 template hipError_t launch_sur_scatter<double>(const SurApply<double>&, int, hipStream_t);
 template hipError_t launch_sur_scatter<float>(const SurApply<float>&, int, hipStream_t);
 

@@ -186,6 +186,55 @@ def test_collect_is_the_exact_resolve_and_labels_sum_to_body_changes():
         assert it.sum() > 0
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_collected_features_are_the_prepared_contacts(precision):
+    """The feature rows (one workgroup per world, the per-body contact counts
+    in LDS) against the same 40 values computed here from a prepare-only
+    resolve of the same state: normal, both lever arms, the bodies' relative
+    velocity and rotation, inverse masses, world inverse inertias,
+    penetration, desiredDeltaVelocity, the closing speed and 1/contacts per
+    body, each rounded to FP32 as the kernel rounds it."""
+    sc, steps = _settled()
+    with World(sc, precision=precision) as a, World(sc, precision=precision) as b:
+        for w in (a, b):
+            w.step(sc.dt, steps, 4)
+            w.integrate(sc.dt)
+            w.narrowphase()
+        a.sur_reserve(200_000)
+        n = a.sur_collect(sc.dt, 4)
+        X, _ = a.sur_dataset()
+        b.resolve(sc.dt, 0, 0)  # prepare, no iterations
+        c = b.contacts(-1, internals=True)
+        assert n == len(c["pen"]) > 0
+        f = np.float32
+        vel, rot = b.get("vel").astype(f), b.get("rot").astype(f)
+        im, iw = b.get("inv_mass").astype(f), b.get("iit_world").astype(f)
+        b0, b1 = c["body0"], c["body1"]
+        assert (b0 >= 0).all()
+        has1 = b1 >= 0
+        b1c = np.where(has1, b1, 0)
+        h = has1[:, None]
+        deg = np.bincount(b0, minlength=len(vel)) + np.bincount(b1[has1], minlength=len(vel))
+        want = np.zeros((n, 40), f)
+        want[:, 0:3] = c["normal"].astype(f)
+        want[:, 3:9] = c["rel"].astype(f)
+        want[:, 9:12] = vel[b0] - np.where(h, vel[b1c], f(0))
+        want[:, 12:15] = rot[b0] - np.where(h, rot[b1c], f(0))
+        want[:, 15] = im[b0]
+        want[:, 16] = np.where(has1, im[b1c], f(0))
+        want[:, 17:26] = iw[b0]
+        want[:, 26:35] = np.where(h, iw[b1c], f(0))
+        want[:, 35] = c["pen"].astype(f)
+        want[:, 36] = c["ddv"].astype(f)
+        want[:, 37] = c["cvel"][:, 0].astype(f)
+        want[:, 38] = f(1) / deg[b0].astype(f)
+        want[:, 39] = np.where(has1, f(1) / np.maximum(deg[b1c], 1).astype(f), f(0))
+        X = np.asarray(X, f)
+        assert np.array_equal(X[:, :38], want[:, :38])
+        # 1/deg: within an ulp whatever the division's rounding
+        assert np.allclose(X[:, 38:], want[:, 38:], rtol=2e-7, atol=0)
+
+
 def _plain_and_sur(tol, params_fn, polish=1, zero=False, norm=None, seed=5, pos=(-1.0, 4)):
     """Five resolves of the same contacts, plain and with the surrogate
     (position polish off unless `pos` gives its tolerance and budget)."""
