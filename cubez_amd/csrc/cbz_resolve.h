@@ -208,8 +208,66 @@ __device__ __forceinline__ void contact_basis(const V3<R>& n, R* m) {
   basis_from_scale(n, basis_scale(n), m);
 }
 
+// A contact's record (Contacts::rec): 16 reals from one 64-byte (FP32) or
+// 128-byte (FP64) aligned address, in 16-byte loads.
+template <typename R>
+__device__ __forceinline__ void load_rec(const R* q, R (&v)[16]) {
+  if constexpr (sizeof(R) == 4) {
+    const float4* p = reinterpret_cast<const float4*>(q);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float4 x = p[i];
+      v[4 * i] = x.x;
+      v[4 * i + 1] = x.y;
+      v[4 * i + 2] = x.z;
+      v[4 * i + 3] = x.w;
+    }
+  } else {
+    const double2* p = reinterpret_cast<const double2*>(q);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const double2 x = p[i];
+      v[2 * i] = x.x;
+      v[2 * i + 1] = x.y;
+    }
+  }
+}
+// contactToWorld and both relative positions
+template <typename R>
+__device__ __forceinline__ void rec_frame(const R* q, R (&m)[9], V3<R> (&r)[2]) {
+  R v[16];
+  load_rec(q, v);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) m[k] = v[k];
+  r[0] = V3<R>{v[9], v[10], v[11]};
+  r[1] = V3<R>{v[12], v[13], v[14]};
+}
+// the normal (contactToWorld's first column) and both relative positions
+template <typename R>
+__device__ __forceinline__ void rec_normal(const R* q, V3<R>& n, V3<R> (&r)[2]) {
+  R v[16];
+  load_rec(q, v);
+  n = V3<R>{v[0], v[1], v[2]};
+  r[0] = V3<R>{v[9], v[10], v[11]};
+  r[1] = V3<R>{v[12], v[13], v[14]};
+}
+template <typename R>
+__device__ __forceinline__ void store_rec(R* q, const R (&m)[9], const V3<R>& r0, const V3<R>& r1) {
+  const R v[16] = {m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, R(0)};
+  if constexpr (sizeof(R) == 4) {
+    float4* p = reinterpret_cast<float4*>(q);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) p[i] = float4{v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]};
+  } else {
+    double2* p = reinterpret_cast<double2*>(q);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) p[i] = double2{v[2 * i], v[2 * i + 1]};
+  }
+}
+
 // calculateInternals, contact.go:59-85, for contact g.
-template <typename R, int KIND>
+// With REC, also the contact's record (Contacts::rec).
+template <typename R, int KIND, bool REC = false>
 __device__ __forceinline__ void prepare_contact(const RS<R>& a, int64_t g) {
   const Bodies<R>& b = a.b;
   const Contacts<R>& c = a.c;
@@ -241,6 +299,7 @@ __device__ __forceinline__ void prepare_contact(const RS<R>& a, int64_t g) {
   st3(c.rel + 3 * CS, CS, g, r1);
   st3(c.cvel, CS, g, cv);
   c.ddv[g] = ddv;
+  if constexpr (REC) store_rec(c.rec + 16 * g, m, r0, r1);
 }
 
 // matchAwakeState, contact.go:185-202.  Returns the awake states after the
@@ -849,62 +908,6 @@ __device__ __forceinline__ V3<R> sel3(bool c, const V3<R>& a, const V3<R>& b) {
   return V3<R>{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z};
 }
 
-// A contact's record (Contacts::rec): 16 reals from one 64-byte (FP32) or
-// 128-byte (FP64) aligned address, in 16-byte loads.
-template <typename R>
-__device__ __forceinline__ void load_rec(const R* q, R (&v)[16]) {
-  if constexpr (sizeof(R) == 4) {
-    const float4* p = reinterpret_cast<const float4*>(q);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float4 x = p[i];
-      v[4 * i] = x.x;
-      v[4 * i + 1] = x.y;
-      v[4 * i + 2] = x.z;
-      v[4 * i + 3] = x.w;
-    }
-  } else {
-    const double2* p = reinterpret_cast<const double2*>(q);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const double2 x = p[i];
-      v[2 * i] = x.x;
-      v[2 * i + 1] = x.y;
-    }
-  }
-}
-// contactToWorld and both relative positions
-template <typename R>
-__device__ __forceinline__ void rec_frame(const R* q, R (&m)[9], V3<R> (&r)[2]) {
-  R v[16];
-  load_rec(q, v);
-#pragma unroll
-  for (int k = 0; k < 9; ++k) m[k] = v[k];
-  r[0] = V3<R>{v[9], v[10], v[11]};
-  r[1] = V3<R>{v[12], v[13], v[14]};
-}
-// the normal (contactToWorld's first column) and both relative positions
-template <typename R>
-__device__ __forceinline__ void rec_normal(const R* q, V3<R>& n, V3<R> (&r)[2]) {
-  R v[16];
-  load_rec(q, v);
-  n = V3<R>{v[0], v[1], v[2]};
-  r[0] = V3<R>{v[9], v[10], v[11]};
-  r[1] = V3<R>{v[12], v[13], v[14]};
-}
-template <typename R>
-__device__ __forceinline__ void store_rec(R* q, const R (&m)[9], const V3<R>& r0, const V3<R>& r1) {
-  const R v[16] = {m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, R(0)};
-  if constexpr (sizeof(R) == 4) {
-    float4* p = reinterpret_cast<float4*>(q);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) p[i] = float4{v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]};
-  } else {
-    double2* p = reinterpret_cast<double2*>(q);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) p[i] = double2{v[2 * i], v[2 * i + 1]};
-  }
-}
 
 template <typename R, int SLOTS, int KIND>
 struct RegLds {

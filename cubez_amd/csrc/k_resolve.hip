@@ -102,6 +102,24 @@ __global__ void __launch_bounds__(kSelectThreads)
   }
 }
 
+// prepareContacts alone (a position launch without iterations): one
+// workgroup per world, a thread per contact, from the bodies' state in HBM
+// (the arithmetic of the worlds' in-kernel prepare), with every contact's
+// record for the one-wavefront kernel's wide classes.  A world kernel
+// spends this launch in a lane's serial walk over its contacts.
+template <typename R, int KIND>
+__global__ void __launch_bounds__(256) k_prepare(RS<R> a) {
+  const int w = blockIdx.x;
+  const int C = a.world_cnt[w];
+  if (a.overflow[0] != 0 || C <= 0) return;  // the resolver's guard
+  // the bodies' positions at prepare time, which the prepared launches read
+  const int b0w = a.body_off[w], n = a.body_off[w + 1] - b0w;
+  const int64_t S = a.b.stride;
+  for (int i = threadIdx.x; i < n; i += 256) st3(a.pos0, S, b0w + i, ld3(a.b.pos, S, b0w + i));
+  const int64_t base = a.world_coff[w];
+  for (int j = threadIdx.x; j < C; j += 256) prepare_contact<R, KIND, true>(a, base + j);
+}
+
 // With the multi-wave kernel: the launch's stream forks to `side`, which
 // selects the worlds and runs the multi-wave kernel; the one-wave kernel
 // starts on the launch's stream once the selection is done (so that the
@@ -172,6 +190,13 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   // per-contact materials: the plain resolver only
   if (a.c.fric && (!a.c.rest || (a.mode & (RM_POLISH | RM_LABEL)))) return hipErrorInvalidValue;
   if (a.n_worlds <= 0) return hipSuccess;
+  if (a.iter_mult == 0 && a.iter_fixed == 0 && (a.mode & ~RM_FRICTIONLESS2) == RM_POS) {
+    if (a.c.fric)
+      k_prepare<R, KIND_MAT><<<a.n_worlds, 256, 0, st>>>(a);
+    else
+      k_prepare<R, KIND_PLAIN><<<a.n_worlds, 256, 0, st>>>(a);
+    return hipGetLastError();
+  }
   if (a.c.fric) return launch_both<R, KIND_MAT>(a, L, st);
   if (a.mode & RM_POLISH)
     return (a.mode & RM_POS) ? launch_both<R, KIND_PPOLISH>(a, L, st) : launch_both<R, KIND_POLISH>(a, L, st);
