@@ -1473,9 +1473,15 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
       const int64_t g = base + j;
       const int jb[2] = {c.body0[g], c.body1[g]};
       R mj[9], imp[kImpPlanes];
+      V3<R> rj[2];
+      if constexpr (WIDE) {
+        rec_frame(c.rec + 16 * g, mj, rj);
+      } else {
 #pragma unroll
-      for (int k = 0; k < 9; ++k) mj[k] = c.c2w[k * CS + g];
-      const V3<R> rj[2] = {ld3(c.rel, CS, g), ld3(c.rel + 3 * CS, CS, g)};
+        for (int k = 0; k < 9; ++k) mj[k] = c.c2w[k * CS + g];
+        rj[0] = ld3(c.rel, CS, g);
+        rj[1] = ld3(c.rel + 3 * CS, CS, g);
+      }
       impulse_data<R, KIND>(a, jb, rj, mj, friction_of<R, KIND>(c, g), imp);
 #pragma unroll
       for (int k = 0; k < kImpPlanes; ++k) c.imp[k * CS + g] = imp[k];
@@ -1528,9 +1534,9 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
           const int64_t gj = base + 64 * s + lane;
           const int jb[2] = {id_lo(ids[s]), id_hi(ids[s])};
           R mj[9];
+          V3<R> rw[2];
           if constexpr (WIDE) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) mj[k] = c.c2w[k * CS + gj];
+            rec_frame(c.rec + 16 * gj, mj, rw);
           } else {
             basis_from_scale(nrm[s], s_bs[64 * s + lane], mj);
           }
@@ -1546,7 +1552,7 @@ __device__ __forceinline__ void resolve_reg(const RS<R>& a, int w, int C, int b0
             moved = true;
             V3<R> rj;
             if constexpr (WIDE)
-              rj = ld3(c.rel + 3 * bb * CS, CS, gj);
+              rj = rw[bb];
             else
               rj = sub(ptr[s], lds3(s_pos0, jb[bb]));
             R sign = R(1);
