@@ -83,7 +83,6 @@ struct cbz_context {
   // least mw_min contacts run on stream `side` beside the one-wave kernel
   hipStream_t side = nullptr;
   hipEvent_t mw_fork = nullptr, mw_sel = nullptr, mw_join = nullptr;
-  hipEvent_t mw_presel = nullptr;  // after the surrogate step's two selections on `side`
   int32_t mw_min = kDefaultMwMin;
   int32_t mw_max = kDefaultMwMax;
   int32_t n_cu = 256;
@@ -181,11 +180,6 @@ struct cbz_context {
   int32_t* mw_list = nullptr;     // [nw] worlds of the multi-wave kernel (resolver launch)
   int32_t* mw_count = nullptr;    // [2] list length, work-queue head
   uint8_t* mw_flag = nullptr;     // [nw] listed
-  // a second list, for the surrogate step's velocity launch, whose selection
-  // runs ahead with the position launch's
-  int32_t* mw_list2 = nullptr;
-  int32_t* mw_count2 = nullptr;
-  uint8_t* mw_flag2 = nullptr;
   int32_t* order = nullptr;
   bool have_contacts = false;
 
@@ -460,9 +454,6 @@ void free_scene(cbz_ctx* c) {
   dfree(c->mw_list);
   dfree(c->mw_count);
   dfree(c->mw_flag);
-  dfree(c->mw_list2);
-  dfree(c->mw_count2);
-  dfree(c->mw_flag2);
   dfree(c->order);
   dfree(c->d_awake);
   dfree(c->d_prof);
@@ -655,11 +646,8 @@ int do_narrowphase(cbz_ctx* c, int32_t list_tiles, hipEvent_t* ev = nullptr) {
 }
 
 template <typename R>
-// presel: 0, the launch selects its multi-wave worlds; 1 or 2, it takes the
-// list that presel_lists built ahead (1: the first list, 2: the second).
 int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode = 3,
-               uint8_t* mask = nullptr, float* labels = nullptr, double polish_tol = 0, int64_t polish_mult = 0,
-               int presel = 0) {
+               uint8_t* mask = nullptr, float* labels = nullptr, double polish_tol = 0, int64_t polish_mult = 0) {
   ResolveLaunch<R> L;
   L.mode = mode;
   if (c->mat_flags & CBZ_MATERIALS_FRICTIONLESS_TWO_BODY) L.mode |= RM_FRICTIONLESS2;
@@ -696,11 +684,9 @@ int do_resolve(cbz_ctx* c, double dt, int64_t mult, int64_t fixed, int32_t mode 
     L.mw_max = 0;
     L.world_iters = nullptr;
   }
-  L.mw_list = presel == 2 ? c->mw_list2 : c->mw_list;
-  L.mw_count = presel == 2 ? c->mw_count2 : c->mw_count;
-  L.mw_flag = presel == 2 ? c->mw_flag2 : c->mw_flag;
-  L.preselected = presel != 0;
-  L.presel = c->mw_presel;
+  L.mw_list = c->mw_list;
+  L.mw_count = c->mw_count;
+  L.mw_flag = c->mw_flag;
   L.side = c->side;
   L.fork = c->mw_fork;
   L.sel = c->mw_sel;
@@ -1300,20 +1286,6 @@ int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int64_t polish_
   u.last_pos = pos;
   if (c->prof) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   if ((rc = do_resolve<R>(c, dt, 0, 0, RM_POS))) return rc;
-  // Both resolver launches' multi-wave lists, built now on the side stream
-  // while the features and the network run: their inputs (contact counts,
-  // the last step's iterations of each phase) are final here, and neither
-  // launch writes what the other's selection reads.
-  const int presel = (c->mw_min > 0 && c->mw_max > 0) ? 1 : 0;
-  if (presel) {
-    HIPCHK(c, hipEventRecord(c->mw_fork, c->stream));
-    HIPCHK(c, hipStreamWaitEvent(c->side, c->mw_fork, 0));
-    HIPCHK(c, launch_mw_select(c->order, c->world_cnt, c->d_body_off, c->world_iters, RM_POS, c->nw, c->mw_min,
-                               c->mw_max, c->mw_list, c->mw_count, c->mw_flag, c->side));
-    HIPCHK(c, launch_mw_select(c->order, c->world_cnt, c->d_body_off, c->world_iters, RM_VEL, c->nw, c->mw_min,
-                               c->mw_max, c->mw_list2, c->mw_count2, c->mw_flag2, c->side));
-    HIPCHK(c, hipEventRecord(c->mw_presel, c->side));
-  }
   if (c->prof) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   if (tol < 0 || (flags & CBZ_SUR_ZERO_PREDICTION)) {
     // every world plain (tol < 0), or polished from zero predictions
@@ -1358,12 +1330,11 @@ int sur_resolve(cbz_ctx* c, double dt, int64_t mult, double tol, int64_t polish_
     HIPCHK(c, launch_sur_scatter<R>(a, c->nw, c->stream));
   }
   if (c->prof) HIPCHK(c, hipEventRecord(ev[4], c->stream));
-  rc = pos ? do_resolve<R>(c, dt, mult, 0, RM_POS | RM_PREPARED | RM_POLISH, u.pmask, nullptr, u.pos_tol, u.pos_polish,
-                           presel)
-           : do_resolve<R>(c, dt, mult, 0, RM_POS | RM_PREPARED, nullptr, nullptr, 0, 0, presel);
+  rc = pos ? do_resolve<R>(c, dt, mult, 0, RM_POS | RM_PREPARED | RM_POLISH, u.pmask, nullptr, u.pos_tol, u.pos_polish)
+           : do_resolve<R>(c, dt, mult, 0, RM_POS | RM_PREPARED);
   if (rc) return rc;
   if (c->prof) HIPCHK(c, hipEventRecord(ev[5], c->stream));
-  rc = do_resolve<R>(c, dt, mult, 0, RM_VEL | RM_POLISH, u.mask, nullptr, tol, polish_mult, presel ? 2 : 0);
+  rc = do_resolve<R>(c, dt, mult, 0, RM_VEL | RM_POLISH, u.mask, nullptr, tol, polish_mult);
   if (rc) return rc;
   if (nfall_dst) HIPCHK(c, launch_count_u8(u.mask, c->nw, nfall_dst, c->stream, pos ? u.pmask : nullptr));
   return c->prof ? sur_profile(c, pos) : CBZ_OK;
@@ -1406,7 +1377,6 @@ int cbz_create(const cbz_config* cfg, cbz_ctx** out) {
        hipEventCreateWithFlags(&c->mw_fork, hipEventDisableTiming) == hipSuccess &&
        hipEventCreateWithFlags(&c->mw_sel, hipEventDisableTiming) == hipSuccess &&
        hipEventCreateWithFlags(&c->mw_join, hipEventDisableTiming) == hipSuccess &&
-       hipEventCreateWithFlags(&c->mw_presel, hipEventDisableTiming) == hipSuccess &&
        hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess;
   ok = ok && scratch_address(c->device, &c->scratch) == hipSuccess;
   if (!ok) {
@@ -1431,7 +1401,6 @@ int cbz_destroy(cbz_ctx* c) {
   if (c->mw_fork) (void)hipEventDestroy(c->mw_fork);
   if (c->mw_sel) (void)hipEventDestroy(c->mw_sel);
   if (c->mw_join) (void)hipEventDestroy(c->mw_join);
-  if (c->mw_presel) (void)hipEventDestroy(c->mw_presel);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1557,9 +1526,6 @@ int cbz_set_scene(cbz_ctx* c, int32_t n_worlds, const int32_t* body_off, const i
   if ((rc = dalloc(c, &c->mw_list, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->mw_count, 2))) return rc;
   if ((rc = dalloc(c, &c->mw_flag, (size_t)n_worlds))) return rc;
-  if ((rc = dalloc(c, &c->mw_list2, (size_t)n_worlds))) return rc;
-  if ((rc = dalloc(c, &c->mw_count2, 2))) return rc;
-  if ((rc = dalloc(c, &c->mw_flag2, (size_t)n_worlds))) return rc;
   if ((rc = dalloc(c, &c->d_awake, kAwakeSlots))) return rc;
   if ((rc = dalloc(c, &c->d_prof, 4))) return rc;
   if ((rc = dalloc_bytes(c, &c->pos0, rs * 3 * (size_t)std::max<int64_t>(nb, 1)))) return rc;

@@ -80,10 +80,6 @@ struct ResolveLaunch {
   uint8_t* mw_flag = nullptr;
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, sel = nullptr, join = nullptr;
-  // the list was built ahead on `side` (launch_mw_select), which recorded
-  // `presel` after it: the launch runs no selection and waits for presel
-  bool preselected = false;
-  hipEvent_t presel = nullptr;
   int32_t n_cu = 256;
   int32_t n_worlds;
   R dt;
@@ -108,11 +104,6 @@ template <typename R>
 hipError_t launch_narrowphase(const NarrowLaunch<R>& L, hipStream_t st, hipEvent_t* ev = nullptr);
 template <typename R>
 hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st);
-// The multi-wave kernel's work list for a later launch of `mode` (RM_POS or
-// RM_VEL costs), built on st.
-hipError_t launch_mw_select(const int32_t* order, const int32_t* world_cnt, const int32_t* body_off,
-                            const int64_t* world_iters, int32_t mode, int32_t n_worlds, int32_t mw_min, int32_t mw_max,
-                            int32_t* mw_list, int32_t* mw_count, uint8_t* mw_flag, hipStream_t st);
 hipError_t resolver_occupancy(bool f64, int* worlds_per_cu);
 
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);

@@ -132,34 +132,21 @@ hipError_t launch_both(const RS<R>& a, const ResolveLaunch<R>& L, hipStream_t st
   if (mw) {
     if ((e = hipEventRecord(L.fork, st)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(L.side, L.fork, 0)) != hipSuccess) return e;
-    if (!L.preselected) {
-      k_mw_select<<<1, kSelectThreads, 0, L.side>>>(a.order, a.world_cnt, a.body_off, a.world_iters, a.mode,
-                                                     a.n_worlds, L.mw_min, L.mw_max, L.mw_list, L.mw_count, L.mw_flag);
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
+    k_mw_select<<<1, kSelectThreads, 0, L.side>>>(a.order, a.world_cnt, a.body_off, a.world_iters, a.mode, a.n_worlds,
+                                                   L.mw_min, L.mw_max, L.mw_list, L.mw_count, L.mw_flag);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipEventRecord(L.sel, L.side)) != hipSuccess) return e;
     int grid = L.mw_max < L.n_cu ? L.mw_max : L.n_cu;
     if (grid > a.n_worlds) grid = a.n_worlds;
     launch_mw_kind<R, KIND>(a, grid, L.side);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipEventRecord(L.join, L.side)) != hipSuccess) return e;
-    // the one-wavefront kernel reads the flags: it waits for the selection
-    // (made ahead on `side` when preselected, whose event is then presel)
-    if ((e = hipStreamWaitEvent(st, L.preselected ? L.presel : L.sel, 0)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(st, L.sel, 0)) != hipSuccess) return e;
   }
   launch_kind<R, KIND>(a, st);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (mw) return hipStreamWaitEvent(st, L.join, 0);
   return hipSuccess;
-}
-
-hipError_t launch_mw_select(const int32_t* order, const int32_t* world_cnt, const int32_t* body_off,
-                            const int64_t* world_iters, int32_t mode, int32_t n_worlds, int32_t mw_min, int32_t mw_max,
-                            int32_t* mw_list, int32_t* mw_count, uint8_t* mw_flag, hipStream_t st) {
-  if (n_worlds <= 0) return hipSuccess;
-  k_mw_select<<<1, kSelectThreads, 0, st>>>(order, world_cnt, body_off, world_iters, mode, n_worlds, mw_min, mw_max,
-                                             mw_list, mw_count, mw_flag);
-  return hipGetLastError();
 }
 
 template <typename R>
@@ -175,7 +162,7 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   a.world_iters = L.world_iters;
   a.world_time = L.world_time;
   const bool mw = L.mw_min > 0 && L.mw_max > 0 && L.mw_list && L.mw_count && L.mw_flag && L.side && L.fork && L.sel &&
-                  L.join && (!L.preselected || L.presel);
+                  L.join;
   a.mw_list = mw ? L.mw_list : nullptr;
   a.mw_count = mw ? L.mw_count : nullptr;
   a.mw_flag = mw ? L.mw_flag : nullptr;
