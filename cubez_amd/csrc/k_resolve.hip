@@ -191,6 +191,7 @@ hipError_t launch_resolve(const ResolveLaunch<R>& L, hipStream_t st) {
   if (a.c.fric && (!a.c.rest || (a.mode & (RM_POLISH | RM_LABEL)))) return hipErrorInvalidValue;
   if (a.n_worlds <= 0) return hipSuccess;
   if (a.iter_mult == 0 && a.iter_fixed == 0 && (a.mode & ~RM_FRICTIONLESS2) == RM_POS) {
+    if (!a.pos0 || !a.c.rec) return hipErrorInvalidValue;  // k_prepare writes both
     if (a.c.fric)
       k_prepare<R, KIND_MAT><<<a.n_worlds, 256, 0, st>>>(a);
     else
